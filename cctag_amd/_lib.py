@@ -1,0 +1,157 @@
+"""ctypes bindings of libcctag_amd.so (include/cctag_amd.h).
+
+The library is loaded from this package directory only; a missing library is
+an error, never a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcctag_amd.so")
+
+OK = 0
+ERRORS = {
+    -1: "bad argument",
+    -2: "HIP runtime error",
+    -3: "capacity exceeded",
+    -4: "call order violated",
+    -5: "no gfx950 device",
+    -6: "unsupported size or parameter",
+}
+
+c_i32 = ctypes.c_int32
+c_f32 = ctypes.c_float
+P = ctypes.POINTER
+
+
+class Params(ctypes.Structure):
+    """cctagx_params, field for field (Params.hpp:108-199)."""
+    _fields_ = [
+        ("canny_thr_low", c_f32),
+        ("canny_thr_high", c_f32),
+        ("dist_search", c_i32),
+        ("thr_gradient_mag_in_vote", c_i32),
+        ("angle_voting", c_f32),
+        ("ratio_voting", c_f32),
+        ("average_vote_min", c_f32),
+        ("thr_median_distance_ellipse", c_f32),
+        ("maximum_nb_seeds", c_i32),
+        ("maximum_nb_candidates_loop_two", c_i32),
+        ("n_crowns", c_i32),
+        ("min_points_segment_candidate", c_i32),
+        ("min_votes_to_select_candidate", c_i32),
+        ("thresh_robust_estimation_of_outer_ellipse", c_f32),
+        ("ellipse_growing_elliptic_hull_width", c_f32),
+        ("window_size_on_inner_elliptic_segment", c_i32),
+        ("number_of_multires_layers", c_i32),
+        ("number_of_processed_multires_layers", c_i32),
+        ("n_samples_outer_ellipse", c_i32),
+        ("num_cuts_in_ident_step", c_i32),
+        ("num_samples_outer_edge_points_refinement", c_i32),
+        ("sample_cut_length", c_i32),
+        ("imaged_center_n_grid_sample", c_i32),
+        ("imaged_center_neighbour_size", c_f32),
+        ("min_ident_proba", c_f32),
+        ("search_for_another_segment", c_i32),
+        ("do_identification", c_i32),
+    ]
+
+
+class LevelView(ctypes.Structure):
+    """cctagx_level_view."""
+    _fields_ = [
+        ("width", c_i32), ("height", c_i32), ("shape_w", c_i32), ("shape_h", c_i32),
+        ("img", P(ctypes.c_uint8)), ("dx", P(ctypes.c_int16)), ("dy", P(ctypes.c_int16)),
+        ("edges", P(ctypes.c_uint8)), ("edge_map", P(c_i32)),
+        ("n_edges", c_i32),
+        ("edge_x", P(ctypes.c_int16)), ("edge_y", P(ctypes.c_int16)),
+        ("edge_dx", P(ctypes.c_int16)), ("edge_dy", P(ctypes.c_int16)),
+        ("before", P(c_i32)), ("after", P(c_i32)),
+        ("chosen", P(c_i32)), ("total_dist", P(c_f32)),
+        ("voters_index", P(c_i32)), ("voters_list", P(c_i32)),
+        ("flow_length", P(c_f32)), ("is_max", P(c_i32)),
+        ("n_seeds", c_i32), ("seeds", P(c_i32)),
+    ]
+
+
+def _arr(p, n, dtype):
+    if n == 0 or not p:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+
+
+def level_view_to_dict(v: LevelView) -> dict:
+    """Copies a level view into numpy arrays."""
+    w, h, n = v.width, v.height, v.n_edges
+    vi = _arr(v.voters_index, n + 1, np.int32)
+    nv = int(vi[-1]) if n + 1 > 0 and len(vi) else 0
+    return {
+        "width": w, "height": h, "shape": (v.shape_w, v.shape_h),
+        "img": _arr(v.img, w * h, np.uint8).reshape(h, w),
+        "dx": _arr(v.dx, w * h, np.int16).reshape(h, w),
+        "dy": _arr(v.dy, w * h, np.int16).reshape(h, w),
+        "edges": _arr(v.edges, w * h, np.uint8).reshape(h, w),
+        "edge_map": _arr(v.edge_map, w * h, np.int32).reshape(h, w),
+        "n_edges": n,
+        "edge_x": _arr(v.edge_x, n, np.int16), "edge_y": _arr(v.edge_y, n, np.int16),
+        "edge_dx": _arr(v.edge_dx, n, np.int16), "edge_dy": _arr(v.edge_dy, n, np.int16),
+        "before": _arr(v.before, n, np.int32), "after": _arr(v.after, n, np.int32),
+        "chosen": _arr(v.chosen, n, np.int32), "total_dist": _arr(v.total_dist, n, np.float32),
+        "voters_index": vi, "voters_list": _arr(v.voters_list, nv, np.int32),
+        "flow_length": _arr(v.flow_length, n, np.float32), "is_max": _arr(v.is_max, n, np.int32),
+        "seeds": _arr(v.seeds, v.n_seeds, np.int32),
+    }
+
+
+class CCTagError(RuntimeError):
+    def __init__(self, rc: int, what: str, detail: str = ""):
+        msg = "%s failed: %s (%d)" % (what, ERRORS.get(rc, "error"), rc)
+        if detail:
+            msg += ": " + detail
+        super().__init__(msg)
+        self.rc = rc
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            "libcctag_amd.so is not built (%s); run python -m cctag_amd.build" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.cctagx_params_default.argtypes = [P(Params), ctypes.c_int]
+    lib.cctagx_params_default.restype = None
+    lib.cctagx_create.argtypes = [P(Params), ctypes.c_int, ctypes.c_int, P(vp)]
+    lib.cctagx_destroy.argtypes = [vp]
+    lib.cctagx_front_end.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.cctagx_get_level.argtypes = [vp, ctypes.c_int, P(LevelView)]
+    lib.cctagx_stage_count.argtypes = [vp]
+    lib.cctagx_stage_time.argtypes = [vp, ctypes.c_int, P(ctypes.c_char_p), P(c_f32)]
+    lib.cctagx_time_front_end.argtypes = [vp, ctypes.c_int, P(c_f32)]
+    lib.cctagx_version.restype = ctypes.c_char_p
+    lib.cctagx_last_error.restype = ctypes.c_char_p
+    lib.cctagx_device_count.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != OK:
+        detail = load().cctagx_last_error().decode(errors="replace")
+        raise CCTagError(rc, what, detail)
+
+
+def default_params(n_crowns: int = 3) -> Params:
+    p = Params()
+    load().cctagx_params_default(ctypes.byref(p), n_crowns)
+    return p

@@ -1,0 +1,252 @@
+// C ABI of the library (include/cctag_amd.h).
+#include "../../include/cctag_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "device.hpp"
+#include "frontend_host.hpp"
+
+using namespace cctagx;
+
+namespace {
+
+// Host copy of one level, backing a cctagx_level_view.
+struct HostLevel {
+  int w = 0, h = 0, W0 = 0, H0 = 0;
+  std::vector<uint8_t> img, edges;
+  std::vector<int16_t> dx, dy, ex, ey, egx, egy;
+  std::vector<int32_t> emap, before, after, chosen, vindex, vlist, ismax, seeds;
+  std::vector<float> tdist, flow;
+  int n_edges = 0, n_seeds = 0;
+};
+
+template <class T>
+int d2h(std::vector<T>& dst, const T* src, size_t n, hipStream_t s) {
+  dst.resize(n);
+  if (n == 0) return 0;
+  CCTAGX_HIP(hipMemcpyAsync(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+template <class T>
+const T* ptr(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
+
+}  // namespace
+
+struct cctagx_handle {
+  cctagx_params params{};
+  int width = 0, height = 0;
+  hipStream_t stream = nullptr;
+  FrontEndDevice fe;
+  StageTimer timer;
+  bool have_front_end = false;
+  std::vector<HostLevel> levels;
+  std::vector<bool> level_loaded;
+};
+
+static int validate(const cctagx_params* p) {
+  if (!p) return CCTAGX_ERR_ARG;
+  if (p->n_crowns != 3 && p->n_crowns != 4) return CCTAGX_ERR_UNSUPPORTED;
+  if (p->angle_voting != 0.0f) return CCTAGX_ERR_ARG;  // Vote.cpp:95-98 throws
+  if (p->number_of_processed_multires_layers < 1 || p->number_of_processed_multires_layers > 4)
+    return CCTAGX_ERR_UNSUPPORTED;
+  if (p->dist_search < 0 || p->min_votes_to_select_candidate < 1) return CCTAGX_ERR_ARG;
+  return CCTAGX_OK;
+}
+
+static FrontEndParams front_end_params(const cctagx_params& p) {
+  // Level.cpp:206-207 passes thr * 256; cvRecode.cpp:67,107-108 orders and floors them.
+  float lo = p.canny_thr_low * 256, hi = p.canny_thr_high * 256;
+  if (lo > hi) std::swap(lo, hi);
+  FrontEndParams f{};
+  f.low = (int)std::floor(lo);
+  f.high = (int)std::floor(hi);
+  f.dist_search = p.dist_search;
+  f.thr_grad = p.thr_gradient_mag_in_vote;
+  f.ratio_voting = p.ratio_voting;
+  f.angle_voting = p.angle_voting;
+  f.n_crowns = p.n_crowns;
+  f.min_votes = p.min_votes_to_select_candidate;
+  return f;
+}
+
+extern "C" {
+
+const char* cctagx_version(void) { return "cctag_amd 0.1 (gfx950)"; }
+
+const char* cctagx_last_error(void) { return last_error(); }
+
+int cctagx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void cctagx_params_default(cctagx_params* p, int n_crowns) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->canny_thr_low = 0.01f;
+  p->canny_thr_high = 0.04f;
+  p->dist_search = 30;
+  p->thr_gradient_mag_in_vote = 2500;
+  p->angle_voting = 0.0f;
+  p->ratio_voting = 4.0f;
+  p->average_vote_min = 0.0f;
+  p->thr_median_distance_ellipse = 3.0f;
+  p->maximum_nb_seeds = 500;
+  p->maximum_nb_candidates_loop_two = 40;
+  p->n_crowns = n_crowns;
+  p->min_points_segment_candidate = 10;
+  p->min_votes_to_select_candidate = 3;
+  p->thresh_robust_estimation_of_outer_ellipse = 30.0f;
+  p->ellipse_growing_elliptic_hull_width = 2.3f;
+  p->window_size_on_inner_elliptic_segment = 20;
+  p->number_of_multires_layers = 4;
+  p->number_of_processed_multires_layers = 4;
+  p->n_samples_outer_ellipse = 150;
+  p->num_cuts_in_ident_step = 22;
+  p->num_samples_outer_edge_points_refinement = 20;
+  p->sample_cut_length = 100;
+  p->imaged_center_n_grid_sample = 5;
+  p->imaged_center_neighbour_size = 0.20f;
+  p->min_ident_proba = 1e-6f;
+  p->search_for_another_segment = 1;
+  p->do_identification = 1;
+}
+
+int cctagx_create(const cctagx_params* params, int width, int height, cctagx_handle** out) {
+  if (!out || width <= 0 || height <= 0) return CCTAGX_ERR_ARG;
+  *out = nullptr;
+  if (int rc = validate(params)) return rc;
+  if (cctagx_device_count() <= 0) return CCTAGX_ERR_NO_DEVICE;
+  std::unique_ptr<cctagx_handle> h(new (std::nothrow) cctagx_handle());
+  if (!h) return CCTAGX_ERR_ARG;
+  h->params = *params;
+  h->width = width;
+  h->height = height;
+  PlaneSet ps;
+  if (!plan_planes({{width, height}}, params->number_of_processed_multires_layers, ps)) {
+    set_error("frame size must halve exactly at every pyramid level");
+    return CCTAGX_ERR_UNSUPPORTED;
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return CCTAGX_ERR_HIP;
+  if (int rc = h->fe.init(ps, front_end_params(*params))) return rc;
+  h->levels.resize(ps.planes.size());
+  h->level_loaded.assign(ps.planes.size(), false);
+  *out = h.release();
+  return CCTAGX_OK;
+}
+
+int cctagx_destroy(cctagx_handle* h) {
+  if (!h) return CCTAGX_ERR_ARG;
+  if (h->stream) {
+    hipStreamSynchronize(h->stream);
+    hipStreamDestroy(h->stream);
+  }
+  delete h;
+  return CCTAGX_OK;
+}
+
+int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, size_t stride) {
+  if (!h || !gray || stride < (size_t)h->width) return CCTAGX_ERR_ARG;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  h->timer.reset();
+  if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
+  CCTAGX_HIP(hipStreamSynchronize(h->stream));
+  h->have_front_end = true;
+  h->level_loaded.assign(h->level_loaded.size(), false);
+  return CCTAGX_OK;
+}
+
+int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* v) {
+  if (!h || !v) return CCTAGX_ERR_ARG;
+  if (!h->have_front_end) return CCTAGX_ERR_STATE;
+  const PlaneSet& ps = h->fe.planes();
+  if (level < 0 || level >= (int)ps.planes.size()) return CCTAGX_ERR_ARG;
+  const PlaneDesc& d = ps.planes[level];
+  HostLevel& L = h->levels[level];
+  if (!h->level_loaded[level]) {
+    const FrontEndBuffers& b = h->fe.buffers();
+    hipStream_t s = h->stream;
+    PlaneCounts c{};
+    CCTAGX_HIP(hipMemcpy(&c, b.counts + level, sizeof(c), hipMemcpyDeviceToHost));
+    const size_t N = (size_t)d.w * d.h, n = c.n_edges;
+    L.w = d.w; L.h = d.h; L.W0 = d.shape_w; L.H0 = d.shape_h;
+    L.n_edges = c.n_edges;
+    L.n_seeds = c.n_seeds;
+    int rc = 0;
+    rc |= d2h(L.img, b.img + d.pix, N, s);
+    rc |= d2h(L.dx, b.dx + d.pix, N, s);
+    rc |= d2h(L.dy, b.dy + d.pix, N, s);
+    rc |= d2h(L.edges, b.edges + d.pix, N, s);
+    rc |= d2h(L.emap, b.emap + d.pix, N, s);
+    rc |= d2h(L.ex, b.ex + d.pix, n, s);
+    rc |= d2h(L.ey, b.ey + d.pix, n, s);
+    rc |= d2h(L.egx, b.egx + d.pix, n, s);
+    rc |= d2h(L.egy, b.egy + d.pix, n, s);
+    rc |= d2h(L.before, b.before + d.pix, n, s);
+    rc |= d2h(L.after, b.after + d.pix, n, s);
+    rc |= d2h(L.chosen, b.chosen + d.pix, n, s);
+    rc |= d2h(L.tdist, b.tdist + d.pix, n, s);
+    rc |= d2h(L.vindex, b.vindex + d.pix + level, n + 1, s);
+    rc |= d2h(L.vlist, b.vlist + d.pix, (size_t)c.n_voters, s);
+    rc |= d2h(L.flow, b.flow + d.pix, n, s);
+    rc |= d2h(L.ismax, b.ismax + d.pix, n, s);
+    rc |= d2h(L.seeds, b.seeds + d.pix, (size_t)c.n_seeds, s);
+    if (rc) return CCTAGX_ERR_HIP;
+    CCTAGX_HIP(hipStreamSynchronize(s));
+    h->level_loaded[level] = true;
+  }
+  v->width = L.w; v->height = L.h; v->shape_w = L.W0; v->shape_h = L.H0;
+  v->img = ptr(L.img); v->dx = ptr(L.dx); v->dy = ptr(L.dy); v->edges = ptr(L.edges);
+  v->edge_map = ptr(L.emap);
+  v->n_edges = L.n_edges;
+  v->edge_x = ptr(L.ex); v->edge_y = ptr(L.ey); v->edge_dx = ptr(L.egx); v->edge_dy = ptr(L.egy);
+  v->before = ptr(L.before); v->after = ptr(L.after);
+  v->chosen = ptr(L.chosen); v->total_dist = ptr(L.tdist);
+  v->voters_index = ptr(L.vindex); v->voters_list = ptr(L.vlist);
+  v->flow_length = ptr(L.flow); v->is_max = ptr(L.ismax);
+  v->n_seeds = L.n_seeds; v->seeds = ptr(L.seeds);
+  return CCTAGX_OK;
+}
+
+int cctagx_stage_count(cctagx_handle* h) {
+  if (!h) return CCTAGX_ERR_ARG;
+  return h->timer.used > 0 ? (int)h->timer.used - 1 : 0;
+}
+
+int cctagx_stage_time(cctagx_handle* h, int i, const char** name, float* usec) {
+  if (!h || !name || !usec || i < 0 || i + 1 >= (int)h->timer.used) return CCTAGX_ERR_ARG;
+  float ms = 0.0f;
+  CCTAGX_HIP(hipEventElapsedTime(&ms, h->timer.events[i], h->timer.events[i + 1]));
+  *name = h->timer.names[i + 1];
+  *usec = ms * 1000.0f;
+  return CCTAGX_OK;
+}
+
+int cctagx_time_front_end(cctagx_handle* h, int iters, float* usec_per_frame) {
+  if (!h || iters <= 0 || !usec_per_frame) return CCTAGX_ERR_ARG;
+  if (!h->have_front_end) return CCTAGX_ERR_STATE;
+  hipEvent_t a, b;
+  CCTAGX_HIP(hipEventCreate(&a));
+  CCTAGX_HIP(hipEventCreate(&b));
+  hipEventRecord(a, h->stream);
+  for (int i = 0; i < iters; ++i)
+    if (int rc = h->fe.run(h->stream, nullptr)) return rc;
+  hipEventRecord(b, h->stream);
+  CCTAGX_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *usec_per_frame = ms * 1000.0f / iters;
+  h->level_loaded.assign(h->level_loaded.size(), false);
+  return CCTAGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Small HIP runtime helpers shared by the host code of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cctagx {
+
+// Last error text of the calling thread, for cctagx_last_error().
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define CCTAGX_HIP(call)                                                          \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::cctagx::set_error(std::string(#call) + ": " + hipGetErrorString(e_));     \
+      return -2;                                                                  \
+    }                                                                             \
+  } while (0)
+
+// One device allocation carved into 256-byte aligned sub-arrays.
+class DeviceArena {
+ public:
+  DeviceArena() = default;
+  DeviceArena(const DeviceArena&) = delete;
+  DeviceArena& operator=(const DeviceArena&) = delete;
+  ~DeviceArena() { release(); }
+
+  // Plan phase: reserve bytes, returns offset.
+  size_t reserve(size_t bytes) {
+    const size_t off = size_;
+    size_ += (bytes + 255) & ~size_t(255);
+    return off;
+  }
+  int allocate() {
+    release();
+    if (size_ == 0) return 0;
+    CCTAGX_HIP(hipMalloc(&base_, size_));
+    return 0;
+  }
+  void release() {
+    if (base_) hipFree(base_);
+    base_ = nullptr;
+  }
+  template <class T>
+  T* at(size_t off) const { return reinterpret_cast<T*>(static_cast<char*>(base_) + off); }
+  size_t size() const { return size_; }
+  void clear_plan() { size_ = 0; }
+
+ private:
+  void* base_ = nullptr;
+  size_t size_ = 0;
+};
+
+// Pinned host buffer.
+class PinnedBuffer {
+ public:
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  ~PinnedBuffer() { release(); }
+  int ensure(size_t bytes) {
+    if (bytes <= cap_) return 0;
+    release();
+    CCTAGX_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+    cap_ = bytes;
+    return 0;
+  }
+  void release() {
+    if (p_) hipHostFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p_); }
+  size_t capacity() const { return cap_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+}  // namespace cctagx

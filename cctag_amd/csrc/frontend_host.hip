@@ -1,0 +1,159 @@
+// Host side of the front end: plane planning and device buffer layout.
+#include "device.hpp"
+#include "frontend_launch.hpp"
+#include "frontend_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace cctagx {
+
+namespace {
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+inline int align256(int v) { return (v + 255) & ~255; }
+thread_local std::string g_error;
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_levels,
+                 PlaneSet& out) {
+  out = PlaneSet();
+  out.n_levels = n_levels;
+  int pix = 0, tp = 0, tf = 0, ts = 0, tc = 0;
+  for (size_t f = 0; f < frame_sizes.size(); ++f) {
+    const int W = frame_sizes[f].first, H = frame_sizes[f].second;
+    int lw = W, lh = H;
+    for (int l = 0; l < n_levels; ++l) {
+      if (l > 0) {
+        const int pw = lw, ph = lh;
+        lw = pw / 2;
+        lh = ph / 2;
+        if (pw != 2 * lw || ph != 2 * lh || lw < 1 || lh < 1) return false;
+      }
+      PlaneDesc d{};
+      d.w = lw;
+      d.h = lh;
+      d.shape_w = W;
+      d.shape_h = H;
+      d.pix = pix;
+      d.parent = l == 0 ? -1 : (int)out.planes.size() - 1;
+      d.level = l;
+      d.frame = (int)f;
+      d.tile_f = tf;
+      tf += cdiv(lw, kFilterTileW) * cdiv(lh, kFilterTileH);
+      d.tile_s = ts;
+      ts += cdiv(lw, kStencilTileW) * cdiv(lh, kStencilTileH);
+      d.tile_c = tc;
+      tc += cdiv(lw * lh, kChunk);
+      d.tile_p = tp;
+      if (l == 0 && n_levels > 1) tp += cdiv(lw, kPyrTile) * cdiv(lh, kPyrTile);
+      out.planes.push_back(d);
+      pix += align256(lw * lh);
+    }
+  }
+  out.total_pix = pix;
+  out.n_tiles_p = tp;
+  out.n_tiles_f = tf;
+  out.n_tiles_s = ts;
+  out.n_tiles_c = tc;
+  return true;
+}
+
+int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
+  ps_ = ps;
+  arena_.release();
+  arena_.clear_plan();
+  const size_t N = (size_t)ps.total_pix;
+  const size_t NP = ps.planes.size();
+  const size_t o_img = arena_.reserve(N);
+  const size_t o_dx = arena_.reserve(2 * N);
+  const size_t o_dy = arena_.reserve(2 * N);
+  const size_t o_cls = arena_.reserve(N);
+  const size_t o_lab = arena_.reserve(4 * N);
+  const size_t o_rs = arena_.reserve(N);
+  const size_t o_edges = arena_.reserve(N);
+  const size_t o_tmp = arena_.reserve(N);
+  const size_t o_emap = arena_.reserve(4 * N);
+  const size_t o_tcount = arena_.reserve(4 * (size_t)std::max(ps.n_tiles_c, 1));
+  const size_t o_ex = arena_.reserve(2 * N);
+  const size_t o_ey = arena_.reserve(2 * N);
+  const size_t o_egx = arena_.reserve(2 * N);
+  const size_t o_egy = arena_.reserve(2 * N);
+  const size_t o_before = arena_.reserve(4 * N);
+  const size_t o_after = arena_.reserve(4 * N);
+  const size_t o_chosen = arena_.reserve(4 * N);
+  const size_t o_tdist = arena_.reserve(4 * N);
+  const size_t o_vcount = arena_.reserve(4 * N);
+  const size_t o_vindex = arena_.reserve(4 * (N + NP));
+  const size_t o_vlist = arena_.reserve(4 * N);
+  const size_t o_flow = arena_.reserve(4 * N);
+  const size_t o_ismax = arena_.reserve(4 * N);
+  const size_t o_seedmark = arena_.reserve(4 * N);
+  const size_t o_seeds = arena_.reserve(4 * N);
+  const size_t o_counts = arena_.reserve(sizeof(PlaneCounts) * NP);
+  const size_t o_planes = arena_.reserve(sizeof(PlaneDesc) * NP);
+  if (int rc = arena_.allocate()) return rc;
+  FrontEndBuffers& b = launch_.buf;
+  b.img = arena_.at<uint8_t>(o_img);
+  b.dx = arena_.at<int16_t>(o_dx);
+  b.dy = arena_.at<int16_t>(o_dy);
+  b.cls = arena_.at<uint8_t>(o_cls);
+  b.lab = arena_.at<int32_t>(o_lab);
+  b.root_strong = arena_.at<uint8_t>(o_rs);
+  b.edges = arena_.at<uint8_t>(o_edges);
+  b.tmp = arena_.at<uint8_t>(o_tmp);
+  b.emap = arena_.at<int32_t>(o_emap);
+  b.tcount = arena_.at<int32_t>(o_tcount);
+  b.ex = arena_.at<int16_t>(o_ex);
+  b.ey = arena_.at<int16_t>(o_ey);
+  b.egx = arena_.at<int16_t>(o_egx);
+  b.egy = arena_.at<int16_t>(o_egy);
+  b.before = arena_.at<int32_t>(o_before);
+  b.after = arena_.at<int32_t>(o_after);
+  b.chosen = arena_.at<int32_t>(o_chosen);
+  b.tdist = arena_.at<float>(o_tdist);
+  b.vcount = arena_.at<int32_t>(o_vcount);
+  b.vindex = arena_.at<int32_t>(o_vindex);
+  b.vlist = arena_.at<int32_t>(o_vlist);
+  b.flow = arena_.at<float>(o_flow);
+  b.ismax = arena_.at<int32_t>(o_ismax);
+  b.seedmark = arena_.at<int32_t>(o_seedmark);
+  b.seeds = arena_.at<int32_t>(o_seeds);
+  b.counts = arena_.at<PlaneCounts>(o_counts);
+  PlaneDesc* dp = arena_.at<PlaneDesc>(o_planes);
+  CCTAGX_HIP(hipMemcpy(dp, ps.planes.data(), sizeof(PlaneDesc) * NP, hipMemcpyHostToDevice));
+  launch_.dev_planes = dp;
+  launch_.n_planes = (int)NP;
+  launch_.n_levels = ps.n_levels;
+  launch_.n_tiles_p = ps.n_tiles_p;
+  launch_.n_tiles_f = ps.n_tiles_f;
+  launch_.n_tiles_s = ps.n_tiles_s;
+  launch_.n_tiles_c = ps.n_tiles_c;
+  // Enough blocks per plane to cover a level-0 plane's edges with one
+  // lane each in a couple of passes; planes loop over their own edges.
+  launch_.edge_blocks = 512;
+  launch_.prm = prm;
+  return 0;
+}
+
+int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s) {
+  int p0 = -1;
+  for (size_t i = 0; i < ps_.planes.size(); ++i)
+    if (ps_.planes[i].frame == frame && ps_.planes[i].level == 0) {
+      p0 = (int)i;
+      break;
+    }
+  if (p0 < 0) return -1;
+  const PlaneDesc& d = ps_.planes[p0];
+  CCTAGX_HIP(hipMemcpy2DAsync(launch_.buf.img + d.pix, d.w, gray, stride, d.w, d.h,
+                              hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+int FrontEndDevice::run(hipStream_t s, StageTimer* timer) {
+  return launch_front_end(launch_, s, timer);
+}
+
+}  // namespace cctagx

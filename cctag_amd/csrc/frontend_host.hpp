@@ -1,0 +1,27 @@
+// Device state of the front end for one plane set.
+#pragma once
+
+#include "device.hpp"
+#include "frontend_launch.hpp"
+
+namespace cctagx {
+
+class FrontEndDevice {
+ public:
+  int init(const PlaneSet& ps, const FrontEndParams& prm);
+  // Copies a row-major 8-bit frame into the level-0 plane of `frame`.
+  int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s);
+  // Enqueues the whole front end (no host synchronization).
+  int run(hipStream_t s, StageTimer* timer);
+
+  const PlaneSet& planes() const { return ps_; }
+  const FrontEndBuffers& buffers() const { return launch_.buf; }
+  const FrontEndLaunch& launch() const { return launch_; }
+
+ private:
+  PlaneSet ps_;
+  DeviceArena arena_;
+  FrontEndLaunch launch_;
+};
+
+}  // namespace cctagx

@@ -1,0 +1,915 @@
+// gfx950 kernels of the CCTag front end: pyramid, derivative filters, Canny
+// (magnitude + NMS, union-find hysteresis), thinning, raster-order edge
+// compaction, gradient descent, vote, voter lists, seeds.
+//
+// Every kernel covers all planes (levels x frames) of a plane set in one
+// launch.  Float arithmetic reproduces the reference CPU path bit for bit:
+// the build disables contraction, and every rounding step (FMA, division,
+// square root, round-half-even) is spelled out with the IEEE intrinsics.
+#include <hip/hip_runtime.h>
+
+#include "frontend.hpp"
+#include "frontend_launch.hpp"
+
+namespace cctagx {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Plane lookup: the last plane whose first tile is <= blk (planes with no
+// tiles in a grid share their start with the following plane).
+template <int FIELD>
+__device__ __forceinline__ int plane_of(const PlaneDesc* P, int np, int blk) {
+  const int32_t* a = reinterpret_cast<const int32_t*>(P);
+  constexpr int S = sizeof(PlaneDesc) / sizeof(int32_t);
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid * S + FIELD] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+constexpr int F_TILE_F = 8, F_TILE_S = 9, F_TILE_C = 10, F_TILE_P = 11;
+static_assert(sizeof(PlaneDesc) == 12 * sizeof(int32_t), "PlaneDesc layout");
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---------------------------------------------------------------------------
+// a1: pyramid (Level.cpp:67).  cv::resize to exactly half size on 8U runs
+// OpenCV's fast 2x2 area path: vector lanes (the first floor(w/8)*8 outputs
+// of a row) compute (a+b+c+d+2)>>2, the scalar tail rounds (a+b+c+d)*0.25f
+// half to even.  One block turns a 64x64 level-0 tile into 32x32, 16x16 and
+// 8x8 tiles of levels 1..3 through LDS.
+__device__ __forceinline__ uint8_t area2(int s, int x, int vec_end) {
+  return (uint8_t)(x < vec_end ? (s + 2) >> 2 : (int)rintf((float)s * 0.25f));
+}
+
+__global__ __launch_bounds__(256) void k_pyramid(const PlaneDesc* __restrict__ P, int np,
+                                                  int n_levels, uint8_t* __restrict__ img) {
+  __shared__ uint8_t s1[32][33];
+  __shared__ uint8_t s2[16][17];
+  const int p0 = plane_of<F_TILE_P>(P, np, blockIdx.x);
+  const PlaneDesc d0 = P[p0];
+  const int t = blockIdx.x - d0.tile_p;
+  const int tx_n = (d0.w + kPyrTile - 1) / kPyrTile;
+  const int X0 = (t % tx_n) * kPyrTile, Y0 = (t / tx_n) * kPyrTile;
+  const int tid = threadIdx.x;
+  // level 1: 32x32, 4 outputs per thread, straight from global level 0
+  {
+    const PlaneDesc d1 = P[p0 + 1];
+    const int vec_end = (d1.w / 8) * 8;
+    for (int k = tid; k < 32 * 32; k += 256) {
+      const int ly = k >> 5, lx = k & 31;
+      const int x = (X0 >> 1) + lx, y = (Y0 >> 1) + ly;
+      uint8_t v = 0;
+      if (x < d1.w && y < d1.h) {
+        const uint8_t* r0 = img + d0.pix + (2 * y) * d0.w + 2 * x;
+        const int s = r0[0] + r0[1] + r0[d0.w] + r0[d0.w + 1];
+        v = area2(s, x, vec_end);
+        img[d1.pix + y * d1.w + x] = v;
+      }
+      s1[ly][lx] = v;
+    }
+  }
+  if (n_levels < 3) return;
+  __syncthreads();
+  {
+    const PlaneDesc d2 = P[p0 + 2];
+    const int vec_end = (d2.w / 8) * 8;
+    const int ly = tid >> 4, lx = tid & 15;
+    const int x = (X0 >> 2) + lx, y = (Y0 >> 2) + ly;
+    uint8_t v = 0;
+    if (x < d2.w && y < d2.h) {
+      const int s = s1[2 * ly][2 * lx] + s1[2 * ly][2 * lx + 1] + s1[2 * ly + 1][2 * lx] +
+                    s1[2 * ly + 1][2 * lx + 1];
+      v = area2(s, x, vec_end);
+      img[d2.pix + y * d2.w + x] = v;
+    }
+    s2[ly][lx] = v;
+  }
+  if (n_levels < 4) return;
+  __syncthreads();
+  if (tid < 64) {
+    const PlaneDesc d3 = P[p0 + 3];
+    const int vec_end = (d3.w / 8) * 8;
+    const int ly = tid >> 3, lx = tid & 7;
+    const int x = (X0 >> 3) + lx, y = (Y0 >> 3) + ly;
+    if (x < d3.w && y < d3.h) {
+      const int s = s2[2 * ly][2 * lx] + s2[2 * ly][2 * lx + 1] + s2[2 * ly + 1][2 * lx] +
+                    s2[2 * ly + 1][2 * lx + 1];
+      img[d3.pix + y * d3.w + x] = area2(s, x, vec_end);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a2: derivative filters (cvRecode.cpp:74-92).  kerneldX(i, j) = -/+ tap for
+// j < 4 / j > 4 (centre column zero); kerneldY = kerneldX^T.  The reference
+// (cv::filter2D, 8U->16S direct path) accumulates the 72 nonzero taps of each
+// kernel in row-major order with one FMA per tap from 0, then rounds half to
+// even.  A thread owns one column of 8 output rows; it streams the 16 window
+// rows from LDS once and applies each kernel row to the outputs that use it,
+// in increasing kernel-row order, so every output keeps the reference order.
+__host__ __device__ constexpr float tap(int r, int c) {
+  return r == 0 ? (c == 0 ? 0.000000143284235f : c == 1 ? 0.000003558691641f
+                 : c == 2 ? 0.000028902492951f : 0.000064765993382f)
+       : r == 1 ? (c == 0 ? 0.000004744922188f : c == 1 ? 0.000117847682078f
+                 : c == 2 ? 0.000957119116802f : 0.002144755142391f)
+       : r == 2 ? (c == 0 ? 0.000057804985902f : c == 1 ? 0.001435678675203f
+                 : c == 2 ? 0.011660097860113f : 0.026128466569370f)
+       : r == 3 ? (c == 0 ? 0.000259063973527f : c == 1 ? 0.006434265427174f
+                 : c == 2 ? 0.052256933138740f : 0.117099663048638f)
+                : (c == 0 ? 0.000427124283626f : c == 1 ? 0.010608310271112f
+                 : c == 2 ? 0.086157117207395f : 0.193064705260108f);
+}
+__host__ __device__ constexpr float kdx(int i, int j) {
+  return j == 4 ? 0.0f : (j < 4 ? -tap(i <= 4 ? i : 8 - i, j) : tap(i <= 4 ? i : 8 - i, 8 - j));
+}
+
+__global__ __launch_bounds__(256) void k_filter(const PlaneDesc* __restrict__ P, int np,
+                                                 const uint8_t* __restrict__ img,
+                                                 int16_t* __restrict__ dxo,
+                                                 int16_t* __restrict__ dyo) {
+  constexpr int TW = kFilterTileW, TH = kFilterTileH, LW = TW + 8, LH = TH + 8;
+  __shared__ float s[LH][LW + 1];
+  const int p = plane_of<F_TILE_F>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_f;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  const uint8_t* src = img + d.pix;
+  for (int k = threadIdx.x; k < LH * LW; k += 256) {
+    const int ly = k / LW, lx = k - ly * LW;
+    const int gy = clampi(y0 + ly - 4, 0, d.h - 1), gx = clampi(x0 + lx - 4, 0, d.w - 1);
+    s[ly][lx] = (float)src[gy * d.w + gx];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) * 8;
+  float ax[8], ay[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) { ax[o] = 0.0f; ay[o] = 0.0f; }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float v[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) v[j] = s[ty + r][tx + j];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int i = r - o;
+      if (i >= 0 && i < 9) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+          if (kdx(i, j) != 0.0f) ax[o] = __builtin_fmaf(v[j], kdx(i, j), ax[o]);
+          if (kdx(j, i) != 0.0f) ay[o] = __builtin_fmaf(v[j], kdx(j, i), ay[o]);
+        }
+      }
+    }
+  }
+  const int x = x0 + tx;
+  if (x < d.w) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int y = y0 + ty + o;
+      if (y < d.h) {
+        const int g = d.pix + y * d.w + x;
+        dxo[g] = (int16_t)(int)rintf(ax[o]);
+        dyo[g] = (int16_t)(int)rintf(ay[o]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a3 + a4: magnitude and NMS (cvRecode.cpp:174-330).  Magnitude
+// rint(sqrt(x*x + y*y)), zero outside the image; TG22 = 13573 in Q15; the
+// horizontal / vertical tests are (>, >=), the diagonal test is (>, >).
+// A candidate (bit 0) passed NMS with m > low; a strong pixel (bit 1) also
+// has m > high.  The reference's prev_flag / upper-neighbour demotions only
+// decide which strong pixels seed its flood fill, not which pixels it
+// reaches, so the final map is: candidates 8-connected to a strong pixel.
+__device__ __forceinline__ int magnitude(int x, int y) {
+  return (int)rintf(__fsqrt_rn((float)x * (float)x + (float)y * (float)y));
+}
+
+__global__ __launch_bounds__(256) void k_nms(const PlaneDesc* __restrict__ P, int np,
+                                              const int16_t* __restrict__ dx,
+                                              const int16_t* __restrict__ dy, int low, int high,
+                                              uint8_t* __restrict__ cls, int32_t* __restrict__ lab,
+                                              uint8_t* __restrict__ root_strong) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH;
+  __shared__ int smag[TH + 2][TW + 2];
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  for (int k = threadIdx.x; k < (TH + 2) * (TW + 2); k += 256) {
+    const int ly = k / (TW + 2), lx = k - ly * (TW + 2);
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    int m = 0;
+    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) {
+      const int g = d.pix + gy * d.w + gx;
+      m = magnitude(dx[g], dy[g]);
+    }
+    smag[ly][lx] = m;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63;
+  const int x = x0 + tx;
+  constexpr int TG22 = 13573;
+  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
+    const int y = y0 + ty;
+    if (x >= d.w || y >= d.h) continue;
+    const int g = d.pix + y * d.w + x;
+    const int sx = dx[g], sy = dy[g];
+    const int m = smag[ty + 1][tx + 1];
+    const int ax = sx < 0 ? -sx : sx;
+    int ay = sy < 0 ? -sy : sy;
+    uint8_t c = 0;
+    if (m > low) {
+      const int tg22x = ax * TG22;
+      const int tg67x = tg22x + ((ax + ax) << 15);
+      ay <<= 15;
+      bool pass;
+      if (ay < tg22x) {
+        pass = m > smag[ty + 1][tx] && m >= smag[ty + 1][tx + 2];
+      } else if (ay > tg67x) {
+        pass = m > smag[ty][tx + 1] && m >= smag[ty + 2][tx + 1];
+      } else {
+        const int s = (sx ^ sy) < 0 ? -1 : 1;
+        pass = m > smag[ty][tx + 1 - s] && m > smag[ty + 2][tx + 1 + s];
+      }
+      if (pass) c = (uint8_t)(m > high ? 3 : 1);
+    }
+    cls[g] = c;
+    lab[g] = c ? g : -1;
+    root_strong[g] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a5: hysteresis as connected components over candidate pixels, with the
+// lock-free union-find of Playne & Hawick (IEEE TPDS 2018): link the larger
+// root under the smaller with atomicMin until both finds agree.
+__device__ __forceinline__ int ld(const int32_t* L, int i) {
+  return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int uf_find(const int32_t* L, int x) {
+  int p = ld(L, x);
+  while (p != x) {
+    x = p;
+    p = ld(L, x);
+  }
+  return x;
+}
+__device__ void uf_merge(int32_t* L, int a, int b) {
+  bool done;
+  do {
+    a = uf_find(L, a);
+    b = uf_find(L, b);
+    if (a < b) {
+      const int old = atomicMin(L + b, a);
+      done = (old == b);
+      b = old;
+    } else if (b < a) {
+      const int old = atomicMin(L + a, b);
+      done = (old == a);
+      a = old;
+    } else {
+      done = true;
+    }
+  } while (!done);
+}
+
+__global__ __launch_bounds__(256) void k_cc_merge(const PlaneDesc* __restrict__ P, int np,
+                                                   const uint8_t* __restrict__ cls,
+                                                   int32_t* __restrict__ lab) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH;
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x = (t % tx_n) * TW + (threadIdx.x & 63);
+  const int y0 = (t / tx_n) * TH;
+  if (x >= d.w) return;
+  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
+    const int y = y0 + ty;
+    if (y >= d.h) break;
+    const int g = d.pix + y * d.w + x;
+    if (!(cls[g] & 1)) continue;
+    if (x > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
+    if (y > 0) {
+      const int u = g - d.w;
+      if (x > 0 && (cls[u - 1] & 1)) uf_merge(lab, g, u - 1);
+      if (cls[u] & 1) uf_merge(lab, g, u);
+      if (x + 1 < d.w && (cls[u + 1] & 1)) uf_merge(lab, g, u + 1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cc_flatten(const PlaneDesc* __restrict__ P, int np,
+                                                     const uint8_t* __restrict__ cls,
+                                                     int32_t* __restrict__ lab,
+                                                     uint8_t* __restrict__ root_strong) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH;
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x = (t % tx_n) * TW + (threadIdx.x & 63);
+  const int y0 = (t / tx_n) * TH;
+  if (x >= d.w) return;
+  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
+    const int y = y0 + ty;
+    if (y >= d.h) break;
+    const int g = d.pix + y * d.w + x;
+    const uint8_t c = cls[g];
+    if (!(c & 1)) continue;
+    const int r = uf_find(lab, g);
+    lab[g] = r;
+    if (c & 2) root_strong[r] = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a5 final map + a6 thinning (cvRecode.cpp:401-414, thinning.cpp:12-51).
+// The 512-entry tables as bit masks: bit i set <=> value 255.
+__device__ __forceinline__ bool lut_bit(int pass, int idx) {
+  constexpr uint32_t T0[16] = {0x337f0000u, 0x33330000u, 0x31ff0000u, 0x33ff0000u,
+                               0xffff0000u, 0xffff0000u, 0xffff0000u, 0xffff0000u,
+                               0xffff0000u, 0x33230000u, 0xffff0000u, 0xffff0000u,
+                               0xffff0000u, 0xff220000u, 0xffff0000u, 0xffff0000u};
+  constexpr uint32_t T1[16] = {0xf7ff0000u, 0xffff0000u, 0xf5ff0000u, 0xffff0000u,
+                               0xf4ff0000u, 0xecee0000u, 0xf4ff0000u, 0xecee0000u,
+                               0xffff0000u, 0xffef0000u, 0xffff0000u, 0xffff0000u,
+                               0xf4ff0000u, 0xecee0000u, 0xf4fe0000u, 0xecee0000u};
+  const uint32_t w = pass == 0 ? T0[idx >> 5] : T1[idx >> 5];
+  return (w >> (idx & 31)) & 1u;
+}
+
+// 3x3 index of thinning.cpp:42-45 over a tile with a 1-pixel halo.
+template <int LW>
+__device__ __forceinline__ int thin_index(const uint8_t (*s)[LW], int ly, int lx) {
+  return (s[ly - 1][lx - 1] == 255) + (s[ly - 1][lx] == 255) * 8 + (s[ly - 1][lx + 1] == 255) * 64 +
+         (s[ly][lx - 1] == 255) * 2 + (s[ly][lx] == 255) * 16 + (s[ly][lx + 1] == 255) * 128 +
+         (s[ly + 1][lx - 1] == 255) * 4 + (s[ly + 1][lx] == 255) * 32 +
+         (s[ly + 1][lx + 1] == 255) * 256;
+}
+
+// Canny map e = 255 for candidates whose component holds a strong pixel;
+// writes the map (kept on the border) and thinning pass 1 (interior only;
+// the temporary's border is 0).
+__global__ __launch_bounds__(256) void k_thin1(const PlaneDesc* __restrict__ P, int np,
+                                                const uint8_t* __restrict__ cls,
+                                                const int32_t* __restrict__ lab,
+                                                const uint8_t* __restrict__ root_strong,
+                                                uint8_t* __restrict__ edges,
+                                                uint8_t* __restrict__ tmp) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH, LW = TW + 2;
+  __shared__ uint8_t se[TH + 2][LW];
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  for (int k = threadIdx.x; k < (TH + 2) * LW; k += 256) {
+    const int ly = k / LW, lx = k - ly * LW;
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    uint8_t e = 0;
+    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) {
+      const int g = d.pix + gy * d.w + gx;
+      if ((cls[g] & 1) && root_strong[lab[g]]) e = 255;
+    }
+    se[ly][lx] = e;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63;
+  const int x = x0 + tx;
+  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
+    const int y = y0 + ty;
+    if (x >= d.w || y >= d.h) continue;
+    const int g = d.pix + y * d.w + x;
+    const uint8_t e = se[ty + 1][tx + 1];
+    edges[g] = e;
+    uint8_t o = 0;
+    if (e != 0 && x > 0 && y > 0 && x < d.w - 1 && y < d.h - 1)
+      o = lut_bit(0, thin_index<LW>(se, ty + 1, tx + 1)) ? 255 : 0;
+    tmp[g] = o;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_thin2(const PlaneDesc* __restrict__ P, int np,
+                                                const uint8_t* __restrict__ tmp,
+                                                uint8_t* __restrict__ edges) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH, LW = TW + 2;
+  __shared__ uint8_t st[TH + 2][LW];
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  for (int k = threadIdx.x; k < (TH + 2) * LW; k += 256) {
+    const int ly = k / LW, lx = k - ly * LW;
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    uint8_t v = 0;
+    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) v = tmp[d.pix + gy * d.w + gx];
+    st[ly][lx] = v;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63;
+  const int x = x0 + tx;
+  if (x <= 0 || x >= d.w - 1) return;
+  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
+    const int y = y0 + ty;
+    if (y <= 0 || y >= d.h - 1) continue;
+    const int g = d.pix + y * d.w + x;
+    uint8_t o = 0;
+    if (st[ty + 1][tx + 1] != 0) o = lut_bit(1, thin_index<LW>(st, ty + 1, tx + 1)) ? 255 : 0;
+    edges[g] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of one int per thread (256 threads = 4 waves).
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int wsum[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int n = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += n;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wid) base += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - v;
+}
+
+// a7: raster-order compaction of edges == 255 (Canny.cpp:24-33) in 4096-px
+// chunks of the plane's linear pixel order.
+__global__ __launch_bounds__(256) void k_compact_count(const PlaneDesc* __restrict__ P, int np,
+                                                        const uint8_t* __restrict__ edges,
+                                                        int32_t* __restrict__ tcount) {
+  const int p = plane_of<F_TILE_C>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int c = blockIdx.x - d.tile_c;
+  const int n = d.w * d.h;
+  const int b = c * kChunk + threadIdx.x * 16;
+  int cnt = 0;
+  if (b + 16 <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(edges + d.pix + b);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cnt += ((w[k] >> (8 * q)) & 0xffu) == 255u;
+  } else {
+    for (int k = b; k < n && k < b + 16; ++k) cnt += edges[d.pix + k] == 255;
+  }
+  int total;
+  block_exclusive_scan(cnt, &total);
+  if (threadIdx.x == 0) tcount[blockIdx.x] = total;
+}
+
+// Per-plane exclusive scan of chunk counts (one 1024-thread block per plane).
+__global__ __launch_bounds__(1024) void k_compact_scan(const PlaneDesc* __restrict__ P, int np,
+                                                        int32_t* __restrict__ tcount,
+                                                        PlaneCounts* __restrict__ counts) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int p = blockIdx.x;
+  const PlaneDesc d = P[p];
+  const int nchunks = (d.w * d.h + kChunk - 1) / kChunk;
+  int32_t* a = tcount + d.tile_c;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < nchunks; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < nchunks ? a[i] : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int n = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += n;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wid) wb += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < nchunks) a[i] = carry + wb + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[p].n_edges = carry;
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(
+    const PlaneDesc* __restrict__ P, int np, const uint8_t* __restrict__ edges,
+    const int16_t* __restrict__ dx, const int16_t* __restrict__ dy,
+    const int32_t* __restrict__ tcount, int32_t* __restrict__ emap, int16_t* __restrict__ ex,
+    int16_t* __restrict__ ey, int16_t* __restrict__ egx, int16_t* __restrict__ egy,
+    int32_t* __restrict__ vcount, int32_t* __restrict__ seedmark) {
+  const int p = plane_of<F_TILE_C>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int c = blockIdx.x - d.tile_c;
+  const int n = d.w * d.h;
+  const int b = c * kChunk + threadIdx.x * 16;
+  uint8_t e[16];
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    e[k] = (b + k < n) ? edges[d.pix + b + k] : 0;
+    cnt += e[k] == 255;
+  }
+  int total;
+  int idx = tcount[blockIdx.x] + block_exclusive_scan(cnt, &total);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int li = b + k;
+    if (li >= n) break;
+    const int g = d.pix + li;
+    if (e[k] == 255) {
+      const int y = li / d.w, x = li - y * d.w;
+      const int s = d.pix + idx;
+      ex[s] = (int16_t)x;
+      ey[s] = (int16_t)y;
+      egx[s] = dx[g];
+      egy[s] = dy[g];
+      vcount[s] = 0;
+      seedmark[s] = -1;
+      emap[g] = idx++;
+    } else {
+      emap[g] = -1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a8: gradientDirectionDescent (Bresenham.cpp:18-209).  One lane per
+// (edge point, direction).  Bounds are the collection shape (the level-0
+// size); lookups beyond the level size find nothing.
+struct EdgeLookup {
+  const int32_t* emap;
+  int pix, w, h;
+  __device__ __forceinline__ int at(int x, int y) const {
+    return (x < w && y < h) ? emap[pix + y * w + x] : -1;
+  }
+};
+
+__device__ __forceinline__ void update_xy(float dx, float dy, int& x, int& y, float& e, int& sx,
+                                          int& sy) {
+  const float a = fabsf(__fdiv_rn(dy, dx));
+  sx = (dx > 0.0f) - (dx < 0.0f);
+  sy = (dy > 0.0f) - (dy < 0.0f);
+  e = __fadd_rn(e, a);
+  x += sx;
+  if (e >= 0.5f) {
+    y += sy;
+    e = __fsub_rn(e, 1.0f);
+  }
+}
+
+__device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float gx, float gy,
+                       int dir, int nmax, int thr) {
+  float e = 0.0f;
+  float dx = (float)dir * gx, dy = (float)dir * gy;
+  const float dxRef = dx, dyRef = dy;
+  const bool ymajor = fabsf(dy) > fabsf(dx);
+  int n = 0, sx = 0, sy = 0, x = px, y = py;
+  // steps 1 and 2, re-orientation in between
+  for (int s = 0; s < 2; ++s) {
+    if (ymajor) update_xy(dy, dx, y, x, e, sy, sx);
+    else update_xy(dx, dy, x, y, e, sx, sy);
+    n = n + 1;
+    if (s == 0 && __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)) > (float)thr) {
+      const float dd = __fadd_rn(__fmul_rn(gx, dxRef), __fmul_rn(gy, dyRef));
+      const float sg = (float)((dd > 0.0f) - (dd < 0.0f));
+      dx = sg * gx;
+      dy = sg * gy;
+    }
+  }
+  if (x >= 0 && x < W && y >= 0 && y < H) {
+    const int r = L.at(x, y);
+    if (r >= 0) return r;
+  } else {
+    return -1;
+  }
+  while (n <= nmax) {
+    if (ymajor) update_xy(dy, dx, y, x, e, sy, sx);
+    else update_xy(dx, dy, x, y, e, sx, sy);
+    n = n + 1;
+    if (!(x >= 0 && x < W && y >= 0 && y < H)) return -1;
+    int r = L.at(x, y);
+    if (r >= 0) return r;
+    const int xs = ymajor ? x : x - sx, ys = ymajor ? y - sy : y;
+    if (!(xs >= 0 && xs < W && ys >= 0 && ys < H)) return -1;
+    r = L.at(xs, ys);
+    if (r >= 0) return r;
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(256) void k_descent(const PlaneDesc* __restrict__ P,
+                                                  const PlaneCounts* __restrict__ counts,
+                                                  const int32_t* __restrict__ emap,
+                                                  const int16_t* __restrict__ ex,
+                                                  const int16_t* __restrict__ ey,
+                                                  const int16_t* __restrict__ egx,
+                                                  const int16_t* __restrict__ egy, int nmax,
+                                                  int thr, int32_t* __restrict__ before,
+                                                  int32_t* __restrict__ after) {
+  const PlaneDesc d = P[blockIdx.y];
+  const int n2 = 2 * counts[blockIdx.y].n_edges;
+  const EdgeLookup L{emap, d.pix, d.w, d.h};
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
+    const int i = k >> 1, s = d.pix + i;
+    const int dir = (k & 1) ? 1 : -1;
+    const int r = descent(L, d.shape_w, d.shape_h, ex[s], ey[s], (float)egx[s], (float)egy[s],
+                          dir, nmax, thr);
+    if (dir < 0) before[s] = r;
+    else after[s] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a9: vote chain walk (Vote.cpp:101-243).  Gradients are small integers, so
+// the dot products are exact; distances are sqrt of exact integer sums.
+__device__ __forceinline__ float neg_dot(const int16_t* gx, const int16_t* gy, int a, int b) {
+  return -__fadd_rn(__fmul_rn((float)gx[a], (float)gx[b]), __fmul_rn((float)gy[a], (float)gy[b]));
+}
+__device__ __forceinline__ float dist2d(const int16_t* ex, const int16_t* ey, int a, int b) {
+  const int ddx = ex[b] - ex[a], ddy = ey[b] - ey[a];
+  return __fsqrt_rn(__fadd_rn((float)(ddx * ddx), (float)(ddy * ddy)));
+}
+
+__global__ __launch_bounds__(256) void k_vote(const PlaneDesc* __restrict__ P,
+                                               const PlaneCounts* __restrict__ counts,
+                                               const int16_t* __restrict__ ex,
+                                               const int16_t* __restrict__ ey,
+                                               const int16_t* __restrict__ egx,
+                                               const int16_t* __restrict__ egy,
+                                               const int32_t* __restrict__ before,
+                                               const int32_t* __restrict__ after, int n_crowns,
+                                               float ratio, float angle,
+                                               int32_t* __restrict__ chosen,
+                                               float* __restrict__ tdist,
+                                               int32_t* __restrict__ vcount) {
+  const PlaneDesc d = P[blockIdx.y];
+  const int n = counts[blockIdx.y].n_edges;
+  const int16_t* X = ex + d.pix;
+  const int16_t* Y = ey + d.pix;
+  const int16_t* GX = egx + d.pix;
+  const int16_t* GY = egy + d.pix;
+  const int32_t* B = before + d.pix;
+  const int32_t* A = after + d.pix;
+  for (int ip = blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += gridDim.x * blockDim.x) {
+    float vd[16];
+    int nv = 0;
+    int current = B[ip];
+    int choosen = -1;
+    float total = 0.0f;
+    if (current >= 0 && neg_dot(GX, GY, ip, current) >= angle) {
+      float dist = dist2d(X, Y, ip, current);
+      vd[nv++] = dist;
+      total = __fadd_rn(total, dist);
+      for (int i = 1; i < n_crowns; ++i) {
+        choosen = -1;
+        int target = A[current];
+        if (target < 0) break;
+        if (!(neg_dot(GX, GY, target, current) >= angle)) break;
+        dist = dist2d(X, Y, target, current);
+        vd[nv++] = dist;
+        total = __fadd_rn(total, dist);
+        bool ok = true;
+        for (int a = 0; a < nv; ++a)
+          for (int b = a + 1; b < nv; ++b)
+            ok = ok && (vd[a] <= __fmul_rn(vd[b], ratio)) && (vd[b] <= __fmul_rn(vd[a], ratio));
+        if (!ok) break;
+        current = target;
+        target = B[current];
+        if (target < 0) break;
+        if (!(neg_dot(GX, GY, target, current) >= angle)) break;
+        dist = dist2d(X, Y, target, current);
+        vd[nv++] = dist;
+        total = __fadd_rn(total, dist);
+        for (int a = 0; a < nv; ++a)
+          for (int b = a + 1; b < nv; ++b)
+            ok = ok && (vd[a] <= __fmul_rn(vd[b], ratio)) && (vd[b] <= __fmul_rn(vd[a], ratio));
+        if (!ok) break;
+        current = target;
+        choosen = current;
+      }
+    }
+    chosen[d.pix + ip] = choosen;
+    tdist[d.pix + ip] = total;
+    if (choosen >= 0) atomicAdd(vcount + d.pix + choosen, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a10: voter lists (Types.cpp:66-84), flow length (Vote.cpp:232), seeds
+// (Vote.cpp:236-240).  CSR offsets by a per-plane scan of vote counts; lists
+// filled with atomics, then each list sorted ascending (the CPU pushes voters
+// in index order), flow length accumulated in that order, and the point's
+// seed key (the voter that brought it to min votes) recorded.
+__global__ __launch_bounds__(1024) void k_voters_scan(const PlaneDesc* __restrict__ P,
+                                                       PlaneCounts* __restrict__ counts,
+                                                       const int32_t* __restrict__ vcount,
+                                                       int32_t* __restrict__ vindex,
+                                                       int32_t* __restrict__ cursor) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int p = blockIdx.x;
+  const PlaneDesc d = P[p];
+  const int n = counts[p].n_edges;
+  const int32_t* a = vcount + d.pix;
+  int32_t* o = vindex + d.pix + p;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < n ? a[i] : 0;
+    int incl = v;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const int t = __shfl_up(incl, s, 64);
+      if (lane >= s) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wid) wb += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < n) {
+      o[i] = carry + wb + incl - v;
+      cursor[d.pix + i] = carry + wb + incl - v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    o[n] = carry;
+    counts[p].n_voters = carry;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_voters_fill(const PlaneDesc* __restrict__ P,
+                                                      const PlaneCounts* __restrict__ counts,
+                                                      const int32_t* __restrict__ chosen,
+                                                      int32_t* __restrict__ cursor,
+                                                      int32_t* __restrict__ vlist) {
+  const PlaneDesc d = P[blockIdx.y];
+  const int n = counts[blockIdx.y].n_edges;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const int c = chosen[d.pix + v];
+    if (c >= 0) {
+      const int pos = atomicAdd(cursor + d.pix + c, 1);
+      vlist[d.pix + pos] = v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_voters_finish(const PlaneDesc* __restrict__ P,
+                                                        const PlaneCounts* __restrict__ counts,
+                                                        const int32_t* __restrict__ vindex,
+                                                        const float* __restrict__ tdist,
+                                                        int min_votes,
+                                                        int32_t* __restrict__ vlist,
+                                                        float* __restrict__ flow,
+                                                        int32_t* __restrict__ ismax,
+                                                        int32_t* __restrict__ seedmark) {
+  const int p = blockIdx.y;
+  const PlaneDesc d = P[p];
+  const int n = counts[p].n_edges;
+  const int32_t* vi = vindex + d.pix + p;
+  int32_t* vl = vlist + d.pix;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int b = vi[c], e = vi[c + 1];
+    // insertion sort of the (short) voter list
+    for (int i = b + 1; i < e; ++i) {
+      const int key = vl[i];
+      int j = i - 1;
+      while (j >= b && vl[j] > key) {
+        vl[j + 1] = vl[j];
+        --j;
+      }
+      vl[j + 1] = key;
+    }
+    float fl = 0.0f;
+    for (int k = 0; k < e - b; ++k)
+      fl = __fdiv_rn(__fadd_rn(__fmul_rn(fl, (float)k), tdist[d.pix + vl[b + k]]), (float)(k + 1));
+    flow[d.pix + c] = fl;
+    const int nv = e - b;
+    ismax[d.pix + c] = nv >= min_votes ? nv : -1;
+    if (nv >= min_votes) seedmark[d.pix + vl[b + min_votes - 1]] = c;
+  }
+}
+
+// Seeds in the order they reached min votes = ascending key voter index.
+__global__ __launch_bounds__(1024) void k_seeds(const PlaneDesc* __restrict__ P,
+                                                 PlaneCounts* __restrict__ counts,
+                                                 const int32_t* __restrict__ seedmark,
+                                                 int32_t* __restrict__ seeds) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int p = blockIdx.x;
+  const PlaneDesc d = P[p];
+  const int n = counts[p].n_edges;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int s = i < n ? seedmark[d.pix + i] : -1;
+    const int v = s >= 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wid) wb += wsum[w];
+      tot += wsum[w];
+    }
+    if (v) seeds[d.pix + carry + wb + incl - 1] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[p].n_seeds = carry;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch sequence.  No host synchronization inside: data-dependent sizes are
+// read on the device, so the whole sequence can be captured in a hipGraph.
+int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) {
+  const FrontEndBuffers& b = L.buf;
+  const PlaneDesc* P = L.dev_planes;
+  const int np = L.n_planes;
+  auto mark = [&](const char* name) {
+    if (timer) timer->mark(name, s);
+  };
+  mark("start");
+  if (L.n_levels > 1 && L.n_tiles_p > 0) {
+    k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img);
+    mark("pyramid");
+  }
+  k_filter<<<L.n_tiles_f, 256, 0, s>>>(P, np, b.img, b.dx, b.dy);
+  mark("filter");
+  k_nms<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.dx, b.dy, L.prm.low, L.prm.high, b.cls, b.lab,
+                                    b.root_strong);
+  mark("nms");
+  k_cc_merge<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab);
+  mark("cc_merge");
+  k_cc_flatten<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong);
+  mark("cc_flatten");
+  k_thin1<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong, b.edges, b.tmp);
+  mark("thin1");
+  k_thin2<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.tmp, b.edges);
+  mark("thin2");
+  k_compact_count<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.tcount);
+  k_compact_scan<<<np, 1024, 0, s>>>(P, np, b.tcount, b.counts);
+  k_compact_write<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.dx, b.dy, b.tcount, b.emap,
+                                               b.ex, b.ey, b.egx, b.egy, b.vcount, b.seedmark);
+  mark("compact");
+  const dim3 g(L.edge_blocks, np);
+  k_descent<<<g, 256, 0, s>>>(P, b.counts, b.emap, b.ex, b.ey, b.egx, b.egy,
+                              L.prm.dist_search, L.prm.thr_grad, b.before, b.after);
+  mark("descent");
+  k_vote<<<g, 256, 0, s>>>(P, b.counts, b.ex, b.ey, b.egx, b.egy, b.before, b.after,
+                           L.prm.n_crowns, L.prm.ratio_voting, L.prm.angle_voting, b.chosen,
+                           b.tdist, b.vcount);
+  mark("vote");
+  k_voters_scan<<<np, 1024, 0, s>>>(P, b.counts, b.vcount, b.vindex, b.ismax);
+  k_voters_fill<<<g, 256, 0, s>>>(P, b.counts, b.chosen, b.ismax, b.vlist);
+  k_voters_finish<<<g, 256, 0, s>>>(P, b.counts, b.vindex, b.tdist, L.prm.min_votes, b.vlist,
+                                    b.flow, b.ismax, b.seedmark);
+  k_seeds<<<np, 1024, 0, s>>>(P, b.counts, b.seedmark, b.seeds);
+  mark("voters_seeds");
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace cctagx

@@ -1,0 +1,77 @@
+"""Python host mirror of the reference's detection entry points.
+
+`Detector` owns one library handle (one frame size), the counterpart of a
+reference `TagPipe` selected by pipeId (Detection.cpp:732-760).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Detector:
+    def __init__(self, width: int, height: int, params: _lib.Params | None = None,
+                 n_crowns: int = 3):
+        self._lib = _lib.load()
+        self.params = params if params is not None else _lib.default_params(n_crowns)
+        self.width, self.height = int(width), int(height)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.cctagx_create(ctypes.byref(self.params), self.width, self.height,
+                                           ctypes.byref(h)), "cctagx_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.cctagx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _gray(self, img: np.ndarray) -> np.ndarray:
+        img = np.ascontiguousarray(img)
+        if img.dtype != np.uint8 or img.ndim != 2:
+            raise ValueError("expected a 2-D uint8 gray frame")  # Detection.cpp:815-817
+        if img.shape != (self.height, self.width):
+            raise ValueError("frame size %s differs from the handle's %s"
+                             % (img.shape, (self.height, self.width)))
+        return img
+
+    def front_end(self, img: np.ndarray) -> None:
+        img = self._gray(img)
+        _lib.check(self._lib.cctagx_front_end(self._h, img.ctypes.data, img.strides[0]),
+                   "cctagx_front_end")
+
+    def level(self, level: int) -> dict:
+        v = _lib.LevelView()
+        _lib.check(self._lib.cctagx_get_level(self._h, level, ctypes.byref(v)),
+                   "cctagx_get_level")
+        return _lib.level_view_to_dict(v)
+
+    def stage_times(self) -> list:
+        out = []
+        n = self._lib.cctagx_stage_count(self._h)
+        for i in range(max(n, 0)):
+            name = ctypes.c_char_p()
+            us = ctypes.c_float()
+            if self._lib.cctagx_stage_time(self._h, i, ctypes.byref(name), ctypes.byref(us)) == 0:
+                out.append((name.value.decode(), us.value))
+        return out
+
+    def time_front_end(self, iters: int) -> float:
+        us = ctypes.c_float()
+        _lib.check(self._lib.cctagx_time_front_end(self._h, iters, ctypes.byref(us)),
+                   "cctagx_time_front_end")
+        return us.value

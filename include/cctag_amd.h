@@ -1,0 +1,129 @@
+/*
+ * cctag_amd -- C ABI of the MI355X-native CCTag detection hot path.
+ *
+ * Plain C: pointers, sizes and int error codes (0 = OK, < 0 = error; the
+ * library never exits).  One handle per frame size and per calling thread;
+ * a handle is not thread-safe.  Host buffers passed in are caller-owned;
+ * views returned point into handle-owned host memory that stays valid until
+ * the next call on the same handle.
+ *
+ * Each entry point names the reference interface it replaces
+ * (alicevision/CCTag v1.0.3, src/cctag/...).
+ */
+#ifndef CCTAG_AMD_H
+#define CCTAG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCTAGX_OK 0
+#define CCTAGX_ERR_ARG (-1)        /* bad argument (null pointer, size, level) */
+#define CCTAGX_ERR_HIP (-2)        /* a HIP runtime call failed */
+#define CCTAGX_ERR_CAPACITY (-3)   /* a fixed capacity was exceeded */
+#define CCTAGX_ERR_STATE (-4)      /* call order violated (e.g. view before front end) */
+#define CCTAGX_ERR_NO_DEVICE (-5)  /* no gfx950 device visible */
+#define CCTAGX_ERR_UNSUPPORTED (-6)/* frame size or parameter outside what is built */
+
+/* Mirrors cctag::Parameters (Params.hpp:108-199); defaults of Params.hpp:27-66. */
+typedef struct cctagx_params {
+  float canny_thr_low;                    /* _cannyThrLow 0.01 */
+  float canny_thr_high;                   /* _cannyThrHigh 0.04 */
+  int32_t dist_search;                    /* _distSearch 30 */
+  int32_t thr_gradient_mag_in_vote;       /* _thrGradientMagInVote 2500 */
+  float angle_voting;                     /* _angleVoting 0 (must be 0) */
+  float ratio_voting;                     /* _ratioVoting 4 */
+  float average_vote_min;                 /* _averageVoteMin 0 */
+  float thr_median_distance_ellipse;      /* _thrMedianDistanceEllipse 3 */
+  int32_t maximum_nb_seeds;               /* _maximumNbSeeds 500 */
+  int32_t maximum_nb_candidates_loop_two; /* _maximumNbCandidatesLoopTwo 40 */
+  int32_t n_crowns;                       /* _nCrowns 3 */
+  int32_t min_points_segment_candidate;   /* _minPointsSegmentCandidate 10 */
+  int32_t min_votes_to_select_candidate;  /* _minVotesToSelectCandidate 3 */
+  float thresh_robust_estimation_of_outer_ellipse; /* 30 */
+  float ellipse_growing_elliptic_hull_width;       /* 2.3 */
+  int32_t window_size_on_inner_elliptic_segment;   /* 20 */
+  int32_t number_of_multires_layers;               /* 4 */
+  int32_t number_of_processed_multires_layers;     /* 4 */
+  int32_t n_samples_outer_ellipse;                 /* 150 */
+  int32_t num_cuts_in_ident_step;                  /* 22 */
+  int32_t num_samples_outer_edge_points_refinement;/* 20 */
+  int32_t sample_cut_length;                       /* 100 */
+  int32_t imaged_center_n_grid_sample;             /* 5 */
+  float imaged_center_neighbour_size;              /* 0.2 */
+  float min_ident_proba;                           /* 1e-6 */
+  int32_t search_for_another_segment;              /* 1 */
+  int32_t do_identification;                       /* 1 */
+} cctagx_params;
+
+/* cctag::Parameters::Parameters(nCrowns), Params.cpp:38-76. */
+void cctagx_params_default(cctagx_params* p, int n_crowns);
+
+typedef struct cctagx_handle cctagx_handle;
+
+/* Replaces TagPipe::initialize(w, h) (cuda/tag.cu:76-127) and the per-pipe
+ * allocation of initCuda (Detection.cpp:732-760).  Allocates every device and
+ * pinned host buffer for frames of width x height once. */
+int cctagx_create(const cctagx_params* params, int width, int height, cctagx_handle** out);
+int cctagx_destroy(cctagx_handle* h);
+
+/* ---- front end (pyramid .. vote) ---------------------------------------
+ * Replaces TagPipe::load + TagPipe::tagframe (cuda/tag.cu:154-216): upload a
+ * row-major 8-bit gray frame (stride in bytes) and run every pyramid level
+ * through derivatives, Canny, thinning, edge compaction, gradient descent,
+ * vote and seed selection on the GPU. */
+int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, size_t stride);
+
+/* Host view of one level after the front end; what TagPipe::convertToHost
+ * (cuda/frame_export.cu:23-165) and TagPipe::getPlane/getDx/getDy/getEdges
+ * export.  All arrays are in raster / point-index order. */
+typedef struct cctagx_level_view {
+  int32_t width, height;          /* level size */
+  int32_t shape_w, shape_h;       /* EdgePointCollection shape (level-0 size) */
+  const uint8_t* img;             /* width*height */
+  const int16_t* dx;              /* width*height */
+  const int16_t* dy;              /* width*height */
+  const uint8_t* edges;           /* width*height, thinned Canny map {0,255} */
+  const int32_t* edge_map;        /* width*height, point index or -1 */
+  int32_t n_edges;
+  const int16_t* edge_x;          /* n_edges, raster order */
+  const int16_t* edge_y;
+  const int16_t* edge_dx;
+  const int16_t* edge_dy;
+  const int32_t* before;          /* n_edges, descent along -gradient, -1 = none */
+  const int32_t* after;           /* n_edges, descent along +gradient */
+  const int32_t* chosen;          /* n_edges, voted point, -1 = none */
+  const float* total_dist;        /* n_edges, field-line length of the vote */
+  const int32_t* voters_index;    /* n_edges + 1, CSR offsets */
+  const int32_t* voters_list;     /* voters_index[n_edges], ascending per point */
+  const float* flow_length;       /* n_edges, running mean of total_dist over voters */
+  const int32_t* is_max;          /* n_edges, vote count when >= min votes, else -1 */
+  int32_t n_seeds;
+  const int32_t* seeds;           /* n_seeds, in the order they reached min votes */
+} cctagx_level_view;
+
+int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* out);
+
+/* ---- timing ------------------------------------------------------------
+ * Stage times of the last front end in microseconds, measured with HIP
+ * events on the library stream; names are static strings. */
+int cctagx_stage_count(cctagx_handle* h);
+int cctagx_stage_time(cctagx_handle* h, int i, const char** name, float* usec);
+/* Times `iters` back-to-back front ends on the resident frame (no upload)
+ * between two HIP events and returns the average in microseconds. */
+int cctagx_time_front_end(cctagx_handle* h, int iters, float* usec_per_frame);
+
+/* Library / device info.  cctagx_last_error() describes the last failure
+ * of the calling thread. */
+const char* cctagx_version(void);
+const char* cctagx_last_error(void);
+int cctagx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCTAG_AMD_H */

@@ -1,0 +1,91 @@
+// ORACLE -- test infrastructure only (see oracle.hpp).
+// ctypes-facing wrappers around the CPU restatement.  The level view struct
+// is the product's (include/cctag_amd.h) so tests compare like with like.
+#include "../include/cctag_amd.h"
+#include "oracle.hpp"
+
+#include <chrono>
+#include <exception>
+#include <vector>
+
+namespace {
+struct OracleRun {
+  oracle::Params p;
+  std::vector<oracle::LevelResult> levels;
+};
+
+template <class T>
+const T* ptr(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
+}  // namespace
+
+extern "C" {
+
+int oracle_front_end(const uint8_t* img, int w, int h, int n_levels, void** out) {
+  try {
+    auto* r = new OracleRun();
+    r->p.n_levels = n_levels;
+    oracle::front_end(img, w, h, r->p, r->levels);
+    *out = r;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+int oracle_level_view(void* handle, int level, cctagx_level_view* v) {
+  auto* r = static_cast<OracleRun*>(handle);
+  if (!r || level < 0 || level >= (int)r->levels.size()) return -1;
+  const oracle::LevelResult& L = r->levels[level];
+  v->width = L.w; v->height = L.h; v->shape_w = L.W0; v->shape_h = L.H0;
+  v->img = ptr(L.img); v->dx = ptr(L.dx); v->dy = ptr(L.dy); v->edges = ptr(L.edges);
+  v->edge_map = ptr(L.edge_map);
+  v->n_edges = (int32_t)L.ex.size();
+  v->edge_x = ptr(L.ex); v->edge_y = ptr(L.ey); v->edge_dx = ptr(L.edx); v->edge_dy = ptr(L.edy);
+  v->before = ptr(L.before); v->after = ptr(L.after);
+  v->chosen = ptr(L.chosen); v->total_dist = ptr(L.total_dist);
+  v->voters_index = ptr(L.voters_index); v->voters_list = ptr(L.voters_list);
+  v->flow_length = ptr(L.flow_length); v->is_max = ptr(L.is_max);
+  v->n_seeds = (int32_t)L.seeds.size(); v->seeds = ptr(L.seeds);
+  return 0;
+}
+
+void oracle_free(void* handle) { delete static_cast<OracleRun*>(handle); }
+
+void oracle_derivatives(const uint8_t* img, int w, int h, int16_t* dx, int16_t* dy) {
+  oracle::derivatives(img, w, h, dx, dy);
+}
+
+void oracle_derivatives_float(const uint8_t* img, int w, int h, float* dx, float* dy) {
+  oracle::derivatives_float(img, w, h, dx, dy);
+}
+
+void oracle_canny(const int16_t* dx, const int16_t* dy, int w, int h, float low, float high,
+                  uint8_t* edges) {
+  oracle::canny(dx, dy, w, h, low, high, edges);
+}
+
+void oracle_thin(uint8_t* edges, int w, int h) { oracle::thin(edges, w, h); }
+
+int oracle_thin_lut(int pass, int index) { return oracle::thin_lut(pass, index); }
+
+int oracle_downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
+  try {
+    oracle::downsample(src, sw, sh, dst, dw, dh);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Wall time of the whole front end on this host, for the CPU baseline.
+double oracle_time_front_end(const uint8_t* img, int w, int h, int n_levels, int iters) {
+  oracle::Params p;
+  p.n_levels = n_levels;
+  std::vector<oracle::LevelResult> levels;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < iters; ++i) oracle::front_end(img, w, h, p, levels);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count() / iters;
+}
+
+}  // extern "C"

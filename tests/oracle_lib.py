@@ -1,0 +1,115 @@
+"""ctypes access to the oracle (oracle/liboracle.so) -- the checker of the
+tests, never the thing under test."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from cctag_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_o = None
+
+
+def load():
+    global _o
+    if _o is not None:
+        return _o
+    if not os.path.exists(ORACLE_LIB):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    o = ctypes.CDLL(ORACLE_LIB)
+    vp = ctypes.c_void_p
+    o.oracle_front_end.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    o.oracle_level_view.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_lib.LevelView)]
+    o.oracle_free.argtypes = [vp]
+    o.oracle_derivatives.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    o.oracle_derivatives_float.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    o.oracle_canny.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                               ctypes.c_float, vp]
+    o.oracle_thin.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    o.oracle_thin_lut.argtypes = [ctypes.c_int, ctypes.c_int]
+    o.oracle_downsample.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
+    o.oracle_time_front_end.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    o.oracle_time_front_end.restype = ctypes.c_double
+    _o = o
+    return o
+
+
+def front_end(img: np.ndarray, n_levels: int = 4) -> list:
+    """All levels of the oracle front end as dicts (same keys as the product)."""
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    handle = ctypes.c_void_p()
+    rc = o.oracle_front_end(img.ctypes.data, w, h, n_levels, ctypes.byref(handle))
+    if rc != 0:
+        raise RuntimeError("oracle front end failed")
+    try:
+        out = []
+        for l in range(n_levels):
+            v = _lib.LevelView()
+            o.oracle_level_view(handle, l, ctypes.byref(v))
+            out.append(_lib.level_view_to_dict(v))
+        return out
+    finally:
+        o.oracle_free(handle)
+
+
+def derivatives(img: np.ndarray):
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    dx = np.empty((h, w), np.int16)
+    dy = np.empty((h, w), np.int16)
+    o.oracle_derivatives(img.ctypes.data, w, h, dx.ctypes.data, dy.ctypes.data)
+    return dx, dy
+
+
+def derivatives_float(img: np.ndarray):
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    dx = np.empty((h, w), np.float32)
+    dy = np.empty((h, w), np.float32)
+    o.oracle_derivatives_float(img.ctypes.data, w, h, dx.ctypes.data, dy.ctypes.data)
+    return dx, dy
+
+
+def canny(dx: np.ndarray, dy: np.ndarray, low=0.01 * 256, high=0.04 * 256) -> np.ndarray:
+    o = load()
+    dx = np.ascontiguousarray(dx, np.int16)
+    dy = np.ascontiguousarray(dy, np.int16)
+    h, w = dx.shape
+    e = np.zeros((h, w), np.uint8)
+    o.oracle_canny(dx.ctypes.data, dy.ctypes.data, w, h, low, high, e.ctypes.data)
+    return e
+
+
+def thin(edges: np.ndarray) -> np.ndarray:
+    o = load()
+    e = np.ascontiguousarray(edges, np.uint8).copy()
+    h, w = e.shape
+    o.oracle_thin(e.ctypes.data, w, h)
+    return e
+
+
+def thin_lut(pass_: int) -> list:
+    o = load()
+    return [o.oracle_thin_lut(pass_, i) for i in range(512)]
+
+
+def downsample(img: np.ndarray) -> np.ndarray:
+    o = load()
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    out = np.zeros((h // 2, w // 2), np.uint8)
+    rc = o.oracle_downsample(img.ctypes.data, w, h, out.ctypes.data, w // 2, h // 2)
+    if rc != 0:
+        raise ValueError("not an exact 2x level")
+    return out

@@ -1,0 +1,88 @@
+"""Front-end parity: HIP kernels through the C ABI against the oracle, bit
+for bit, at every pyramid level (pyramid, derivatives, Canny, thinning, edge
+list, descent links, votes, voter lists, flow lengths, seeds)."""
+import numpy as np
+import pytest
+
+from cctag_amd.detector import Detector
+from tests import frames, oracle_lib
+
+KEYS_EXACT = ["img", "dx", "dy", "edges", "edge_map", "edge_x", "edge_y", "edge_dx",
+              "edge_dy", "before", "after", "chosen", "voters_index", "voters_list",
+              "is_max", "seeds"]
+KEYS_FLOAT = ["total_dist", "flow_length"]
+
+
+def compare_levels(got, ref, where):
+    assert got["width"] == ref["width"] and got["height"] == ref["height"], where
+    assert got["shape"] == ref["shape"], where
+    assert got["n_edges"] == ref["n_edges"], (where, got["n_edges"], ref["n_edges"])
+    for k in KEYS_EXACT:
+        a, b = got[k], ref[k]
+        assert a.shape == b.shape, (where, k, a.shape, b.shape)
+        if not np.array_equal(a, b):
+            bad = np.argwhere(a != b)
+            raise AssertionError("%s %s differs at %d places, first %s: got %s ref %s" % (
+                where, k, len(bad), bad[0], a[tuple(bad[0])], b[tuple(bad[0])]))
+    for k in KEYS_FLOAT:  # bit-exact floats
+        assert np.array_equal(got[k].view(np.int32), ref[k].view(np.int32)), (where, k)
+
+
+def run_parity(img, n_levels=4):
+    ref = oracle_lib.front_end(img, n_levels)
+    h, w = img.shape
+    with Detector(w, h) as det:
+        det.front_end(img)
+        for l in range(n_levels):
+            compare_levels(det.level(l), ref[l], "level %d" % l)
+    return ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", [1, 2])
+def test_front_end_parity_configs(config):
+    img, _ = frames.config_frame(config)
+    print("front-end parity config", config, img.shape, flush=True)
+    ref = run_parity(img)
+    assert ref[0]["n_edges"] > 1000 and len(ref[0]["seeds"]) > 0
+
+
+@pytest.mark.gpu
+def test_front_end_parity_small_markers():
+    img, _ = frames.small_marker_frame()
+    run_parity(img)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["blank", "noise", "checker", "rings", "tiny"])
+def test_front_end_parity_edge_cases(name):
+    img = frames.edge_case(name)
+    print("front-end parity edge case", name, img.shape, flush=True)
+    n_levels = 1 if name == "tiny" else 4
+    if name == "tiny":
+        n_levels = 1
+    ref = oracle_lib.front_end(img, n_levels)
+    h, w = img.shape
+    from cctag_amd import _lib
+    prm = _lib.default_params(3)
+    prm.number_of_processed_multires_layers = n_levels
+    with Detector(w, h, params=prm) as det:
+        det.front_end(img)
+        for l in range(n_levels):
+            compare_levels(det.level(l), ref[l], "%s level %d" % (name, l))
+    if name == "blank":
+        assert ref[0]["n_edges"] == 0
+
+
+@pytest.mark.gpu
+def test_front_end_repeatable():
+    """Two runs on one handle give identical results (union-find and atomics
+    leave no trace in the outputs)."""
+    img, _ = frames.config_frame(1)
+    h, w = img.shape
+    with Detector(w, h) as det:
+        det.front_end(img)
+        a = [det.level(l) for l in range(4)]
+        det.front_end(img)
+        for l in range(4):
+            compare_levels(det.level(l), a[l], "rerun level %d" % l)
