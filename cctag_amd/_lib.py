@@ -78,6 +78,21 @@ class LevelView(ctypes.Structure):
     ]
 
 
+class Marker(ctypes.Structure):
+    """cctagx_marker (ICCTag.hpp:28-82 plus quality / level / homography)."""
+    _fields_ = [
+        ("x", c_f32), ("y", c_f32), ("id", c_i32), ("status", c_i32), ("quality", c_f32),
+        ("pyramid_level", c_i32), ("scale", c_f32), ("ellipse", c_f32 * 5),
+        ("homography", c_f32 * 9),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"x": self.x, "y": self.y, "id": self.id, "status": self.status,
+                "quality": self.quality, "pyramid_level": self.pyramid_level,
+                "scale": self.scale, "ellipse": list(self.ellipse),
+                "homography": list(self.homography)}
+
+
 def _arr(p, n, dtype):
     if n == 0 or not p:
         return np.zeros(0, dtype=dtype)

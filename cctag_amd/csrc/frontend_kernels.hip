@@ -34,6 +34,11 @@ static_assert(sizeof(PlaneDesc) == 12 * sizeof(int32_t), "PlaneDesc layout");
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Correctly rounded fp32 square root.  On gfx950 the library's __fsqrt_rn
+// lowers to a bare v_sqrt_f32 (not correctly rounded); sqrtf under the
+// default correctly-rounded-sqrt mode adds the FMA correction step.
+__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+
 // ---------------------------------------------------------------------------
 // a1: pyramid (Level.cpp:67).  cv::resize to exactly half size on 8U runs
 // OpenCV's fast 2x2 area path: vector lanes (the first floor(w/8)*8 outputs
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(256) void k_filter(const PlaneDesc* __restrict__ P,
 // decide which strong pixels seed its flood fill, not which pixels it
 // reaches, so the final map is: candidates 8-connected to a strong pixel.
 __device__ __forceinline__ int magnitude(int x, int y) {
-  return (int)rintf(__fsqrt_rn((float)x * (float)x + (float)y * (float)y));
+  return (int)rintf(sqrt_rn((float)x * (float)x + (float)y * (float)y));
 }
 
 __global__ __launch_bounds__(256) void k_nms(const PlaneDesc* __restrict__ P, int np,
@@ -652,7 +657,7 @@ __device__ __forceinline__ float neg_dot(const int16_t* gx, const int16_t* gy, i
 }
 __device__ __forceinline__ float dist2d(const int16_t* ex, const int16_t* ey, int a, int b) {
   const int ddx = ex[b] - ex[a], ddy = ey[b] - ey[a];
-  return __fsqrt_rn(__fadd_rn((float)(ddx * ddx), (float)(ddy * ddy)));
+  return sqrt_rn(__fadd_rn((float)(ddx * ddx), (float)(ddy * ddy)));
 }
 
 __global__ __launch_bounds__(256) void k_vote(const PlaneDesc* __restrict__ P,

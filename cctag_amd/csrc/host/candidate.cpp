@@ -1,0 +1,874 @@
+// Candidate stage (see candidate.hpp).  Serial restatement: the reference
+// runs three tbb::parallel_for loops here (Detection.cpp:616,644,680); run in
+// index order with one rand_5_k stream they give the result of its
+// CCTAG_SERIALIZE build (Detection.cpp:618,646,682).
+#include "candidate.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <list>
+#include <memory>
+#include <unordered_set>
+
+#ifdef CCTAGX_TRACE
+#include <cstdio>
+#define TRACE(...) std::fprintf(stderr, __VA_ARGS__)
+#else
+#define TRACE(...) ((void)0)
+#endif
+
+namespace cctagx {
+
+namespace {
+constexpr float kPi = 3.14159265358979323846f;
+constexpr int kEdgeNotFound = -1, kConvexityLost = -2;
+constexpr int kNoWeight = 0, kInvGradWeight = 1;
+
+inline uint32_t packxy(int x, int y) { return ((uint32_t)x << 16) | ((uint32_t)y & 0xFFFFu); }
+
+inline float dist2d(float ax, float ay, float bx, float by) {
+  const float dx = bx - ax, dy = by - ay;
+  return std::sqrt(dx * dx + dy * dy);
+}
+
+inline Pt2 pt_of(const EdgePoint& e) { return Pt2((float)e.x, (float)e.y, e.gx, e.gy); }
+}  // namespace
+
+void EdgeSet::load(const LevelEdges& L) {
+  w_ = L.w;
+  h_ = L.h;
+  shape_w_ = L.shape_w;
+  shape_h_ = L.shape_h;
+  map_ = L.map;
+  before_ = L.before;
+  after_ = L.after;
+  vindex_ = L.vindex;
+  vlist_ = L.vlist;
+  pts.resize(L.n);
+  for (int i = 0; i < L.n; ++i) {
+    EdgePoint& e = pts[i];
+    e.x = L.x[i];
+    e.y = L.y[i];
+    e.gx = (float)L.gx[i];
+    e.gy = (float)L.gy[i];
+    e.norm = std::sqrt(e.gx * e.gx + e.gy * e.gy);
+    e.flow_length = L.flow[i];
+    e.is_max = L.ismax[i];
+    e.processed = 0;
+    e.n_segment_out = -1;
+  }
+  pin_.assign(L.n, 0);
+  paux_.assign(L.n, 0);
+  seeds.assign(L.seeds, L.seeds + L.n_seeds);
+}
+
+Marker Marker::make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int level,
+                    float scale, float quality) {
+  Marker m;
+  m.id = -1;
+  m.center_img = Pt2(outer.cx(), outer.cy());
+  m.outer_ellipse = outer;
+  m.points = std::move(pts);
+  m.homography = Mat3::zero();
+  m.quality = quality;
+  m.pyramid_level = level;
+  m.scale = scale;
+  m.outer_ellipse.set_center(m.outer_ellipse.cx() + 0.5f, m.outer_ellipse.cy() + 0.5f);
+  m.rescaled_outer_ellipse = scaled(m.outer_ellipse, scale);
+  m.status = 0;
+  return m;
+}
+
+bool Marker::is_equal(const Marker& other) const {
+  Ellipse a = rescaled_outer_ellipse;
+  a.set_a(a.b() * 0.5f);
+  a.set_b(a.b() * 0.5f);
+  Ellipse b = other.rescaled_outer_ellipse;
+  b.set_a(b.b() * 0.5f);
+  b.set_b(b.b() * 0.5f);
+  return is_overlapping_ellipses(a, b);
+}
+
+// ---------------------------------------------------------------------------
+// Vote.cpp:255-394: convex edge linking from a seed in both directions.
+static void edge_linking_dir(EdgeSet& es, std::unordered_set<uint32_t>& processed, int p,
+                             int dir, std::deque<int>& seg, size_t window, float average_vote_min) {
+  std::deque<float> phi;
+  size_t i = 0;
+  bool found = true;
+  int stop = 0;
+  const size_t maxLength = 100;
+  float averageVote = (float)es.voters_size(p);
+  static const int xoff[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+  const int yoff[8] = {0, dir * -1, dir * -1, dir * -1, 0, dir * 1, dir * 1, dir * 1};
+  while ((i < maxLength) && found && (averageVote >= average_vote_min)) {
+    const EdgePoint& ep = es.pts[p];
+    const float angle = std::fmod(std::atan2(ep.gy, ep.gx) + 2.0f * kPi, 2.0f * kPi);
+    phi.push_back(angle);
+    if (phi.size() > window) phi.pop_front();
+    const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
+    int j = 0;
+    stop = 0;
+    while (!stop) {
+      if (j >= 8) {
+        stop = kEdgeNotFound;
+      } else {
+        const int k = dir == 1 ? (8 - shifting + j) % 8 : (shifting + j) % 8;
+        const int sx = es.pts[p].x + xoff[k];
+        const int sy = es.pts[p].y + yoff[k];
+        if (sx >= 0 && sx < es.shape_w() && sy >= 0 && sy < es.shape_h() && es.at(sx, sy) >= 0 &&
+            processed.find(packxy(sx, sy)) == processed.end()) {
+          if (phi.size() == window) {
+            if (dir * std::sin(phi.back() - phi.front()) < 0.0f) stop = kConvexityLost;
+          } else {
+            const float s = dir * std::sin(phi.back() - phi.front());
+            const float c = std::cos(phi.back() - phi.front());
+            if (((s < -0.707f) && (c > 0.f)) || ((s < 0.f) && (c < 0.f))) stop = kConvexityLost;
+          }
+          if (stop == 0) {
+            processed.insert(packxy(es.pts[p].x, es.pts[p].y));
+            p = es.at(sx, sy);
+            if (dir > 0) seg.push_back(p);
+            else seg.push_front(p);
+            averageVote = (averageVote * seg.size() + es.voters_size(p)) / (seg.size() + 1.f);
+            stop = 1;
+          }
+          processed.insert(packxy(es.pts[p].x, es.pts[p].y));
+        }
+      }
+      ++j;
+    }
+    found = stop == 1;
+    ++i;
+  }
+  if ((i == maxLength) || (stop == kConvexityLost)) {
+    if (seg.size() > window) {
+      size_t n = 0;
+      for (int q : seg) {
+        if (n == seg.size() - window) break;
+        es.set_processed_in(q, true);
+        ++n;
+      }
+    }
+  } else if (stop == kEdgeNotFound) {
+    for (int q : seg) es.set_processed_in(q, true);
+  }
+}
+
+static void edge_linking(EdgeSet& es, std::deque<int>& seg, int pmax, size_t window,
+                         float average_vote_min) {
+  std::unordered_set<uint32_t> processed;
+  if (pmax >= 0) {
+    seg.push_back(pmax);
+    es.set_processed_in(pmax, true);
+    processed.insert(packxy(es.pts[pmax].x, es.pts[pmax].y));
+    edge_linking_dir(es, processed, pmax, 1, seg, window, average_vote_min);
+    edge_linking_dir(es, processed, pmax, -1, seg, window, average_vote_min);
+  }
+}
+
+// Vote.cpp:396-412.
+static void children_of(const EdgeSet& es, const std::deque<int>& edges, std::vector<int>& children) {
+  size_t voteMax = 1;
+  for (int e : edges) voteMax = std::max((int)voteMax, es.voters_size(e));
+  for (int e : edges) {
+    const int n = es.voters_size(e);
+    if (n > 0 && (size_t)n >= voteMax / 14)
+      for (const int32_t* v = es.voters_begin(e); v != es.voters_end(e); ++v) children.push_back(*v);
+  }
+}
+
+// 5x5 partial-pivot LU solve (PartialPivLU: unblocked elimination, first
+// largest pivot, column-oriented triangular solves).  Returns false when a
+// pivot is zero (determinant zero).
+static bool lu_solve5(float A[5][5], float b[5], float x[5]) {
+  int perm[5] = {0, 1, 2, 3, 4};
+  int ntrans = 0;
+  bool singular = false;
+  for (int k = 0; k < 5; ++k) {
+    int piv = k;
+    float best = std::fabs(A[k][k]);
+    for (int i = k + 1; i < 5; ++i)
+      if (std::fabs(A[i][k]) > best) {
+        best = std::fabs(A[i][k]);
+        piv = i;
+      }
+    if (best != 0.f) {
+      if (piv != k) {
+        for (int j = 0; j < 5; ++j) std::swap(A[k][j], A[piv][j]);
+        std::swap(perm[k], perm[piv]);
+        ++ntrans;
+      }
+      for (int i = k + 1; i < 5; ++i) A[i][k] /= A[k][k];
+    } else {
+      singular = true;
+    }
+    for (int i = k + 1; i < 5; ++i)
+      for (int j = k + 1; j < 5; ++j) A[i][j] -= A[i][k] * A[k][j];
+  }
+  const float det = (A[0][0] * A[1][1]) * (A[2][2] * (A[3][3] * A[4][4])) * (ntrans % 2 ? -1.f : 1.f);
+  if (singular || det == 0.f) return false;
+  for (int i = 0; i < 5; ++i) x[i] = b[perm[i]];
+  for (int i = 0; i < 5; ++i)
+    for (int s = i + 1; s < 5; ++s) x[s] -= x[i] * A[s][i];
+  for (int i = 4; i >= 0; --i) {
+    x[i] /= A[i][i];
+    for (int s = 0; s < i; ++s) x[s] -= x[i] * A[s][i];
+  }
+  return true;
+}
+
+// Vote.cpp:414-576: RANSAC-like ellipse fit over a subsample, then filter.
+static void outlier_removal(const EdgeSet& es, const std::vector<int>& children,
+                            std::vector<int>& filtered, float& SmFinal, float threshold,
+                            int weightedType, size_t maxSize, Pcg32& rng) {
+  filtered.reserve(children.size());
+  const size_t nSub = std::min(children.size(), maxSize);
+  const float step = (float)children.size() / (float)nSub;
+  if (nSub < 5) return;
+  Ellipse qm;
+  float Sm = 10000000.0f;
+  const float f = 1.f;
+  std::vector<Pt2> pts;
+  std::vector<float> weights;
+  size_t k = 0, iEdge = 0;
+  for (int c : children) {
+    if (iEdge == (size_t)(k * step)) {
+      ++k;
+      const EdgePoint& e = es.pts[c];
+      pts.emplace_back((float)e.x, (float)e.y);
+      if (weightedType == kInvGradWeight) weights.push_back(255 / e.norm);
+    }
+    ++iEdge;
+  }
+  size_t counter = 0, guard = 0;
+  std::vector<float> dist;
+  while (counter < 70 && guard++ < 100000) {
+    int perm[5];
+    rng.rand_5_k(perm, pts.size());
+    float A[5][5], b[5], t[5];
+    for (int i = 0; i < 5; ++i) {
+      const float x = pts[perm[i]].x, y = pts[perm[i]].y;
+      A[i][0] = x * x;
+      A[i][1] = 2.0f * x * y;
+      A[i][2] = y * y;
+      A[i][3] = 2.0f * f * x;
+      A[i][4] = 2.0f * f * y;
+      b[i] = -f * f;
+    }
+    if (!lu_solve5(A, b, t)) {
+      ++counter;
+      continue;
+    }
+    if (!(t[0] * t[2] - t[1] * t[1] > 0)) {
+      ++counter;
+      continue;
+    }
+    Mat3 Q = Mat3::zero();
+    Q.m[0][0] = t[0];
+    Q.m[0][1] = Q.m[1][0] = t[1];
+    Q.m[0][2] = Q.m[2][0] = t[3];
+    Q.m[1][1] = t[2];
+    Q.m[1][2] = Q.m[2][1] = t[4];
+    Q.m[2][2] = 1.f;
+    try {
+      Ellipse q(Q);
+      const float ratio = q.a() / q.b();
+      if ((ratio < 0.04f) || (ratio > 25)) {
+        ++counter;
+        continue;
+      }
+      distance_points_ellipse(dist, pts, q);
+      if (weightedType != kNoWeight)
+        for (size_t i = 0; i < dist.size(); ++i) dist[i] = dist[i] * weights[i];
+      const float S = median_ref(dist);
+      if (S < Sm) {
+        counter = 0;
+        qm = Ellipse(Q);
+        Sm = S;
+      } else {
+        ++counter;
+      }
+    } catch (...) {
+    }
+  }
+  std::vector<float> vDistFinal;
+  vDistFinal.reserve(children.size());
+  for (int c : children) {
+    const EdgePoint& e = es.pts[c];
+    float d = std::numeric_limits<float>::max();
+    if (weightedType == kNoWeight) d = distance_point_ellipse((float)e.x, (float)e.y, qm);
+    else if (weightedType == kInvGradWeight)
+      d = distance_point_ellipse((float)e.x, (float)e.y, qm) * 255 / e.norm;
+    if (d < threshold * Sm) {
+      filtered.push_back(c);
+      vDistFinal.push_back(d);
+    }
+  }
+  if (vDistFinal.empty()) return;
+  SmFinal = median_ref(vDistFinal);
+}
+
+// Fitting.cpp:228-316.
+static float inner_prod_min(const EdgeSet& es, const std::vector<int>& fc, float thr, int& p1,
+                            int& p2) {
+  int pAngle1 = -1;
+  float mn = 1.1f, distMax = 0.f;
+  const EdgePoint& p0 = es.pts[fc.front()];
+  float normGrad = std::sqrt(p0.gx * p0.gx + p0.gy * p0.gy);
+  const float gx0 = p0.gx / normGrad, gy0 = p0.gy / normGrad;
+  for (size_t i = 1; i < fc.size(); ++i) {
+    const EdgePoint& c = es.pts[fc[i]];
+    normGrad = std::sqrt(c.gx * c.gx + c.gy * c.gy);
+    const float gx = c.gx / normGrad, gy = c.gy / normGrad;
+    const float ip = gx0 * gx + gy0 * gy;
+    if (ip <= thr) return ip;
+    if (ip < mn) {
+      mn = ip;
+      pAngle1 = fc[i];
+    }
+    const float d = dist2d((float)p0.x, (float)p0.y, (float)c.x, (float)c.y);
+    if (d > distMax) {
+      distMax = d;
+      p1 = fc[i];
+    }
+  }
+  const EdgePoint& pa = es.pts[pAngle1];
+  normGrad = std::sqrt(pa.gx * pa.gx + pa.gy * pa.gy);
+  const float gxmin = pa.gx / normGrad, gymin = pa.gy / normGrad;
+  mn = 1.f;
+  distMax = 0.f;
+  for (size_t i = 0; i < fc.size(); ++i) {
+    const EdgePoint& c = es.pts[fc[i]];
+    normGrad = std::sqrt(c.gx * c.gx + c.gy * c.gy);
+    const float chgx = c.gx / normGrad, chgy = c.gy / normGrad;
+    const float ip = gxmin * chgx + gymin * chgy;
+    if (ip <= thr) return ip;
+    if (ip < mn) mn = ip;
+    const EdgePoint& q1 = es.pts[p1];
+    const float d = dist2d((float)q1.x, (float)q1.y, (float)c.x, (float)c.y);
+    if (d > distMax) {
+      distMax = d;
+      p2 = fc[i];
+    }
+  }
+  return mn;
+}
+
+static std::vector<Pt2> as_points(const EdgeSet& es, const std::vector<int>& idx) {
+  std::vector<Pt2> v;
+  v.reserve(idx.size());
+  for (int i : idx) v.push_back(pt_of(es.pts[i]));
+  return v;
+}
+
+// EllipseGrowing.cpp:302-325.
+static bool ellipse_growing_init(const EdgeSet& es, const std::vector<int>& fc, Ellipse& e) {
+  int p1 = fc.front(), p2 = fc.front();
+  const float mn = inner_prod_min(es, fc, 0.25f, p1, p2);
+  if (mn <= 0.25f) {
+    fit_ellipse(as_points(es, fc), e);
+    return true;
+  }
+  fit_circle(as_points(es, fc), e);
+  return false;
+}
+
+// EllipseGrowing.cpp:327-368, with an explicit stack in the recursion's order.
+static void connected_point(std::vector<int>& pts, uint64_t mask, EdgeSet& es, const Ellipse& qin,
+                            const Ellipse& qout, int x0, int y0) {
+  static const int xoff[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+  static const int yoff[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+  struct Frame {
+    int x, y, i;
+  };
+  std::vector<Frame> stack;
+  es.pts[es.at(x0, y0)].processed |= mask;
+  stack.push_back({x0, y0, 0});
+  while (!stack.empty()) {
+    Frame& f = stack.back();
+    if (f.i >= 8) {
+      stack.pop_back();
+      continue;
+    }
+    const int sx = f.x + xoff[f.i], sy = f.y + yoff[f.i];
+    ++f.i;
+    if (!(sx >= 0 && sx < es.shape_w() && sy >= 0 && sy < es.shape_h())) continue;
+    const int ie = es.at(sx, sy);
+    if (ie < 0) continue;
+    EdgePoint& e = es.pts[ie];
+    if (!is_in_hull(qin, qout, (float)e.x, (float)e.y) || (e.processed & mask)) continue;
+    const float ex = qin.cx() - e.x, ey = qin.cy() - e.y;
+    if (e.gx * ex + e.gy * ey < 0) {
+      pts.push_back(ie);
+      e.processed |= mask;
+      stack.push_back({sx, sy, 0});
+    }
+  }
+}
+
+// EllipseGrowing.cpp:386-402.
+static void ellipse_hull(EdgeSet& es, std::vector<int>& pts, const Ellipse& e, float delta,
+                         uint64_t mask) {
+  Ellipse qin, qout;
+  compute_hull(e, delta, qin, qout);
+  const size_t n0 = pts.size();
+  for (size_t i = 0; i < n0; ++i) {
+    const EdgePoint& p = es.pts[pts[i]];
+    connected_point(pts, mask, es, qin, qout, p.x, p.y);
+  }
+}
+
+static int count_in_hull(const EdgeSet& es, const std::vector<int>& pts, const Ellipse& e,
+                         float delta) {
+  Ellipse qin, qout;
+  compute_hull(e, delta, qin, qout);
+  int n = 0;
+  for (int i : pts)
+    if (is_in_hull(qin, qout, (float)es.pts[i].x, (float)es.pts[i].y)) ++n;
+  return n;
+}
+
+// EllipseGrowing.cpp:404-510.
+static void ellipse_growing2(EdgeSet& es, const std::vector<int>& fc, std::vector<int>& outer,
+                             Ellipse& ellipse, float width, size_t run_id, bool good_init) {
+  const uint64_t mask = (uint64_t)1 << run_id;
+  outer.reserve(fc.size() * 3);
+  for (int c : fc) {
+    outer.push_back(c);
+    es.pts[c].processed |= mask;
+  }
+  int lastSize = 0;
+  int nIter = 0;
+  if (!good_init) {
+    int newSize = (int)outer.size();
+    int maxNb = newSize;
+    int nIterMax = 0;
+    std::vector<std::vector<int>> sets;
+    std::vector<Ellipse> ellipses;
+    sets.push_back(outer);
+    ellipses.push_back(ellipse);
+    ++nIter;
+    while (newSize - lastSize > 0) {
+      lastSize = count_in_hull(es, outer, ellipse, width);
+      ellipse_hull(es, outer, ellipse, width, mask);
+      sets.push_back(outer);
+      ellipses.push_back(ellipse);
+      fit_circle(as_points(es, outer), ellipse);
+      newSize = count_in_hull(es, outer, ellipse, width);
+      if (newSize > maxNb) {
+        maxNb = newSize;
+        nIterMax = nIter;
+      }
+      ++nIter;
+    }
+    outer = sets[nIterMax];
+    ellipse = ellipses[nIterMax];
+    for (auto& s : sets)
+      for (int i : s) es.pts[i].processed &= ~mask;
+    for (int i : outer) es.pts[i].processed |= mask;
+  }
+  lastSize = 0;
+  fit_ellipse(as_points(es, outer), ellipse);
+  while ((int)outer.size() - lastSize > 0) {
+    lastSize = (int)outer.size();
+    ellipse_hull(es, outer, ellipse, width, mask);
+    fit_ellipse(as_points(es, outer), ellipse);
+  }
+}
+
+// EllipseGrowing.cpp:80-203.
+static bool add_candidate_flow(EdgeSet& es, const std::vector<int>& fc,
+                               const std::vector<int>& outer_pts, const Ellipse& outer,
+                               std::vector<std::vector<Pt2>>& cctagPoints, size_t numCircles) {
+  cctagPoints.resize(numCircles);
+  for (int e : outer_pts) cctagPoints.back().push_back(pt_of(es.pts[e]));
+  std::vector<int> processed;
+  size_t nGradientOut = 0, nAddedPoint = 0;
+  auto release = [&]() {
+    for (int q : processed) es.set_processed_aux(q, false);
+  };
+  for (int p0 : fc) {
+    int dir = -1;
+    int p = p0;
+    const Pt2 outerPoint((float)es.pts[p0].x, (float)es.pts[p0].y);
+    const float a = outerPoint.x - outer.cx();
+    const float b = outerPoint.y - outer.cy();
+    const Vec3 line{{a, b, -a * outer.cx() - b * outer.cy()}};
+    for (size_t j = 1; j < numCircles; ++j) {
+      p = dir == -1 ? es.before(p) : es.after(p);
+      // A missing link throws out of here with the flags still set, as the
+      // reference's test_processed_aux(nullptr) does (Types.hpp:150-155).
+      if (!es.test_processed_aux(p)) {
+        es.set_processed_aux(p, true);
+        processed.push_back(p);
+        const EdgePoint& ep = es.pts[p];
+        const float normGrad = std::sqrt(ep.gx * ep.gx + ep.gy * ep.gy);
+        const float ge0 = ep.gx / normGrad, ge1 = ep.gy / normGrad;
+        float t0 = outer.cx() - ep.x, t1 = outer.cy() - ep.y;
+        const float dc = std::sqrt(t0 * t0 + t1 * t1);
+        t0 /= dc;
+        t1 /= dc;
+        const Pt2 add = pt_of(ep);
+        if (is_in_ellipse(outer, add.x, add.y) && is_on_the_same_side(outerPoint, add, line)) {
+          if ((float(-dir) * (ge0 * t0 + ge1 * t1) < -0.5f) && (j >= numCircles - 2)) ++nGradientOut;
+          cctagPoints[numCircles - j - 1].push_back(add);
+          if (j >= numCircles - 2) ++nAddedPoint;
+        } else {
+          cctagPoints.clear();
+          release();
+          return false;
+        }
+      }
+      dir = -dir;
+    }
+  }
+  release();
+  if (float(nGradientOut) / float(nAddedPoint) > 0.5f) {
+    cctagPoints.clear();
+    return false;
+  }
+  return true;
+}
+
+namespace {
+struct Candidate {
+  int seed = -1;
+  std::deque<int> convex_segment;
+  std::vector<int> outer_points;
+  Ellipse outer;
+  std::vector<int> filtered_children;
+  int score = 0;
+  size_t label = 0;
+  float average_received_vote = 0.f;
+};
+}  // namespace
+
+// Vote.cpp:578-742.
+static bool is_another_segment(EdgeSet& es, Ellipse& outerEllipse, std::vector<int>& outerPts,
+                               const Candidate& another,
+                               std::vector<std::vector<Pt2>>& cctagPoints, size_t numCircles,
+                               float thrMedian, Pcg32& rng) {
+  const std::vector<int>& anotherOuter = another.outer_points;
+  Ellipse qm;
+  float Sm = std::numeric_limits<float>::max();
+  const std::vector<Pt2> pts = as_points(es, outerPts);
+  const std::vector<Pt2> anotherPts = as_points(es, anotherOuter);
+  std::vector<float> distRef;
+  distance_points_ellipse(distRef, pts, outerEllipse);
+  const float SRef = median_ref(distRef);
+  size_t cnt = 0;
+  std::vector<float> dist, anotherDist;
+  while (cnt < 100) {
+    std::vector<Pt2> points;
+    int perm[5];
+    rng.rand_5_k(perm, outerPts.size());
+    for (int i = 0; i < 4; ++i) points.emplace_back(pts[perm[i]].x, pts[perm[i]].y);
+    rng.rand_5_k(perm, anotherOuter.size());
+    for (int i = 0; i < 4; ++i)
+      points.emplace_back((float)es.pts[anotherOuter[perm[i]]].x, (float)es.pts[anotherOuter[perm[i]]].y);
+    try {
+      Ellipse eToto;
+      fit_ellipse(points, eToto);
+      Ellipse q(eToto.matrix());
+      const float ratio = q.a() / q.b();
+      if ((ratio < 0.12) || (ratio > 8)) {
+        ++cnt;
+        continue;
+      }
+      distance_points_ellipse(dist, pts, q);
+      const float S1 = median_ref(dist);
+      distance_points_ellipse(anotherDist, anotherPts, q);
+      const float S2 = median_ref(anotherDist);
+      const float S = S1 + S2;
+      if (S < Sm) {
+        cnt = 0;
+        qm = q;
+        Sm = S;
+      } else {
+        ++cnt;
+      }
+    } catch (...) {
+      ++cnt;
+    }
+  }
+  const float thr = 6;
+  if (!(Sm < thr * SRef)) return false;
+  std::vector<int> tmpPts = outerPts;
+  tmpPts.insert(tmpPts.end(), anotherOuter.begin(), anotherOuter.end());
+  Ellipse tmpEllipse;
+  fit_ellipse(as_points(es, tmpPts), tmpEllipse);
+  const float quality = (float)tmpPts.size() / (float)rasterize_ellipse_perimeter(tmpEllipse);
+  if (!(quality < 1.1)) return false;
+  std::vector<float> vDistFinal;
+  for (int p : tmpPts) vDistFinal.push_back(distance_point_ellipse((float)es.pts[p].x, (float)es.pts[p].y, tmpEllipse));
+  const float SmFinal = median_ref(vDistFinal);
+  if (SmFinal < thrMedian) {
+    if (add_candidate_flow(es, another.filtered_children, anotherOuter, tmpEllipse, cctagPoints,
+                           numCircles)) {
+      outerPts = tmpPts;
+      outerEllipse = tmpEllipse;
+      return true;
+    }
+    return false;
+  }
+  return false;
+}
+
+// Detection.cpp:69-113.
+static void construct_flow_component(int seed, EdgeSet& es,
+                                     std::vector<std::unique_ptr<Candidate>>& loopOne,
+                                     const HostParams& p) {
+  if (es.test_processed_in(seed)) return;
+  auto cand = std::make_unique<Candidate>();
+  cand->seed = seed;
+  edge_linking(es, cand->convex_segment, seed, p.window_size_on_inner_elliptic_segment,
+               p.average_vote_min);
+  int nReceivedVote = 0, nVotedPoints = 0;
+  for (int q : cand->convex_segment) {
+    const int vs = es.voters_size(q);
+    nReceivedVote += vs;
+    if (vs > 0) ++nVotedPoints;
+  }
+  cand->average_received_vote = (float)(nReceivedVote * nReceivedVote) / (float)nVotedPoints;
+  auto it = std::lower_bound(loopOne.begin(), loopOne.end(), cand,
+                             [](const std::unique_ptr<Candidate>& a, const std::unique_ptr<Candidate>& b) {
+                               return a->average_received_vote > b->average_received_vote;
+                             });
+  loopOne.insert(it, std::move(cand));
+}
+
+// Detection.cpp:115-267.
+static void complete_flow_component(Candidate& cand, EdgeSet& es, std::vector<Candidate>& loopTwo,
+                                    size_t& nSegmentOut, size_t run_id, const HostParams& p,
+                                    Pcg32& rng) {
+  try {
+    std::vector<int> children;
+    children_of(es, cand.convex_segment, children);
+    TRACE("  loop-one seed (%d,%d) seg %zu avg %.1f children %zu\n", es.pts[cand.seed].x, es.pts[cand.seed].y,
+          cand.convex_segment.size(), cand.average_received_vote, children.size());
+    if (children.size() < (size_t)p.min_points_segment_candidate) return;
+    cand.score = (int)children.size();
+    float SmFinal = 1e+10f;
+    std::vector<int>& filtered = cand.filtered_children;
+    outlier_removal(es, children, filtered, SmFinal, p.thresh_robust_estimation_of_outer_ellipse,
+                    kInvGradWeight, 60, rng);
+    TRACE("    filtered %zu SmFinal %g\n", filtered.size(), SmFinal);
+    if (filtered.size() < 5) return;
+    size_t nLabel = (size_t)-1;
+    {
+      long nSegmentCommon = -1;
+      for (int q : filtered)
+        if (es.pts[q].n_segment_out != -1) {
+          nSegmentCommon = es.pts[q].n_segment_out;
+          break;
+        }
+      if (nSegmentCommon == -1) nLabel = nSegmentOut++;
+      else nLabel = (size_t)nSegmentCommon;
+      for (int q : filtered) es.pts[q].n_segment_out = (int32_t)nLabel;
+    }
+    std::vector<int>& outer = cand.outer_points;
+    Ellipse& outerEllipse = cand.outer;
+    const bool goodInit = ellipse_growing_init(es, filtered, outerEllipse);
+    ellipse_growing2(es, filtered, outer, outerEllipse, p.ellipse_growing_elliptic_hull_width,
+                     run_id, goodInit);
+    cand.label = nLabel;
+    std::vector<float> vDist;
+    for (int q : outer) vDist.push_back(distance_point_ellipse((float)es.pts[q].x, (float)es.pts[q].y, outerEllipse));
+    SmFinal = median_ref(vDist);
+    TRACE("    grown c=(%.1f,%.1f) a=%.1f b=%.1f n=%zu Sm %g quality %g\n", outerEllipse.cx(), outerEllipse.cy(),
+          outerEllipse.a(), outerEllipse.b(), outer.size(), SmFinal,
+          (float)outer.size() / (float)rasterize_ellipse_perimeter(outerEllipse));
+    if (SmFinal > p.thr_median_distance_ellipse) return;
+    const float quality = (float)outer.size() / (float)rasterize_ellipse_perimeter(outerEllipse);
+    if (quality > 1.1) return;
+    const float ratio = outerEllipse.a() / outerEllipse.b();
+    if ((ratio < 0.05) || (ratio > 20)) return;
+    loopTwo.push_back(cand);
+  } catch (const std::exception& ex) {
+    TRACE("    exception %s\n", ex.what());
+  }
+}
+
+// Detection.cpp:278-377.
+static void flow_component_assembling(EdgeSet& es, float& quality, size_t ic,
+                                      const std::vector<Candidate>& loopTwo, Ellipse& outerEllipse,
+                                      std::vector<int>& outerPts,
+                                      std::vector<std::vector<Pt2>>& cctagPoints,
+                                      const HostParams& p, Pcg32& rng) {
+  const Candidate& cand = loopTwo[ic];
+  int score = -1;
+  size_t iMax = 0;
+  const float ratioExpension = 2.5f;
+  const EdgePoint& s = es.pts[cand.seed];
+  const Ellipse circle((float)s.x, (float)s.y, s.flow_length * ratioExpension,
+                       s.flow_length * ratioExpension, 0.f);
+  for (size_t i = 0; i < loopTwo.size(); ++i) {
+    if (i == ic) continue;
+    const Candidate& another = loopTwo[i];
+    if (cand.label == another.label) continue;
+    const float r = es.pts[another.seed].flow_length / s.flow_length;
+    if (!((r > 0.666) && (r < 1.5))) continue;
+    if (is_in_ellipse(circle, (float)es.pts[another.seed].x, (float)es.pts[another.seed].y) &&
+        another.score > score) {
+      score = another.score;
+      iMax = i;
+    }
+  }
+  if (score > 0) {
+    const Candidate& sel = loopTwo[iMax];
+    if (is_another_segment(es, outerEllipse, outerPts, sel, cctagPoints, p.n_crowns * 2,
+                           p.thr_median_distance_ellipse, rng))
+      quality = (float)outerPts.size() / (float)rasterize_ellipse_perimeter(outerEllipse);
+  }
+}
+
+// Detection.cpp:379-566.
+static void loop_two_iteration(std::vector<Marker>& markers, EdgeSet& es,
+                               const std::vector<Candidate>& loopTwo, size_t ic, int level,
+                               float scale, const HostParams& p, Pcg32& rng) {
+  const Candidate& cand = loopTwo[ic];
+  std::vector<int> outerPts = cand.outer_points;
+  Ellipse outerEllipse = cand.outer;
+  std::vector<std::vector<Pt2>> cctagPoints;
+  try {
+    float quality = (float)outerPts.size() / (float)rasterize_ellipse_perimeter(outerEllipse);
+    if (p.search_for_another_segment && (quality > 0.25) && (quality < 0.7))
+      flow_component_assembling(es, quality, ic, loopTwo, outerEllipse, outerPts, cctagPoints, p, rng);
+    if (!add_candidate_flow(es, cand.filtered_children, cand.outer_points, outerEllipse,
+                            cctagPoints, p.n_crowns * 2))
+      return;
+    const Ellipse rescaleEllipse(outerEllipse.cx(), outerEllipse.cy(), outerEllipse.a() * scale,
+                                 outerEllipse.b() * scale, outerEllipse.angle());
+    const int realPixelPerimeter = rasterize_ellipse_perimeter(rescaleEllipse);
+    const float rs = quality * realPixelPerimeter;
+    if (((quality <= 0.35) && (rs >= 300.0)) ||
+        ((quality <= 0.45f) && (rs >= 200.0) && (rs < 300.0)) ||
+        ((quality <= 0.5f) && (rs >= 100.0) && (rs < 200.0)) ||
+        ((quality <= 0.5f) && (rs >= 70.0) && (rs < 100.0)) ||
+        ((quality <= 0.96f) && (rs >= 50.0) && (rs < 70.0)) || (rs < 50.0))
+      return;
+    const float ratio = outerEllipse.a() / outerEllipse.b();
+    if (ratio > 8.0 || ratio < 0.125) return;
+    Ellipse qin, qout;
+    compute_hull(outerEllipse, 3.6f, qin, qout);
+    for (int q : outerPts)
+      if (!is_in_hull(qin, qout, (float)es.pts[q].x, (float)es.pts[q].y)) return;
+    float quality2 = 0;
+    for (int q : outerPts) quality2 += es.pts[q].norm;
+    quality2 *= scale;
+    markers.push_back(Marker::make(outerEllipse, cctagPoints, level, scale, quality2));
+  } catch (...) {
+  }
+}
+
+void detection_from_edges(std::vector<Marker>& markers, EdgeSet& es, int level_h,
+                          int pyramid_level, float scale, const HostParams& p, Pcg32& rng) {
+  size_t nSegmentOut = 0;
+  if (es.seeds.empty()) return;
+  // Multiresolution.cpp:239-242: seeds by received votes, std::sort (its
+  // order among equal votes is the library's; the input order is the order in
+  // which the points reached min votes, as on the CPU).
+  if (es.seeds.size() > 1)
+    std::sort(es.seeds.begin(), es.seeds.end(),
+              [&](int a, int b) { return es.pts[a].is_max > es.pts[b].is_max; });
+  const size_t nMaxSeeds = (size_t)std::max(level_h / 2, p.maximum_nb_seeds);
+  const size_t nSeeds = std::min(es.seeds.size(), nMaxSeeds);
+  std::vector<std::unique_ptr<Candidate>> loopOne;
+  for (size_t i = 0; i < nSeeds; ++i) construct_flow_component(es.seeds[i], es, loopOne, p);
+  TRACE("level %d: %zu seeds, %zu loop-one candidates\n", pyramid_level, es.seeds.size(), loopOne.size());
+  const size_t nLoopTwo = std::min(loopOne.size(), (size_t)p.maximum_nb_candidates_loop_two);
+  std::vector<Candidate> loopTwo;
+  loopTwo.reserve(nLoopTwo);
+  for (size_t i = 0; i < nLoopTwo; ++i)
+    complete_flow_component(*loopOne[i], es, loopTwo, nSegmentOut, i, p, rng);
+  TRACE("level %d: %zu loop-two candidates\n", pyramid_level, loopTwo.size());
+  for (size_t i = 0; i < loopTwo.size(); ++i) {
+    TRACE("  cand %zu seed (%d,%d) ellipse c=(%.1f,%.1f) a=%.1f b=%.1f n=%zu\n", i, es.pts[loopTwo[i].seed].x, es.pts[loopTwo[i].seed].y,
+          loopTwo[i].outer.cx(), loopTwo[i].outer.cy(), loopTwo[i].outer.a(), loopTwo[i].outer.b(), loopTwo[i].outer_points.size());
+    [[maybe_unused]] const size_t before = markers.size();
+    loop_two_iteration(markers, es, loopTwo, i, pyramid_level, scale, p, rng);
+    TRACE("    -> %s\n", markers.size() > before ? "marker" : "rejected");
+  }
+}
+
+// Multiresolution.cpp:43-170.
+static bool intersect_line_two_ellipses(int y, const Ellipse& qIn, const Ellipse& qOut,
+                                        const EdgeSet& es, std::vector<int>& hull) {
+  float io[2], ii[2];
+  const int no = intersect_ellipse_with_line(qOut, (float)y, true, io);
+  const int ni = intersect_ellipse_with_line(qIn, (float)y, true, ii);
+  auto take = [&](int x) {
+    const int ie = es.at(x, y);
+    if (ie >= 0) {
+      const EdgePoint& e = es.pts[ie];
+      const float c0 = qIn.cx() - e.x, c1 = qIn.cy() - e.y;
+      if (e.gx * c0 + e.gy * c1 < 0) hull.push_back(ie);
+    }
+  };
+  const int W = es.shape_w();
+  if (no == 2 && ni == 2) {
+    const int b1 = std::max(0, (int)io[0]), e1 = std::min(W - 1, (int)ii[0]);
+    const int b2 = std::max(0, (int)ii[1]), e2 = std::min(W - 1, (int)io[1]);
+    for (int x = b1; x <= e1; ++x) take(x);
+    for (int x = b2; x <= e2; ++x) take(x);
+  } else if (no == 2 && ni <= 1) {
+    const int b = std::max(0, (int)io[0]), e = std::min(W - 1, (int)io[1]);
+    for (int x = b; x <= e; ++x) take(x);
+  } else if (no == 1 && ni == 0) {
+    if (io[0] >= 0 && io[0] < W) take((int)io[0]);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
+                     Pcg32& rng) {
+  for (Marker& m : markers) {
+    if (m.pyramid_level > 0) {
+      const float scale = m.scale;
+      Ellipse rescaled = m.rescaled_outer_ellipse;
+      std::vector<int> hull;
+      Ellipse qIn, qOut;
+      compute_hull(rescaled, scale, qIn, qOut);
+      const float yCenter = rescaled.cy();
+      const int maxY = std::max((int)yCenter, 0);
+      const int minY = std::min((int)yCenter, level0.shape_h() - 1);
+      for (int y = maxY; y < level0.shape_h(); ++y)
+        if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
+      for (int y = minY; y >= 0; --y)
+        if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
+      if (hull.size() < 5) continue;
+      std::vector<int> refit;
+      float SmFinal = 1e+10f;
+      outlier_removal(level0, hull, refit, SmFinal, 20.0f, kNoWeight, 60, rng);
+      if (refit.size() < 5) continue;
+      try {
+        fit_ellipse(as_points(level0, refit), rescaled);
+        std::vector<Pt2> pts;
+        for (int q : refit) pts.push_back(pt_of(level0.pts[q]));
+        m.center_img = Pt2(m.center_img.x * scale, m.center_img.y * scale);
+        m.rescaled_outer_ellipse = rescaled;
+        m.rescaled_outer_ellipse_points = pts;
+      } catch (...) {
+      }
+    } else {
+      m.rescaled_outer_ellipse_points = m.points.back();
+    }
+  }
+}
+
+void update_markers(std::vector<Marker>& markers, const Marker& add) {
+  bool flag = false;
+  for (Marker& cur : markers) {
+    if (cur.status > 0 && add.status > 0 && cur.is_equal(add)) {
+      if (add.quality > cur.quality) cur = add;
+      flag = true;
+    }
+  }
+  if (!flag) markers.push_back(add);
+}
+
+}  // namespace cctagx

@@ -1,0 +1,133 @@
+// Candidate stage: from one level's edge points, links, votes and seeds to
+// marker candidates (outer ellipse + ring points), then the level-0 re-fit of
+// markers found at coarser levels.  Serial host code; the front end (GPU, or
+// the oracle's CPU restatement) provides its input.
+//
+// Reference: Detection.cpp:69-693, Vote.cpp:249-742, EllipseGrowing.cpp,
+// Fitting.cpp:228-364, Multiresolution.cpp:43-193,268-413, CCTag.{hpp,cpp}.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace cctagx {
+
+// Borrowed view of one level's front-end output (raster-order points).
+struct LevelEdges {
+  int w = 0, h = 0;              // level size
+  int shape_w = 0, shape_h = 0;  // EdgePointCollection shape (level-0 size)
+  const int32_t* map = nullptr;  // w*h, point index or -1
+  int n = 0;
+  const int16_t *x = nullptr, *y = nullptr, *gx = nullptr, *gy = nullptr;
+  const int32_t *before = nullptr, *after = nullptr;
+  const int32_t *vindex = nullptr, *vlist = nullptr;  // CSR voter lists
+  const float* flow = nullptr;
+  const int32_t* ismax = nullptr;
+  int n_seeds = 0;
+  const int32_t* seeds = nullptr;  // in the order they reached min votes
+};
+
+// EdgePoint (EdgePoint.hpp:24-86) with the candidate stage's mutable state.
+struct EdgePoint {
+  int x, y;
+  float gx, gy, norm;  // gradient and its norm
+  float flow_length;
+  int32_t is_max;
+  uint64_t processed;  // per-candidate bit masks (EllipseGrowing.cpp:332)
+  int32_t n_segment_out;
+};
+
+// EdgePointCollection (Types.hpp:19-158).
+class EdgeSet {
+ public:
+  void load(const LevelEdges& L);
+  int at(int x, int y) const {  // caller guarantees 0 <= x < shape_w, 0 <= y < shape_h
+    return (x < w_ && y < h_) ? map_[(size_t)y * w_ + x] : -1;
+  }
+  int shape_w() const { return shape_w_; }
+  int shape_h() const { return shape_h_; }
+  int size() const { return (int)pts.size(); }
+  int before(int i) const { return before_[i]; }
+  int after(int i) const { return after_[i]; }
+  int voters_size(int i) const { return vindex_[i + 1] - vindex_[i]; }
+  const int32_t* voters_begin(int i) const { return vlist_ + vindex_[i]; }
+  const int32_t* voters_end(int i) const { return vlist_ + vindex_[i + 1]; }
+  // Types.hpp:142-155: an invalid index throws std::out_of_range.
+  void set_processed_in(int i, bool f) { pin_.at(check(i)) = f; }
+  bool test_processed_in(int i) const { return pin_.at(check(i)); }
+  void set_processed_aux(int i, bool f) { paux_.at(check(i)) = f; }
+  bool test_processed_aux(int i) const { return paux_.at(check(i)); }
+
+  std::vector<EdgePoint> pts;
+  std::vector<int32_t> seeds;
+
+ private:
+  static int check(int i) {
+    if (i < 0) throw std::out_of_range("EdgePointCollection: null point");
+    return i;
+  }
+  int w_ = 0, h_ = 0, shape_w_ = 0, shape_h_ = 0;
+  const int32_t* map_ = nullptr;
+  const int32_t *before_ = nullptr, *after_ = nullptr, *vindex_ = nullptr, *vlist_ = nullptr;
+  std::vector<uint8_t> pin_, paux_;
+};
+
+// The detection parameters the host stages read (Params.hpp:108-199).
+struct HostParams {
+  float average_vote_min = 0.f;
+  float thr_median_distance_ellipse = 3.f;
+  int maximum_nb_seeds = 500;
+  int maximum_nb_candidates_loop_two = 40;
+  int n_crowns = 3;
+  int min_points_segment_candidate = 10;
+  float thresh_robust_estimation_of_outer_ellipse = 30.f;
+  float ellipse_growing_elliptic_hull_width = 2.3f;
+  int window_size_on_inner_elliptic_segment = 20;
+  int n_samples_outer_ellipse = 150;
+  int num_cuts_in_ident_step = 22;
+  int num_samples_outer_edge_points_refinement = 20;
+  int sample_cut_length = 100;
+  int imaged_center_n_grid_sample = 5;
+  float imaged_center_neighbour_size = 0.2f;
+  float min_ident_proba = 1e-6f;
+  bool search_for_another_segment = true;
+  bool do_identification = true;
+};
+
+// Marker record (CCTag.hpp:49-409, the fields detection and identification use).
+struct Marker {
+  int id = -1;
+  int status = 0;
+  float quality = 0.f;
+  int pyramid_level = 0;
+  float scale = 1.f;
+  Pt2 center_img;                       // _centerImg
+  Ellipse outer_ellipse;                // _outerEllipse (level coordinates, +0.5 shift)
+  Ellipse rescaled_outer_ellipse;       // _rescaledOuterEllipse (level-0 coordinates)
+  std::vector<std::vector<Pt2>> points; // _points, one list per circle, outer last
+  std::vector<Pt2> rescaled_outer_ellipse_points;
+  Mat3 homography = Mat3::zero();
+  std::vector<float> radius_ratios;
+
+  // CCTag.hpp:68-93 constructor.
+  static Marker make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int level,
+                     float scale, float quality);
+  // CCTag.cpp:42-59.
+  bool is_equal(const Marker& other) const;
+};
+
+// cctagDetectionFromEdges (Detection.cpp:568-693), serial; rng is the single
+// rand_5_k stream of the call.
+void detection_from_edges(std::vector<Marker>& markers, EdgeSet& es, int level_h,
+                          int pyramid_level, float scale, const HostParams& p, Pcg32& rng);
+
+// Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges.
+void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
+                     Pcg32& rng);
+
+// Multiresolution.cpp:172-193.
+void update_markers(std::vector<Marker>& markers, const Marker& m);
+
+}  // namespace cctagx

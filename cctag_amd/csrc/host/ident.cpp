@@ -1,0 +1,235 @@
+// Identification host control (see ident.hpp).
+#include "ident.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <list>
+#include <string>
+
+#include "../bank_radii.hpp"
+
+namespace cctagx {
+
+std::vector<std::vector<float>> marker_bank(int n_crowns) {
+  const char* text = n_crowns == 4 ? kRadii4 : kRadii3;
+  std::vector<std::vector<float>> bank;
+  std::vector<int> radii;
+  std::string cur;
+  auto flush_row = [&]() {
+    if (radii.empty()) return;
+    std::vector<float> row;
+    for (auto it = radii.rbegin(); it != radii.rend(); ++it) {
+      char buf[32];
+      std::snprintf(buf, sizeof(buf), "%.6f", 100.0 / *it);
+      row.push_back(std::strtof(buf, nullptr));
+    }
+    bank.push_back(row);
+    radii.clear();
+  };
+  for (const char* c = text;; ++c) {
+    if (*c == ' ' || *c == ';' || *c == 0) {
+      if (!cur.empty()) radii.push_back(std::atoi(cur.c_str()));
+      cur.clear();
+      if (*c == ';' || *c == 0) flush_row();
+      if (*c == 0) break;
+    } else {
+      cur.push_back(*c);
+    }
+  }
+  return bank;
+}
+
+IdentConstants ident_constants(const HostParams& p, const std::vector<std::vector<float>>& bank) {
+  IdentConstants k{};
+  // Identification.cpp:1221-1235
+  if (p.n_crowns == 3) k.start_sig = 1 - (2 * (size_t)p.n_crowns - 1) * 0.15f;
+  else if (p.n_crowns == 4) k.start_sig = 0.26f;
+  else k.start_sig = 0.f;
+  k.n_samples = p.sample_cut_length;
+  k.n_refine_samples = p.num_samples_outer_edge_points_refinement;
+  k.bank_rows = (int)bank.size();
+  k.bank_cols = bank.empty() ? 0 : (int)bank[0].size();
+  for (int i = 0; i < k.bank_rows && i < kMaxBank; ++i)
+    for (int j = 0; j < k.bank_cols && j < kMaxRatios; ++j) k.inv_ratio[i][j] = 1.f / bank[i][j];
+  return k;
+}
+
+// Identification.hpp:344-371.
+static float compute_median(const std::vector<float>& vec) {
+  const size_t s = vec.size() / 2 + 1;
+  std::vector<float> out(s);
+  std::partial_sort_copy(vec.begin(), vec.end(), out.begin(), out.end());
+  if (vec.size() % 2) return out.back();
+  return (float)((out[s - 1] + out[s - 2]) / 2.0);
+}
+
+void identify_markers(std::vector<Marker>& markers, const HostParams& p,
+                      const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
+  const size_t n = markers.size();
+  std::vector<int> detected(n, -1);
+  // --- identify_step_1 (Identification.cpp:1172-1301) ----------------------
+  std::vector<CutJob> jobs;
+  std::vector<size_t> job_marker;
+  for (size_t i = 0; i < n; ++i) {
+    const Marker& m = markers[i];
+    const Ellipse& e = m.rescaled_outer_ellipse;
+    CutJob job;
+    sorted_outer_points(e, m.rescaled_outer_ellipse_points, job.outer_points,
+                        (size_t)p.n_samples_outer_ellipse);
+    if (job.outer_points.size() < 5) {
+      detected[i] = status::too_few_outer_points;
+      continue;
+    }
+    job.cx = e.cx();
+    job.cy = e.cy();
+    job.ca = std::cos((double)e.angle());
+    job.sa = std::sin((double)e.angle());
+    job.ea = e.a();
+    job.eb = e.b();
+    job.scale = m.scale;
+    jobs.push_back(std::move(job));
+    job_marker.push_back(i);
+  }
+  std::vector<CutSelection> sel;
+  backend.select_cuts(jobs, p, sel);
+  std::vector<std::vector<Pt2>> stops(n);
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const size_t i = job_marker[j];
+    if (sel[j].n_collected == 0) detected[i] = status::no_collected_cuts;
+    else if (sel[j].stops.empty()) detected[i] = status::no_selected_cuts;
+    else {
+      detected[i] = status::id_reliable;
+      stops[i] = sel[j].stops;
+    }
+  }
+  // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
+  std::vector<RefineJob> rjobs;
+  std::vector<size_t> rmarker;
+  bool have_mean_ab = false;
+  float mean_ab = 0.f;
+  const float sqrt2 = std::sqrt(2.f);
+  const IdentConstants kc = ident_constants(p, bank);
+  for (size_t i = 0; i < n; ++i) {
+    if (detected[i] != status::id_reliable) continue;
+    const Marker& m = markers[i];
+    const Ellipse& e = m.rescaled_outer_ellipse;
+    RefineJob r;
+    e.canonic_form(r.canonic, r.t_can, r.t_inv_can);
+    // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
+    // first ellipse it sees.
+    if (!have_mean_ab) {
+      mean_ab = (e.a() + e.b()) / 2.f;
+      have_mean_ab = true;
+    }
+    r.cond = Mat3::zero();
+    r.cond.m[0][0] = sqrt2 / mean_ab;
+    r.cond.m[0][2] = -sqrt2 * e.cx() / mean_ab;
+    r.cond.m[1][1] = sqrt2 / mean_ab;
+    r.cond.m[1][2] = -sqrt2 * e.cy() / mean_ab;
+    r.cond.m[2][2] = 1.f;
+    r.cond_inv = inverse3(r.cond);
+    float gs = 0.f;
+    try {
+      const Ellipse te(mul(mul(transpose(r.cond_inv), e.matrix()), r.cond_inv));
+      gs = std::max(te.a(), te.b());
+    } catch (...) {
+      detected[i] = status::opti_has_diverged;
+      continue;
+    }
+    r.grid_scale = gs;
+    r.max_semi_axis = std::max(e.a(), e.b());
+    r.neighbour0 = p.imaged_center_neighbour_size;
+    float ns = p.imaged_center_neighbour_size;
+    int it = 0;
+    const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
+    while (ns * r.max_semi_axis > 0.02) {
+      ++it;
+      ns /= div;
+    }
+    r.n_iter = it;
+    r.center0 = m.center_img;
+    r.stops = stops[i];
+    r.begin_sig = kc.start_sig;
+    r.end_sig = 1.f;
+    rjobs.push_back(std::move(r));
+    rmarker.push_back(i);
+  }
+  std::vector<RefineResult> res;
+  backend.refine(rjobs, p, res);
+  const int nb = (int)bank.size();
+  for (size_t j = 0; j < rjobs.size(); ++j) {
+    const size_t i = rmarker[j];
+    Marker& m = markers[i];
+    const RefineResult& r = res[j];
+    if (!r.ok) {
+      m.quality = 1.f / FLT_MAX;
+      m.center_img = Pt2(0.f, 0.f);
+      detected[i] = status::opti_has_diverged;
+      continue;
+    }
+    m.center_img = r.center;
+    m.homography = r.H;
+    const size_t ncut = rjobs[j].stops.size();
+    const size_t ns = r.signals.size() / ncut;
+    std::vector<size_t> correct;
+    for (size_t k = 0; k < ncut; ++k)
+      if (!r.oob[k]) correct.push_back(k);
+    std::vector<float> bar(ns), along(correct.size());
+    for (size_t s = 0; s < ns; ++s) {
+      for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
+      bar[s] = compute_median(along);
+    }
+    const float vMin = *std::min_element(bar.begin(), bar.end());
+    const float vMax = *std::max_element(bar.begin(), bar.end());
+    const float magnitude = vMax - vMin;
+    const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
+    m.quality = 1.f / residual;
+    if (residual > 2.7f) {
+      detected[i] = status::opti_has_diverged;
+      continue;
+    }
+    // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
+    std::vector<std::list<float>> vScore(nb);
+    for (size_t k = 0; k < ncut; ++k) {
+      if (r.oob[k]) continue;
+      std::map<float, int> sortedId;
+      for (int idc = 0; idc < nb; ++idc) {
+        const float v = std::exp(-r.id_dist[k * nb + idc]);
+        sortedId[v] = idc;
+      }
+      const auto best = sortedId.rbegin();
+      vScore[best->second].push_back(best->first);
+    }
+    size_t maxSize = 0;
+    int iMax = 0;
+    for (int c = 0; c < nb; ++c)
+      if (vScore[c].size() > maxSize) {
+        iMax = c;
+        maxSize = vScore[c].size();
+      }
+    float score = 0;
+    for (float v : vScore[iMax]) score += v;
+    score /= vScore[iMax].size();
+    m.id = iMax;
+    m.radius_ratios = bank[iMax];
+    try {
+      const Mat3 inv = inverse3(m.homography);
+      for (float rr : m.radius_ratios) {
+        const Ellipse circle(0.f, 0.f, 1.f / rr, 1.f / rr, 0.f);
+        const Ellipse ring(mul(mul(transpose(inv), circle.matrix()), inv));
+        (void)ring;
+      }
+    } catch (...) {
+      detected[i] = status::degenerate;
+      continue;
+    }
+    detected[i] = (score > p.min_ident_proba) ? status::id_reliable : status::id_not_reliable;
+  }
+  for (size_t i = 0; i < n; ++i) markers[i].status = detected[i];
+}
+
+}  // namespace cctagx

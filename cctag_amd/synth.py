@@ -8,8 +8,10 @@ white when #{j : 1/rr_j <= rho} is even and black otherwise, with a white
 quiet zone for 1 <= rho <= 1.4 (markersToPrint/generators/generate.py:18-21).
 The ground-truth imaged centre is H.(0,0,1).
 
-Pixel convention: pixel (x, y) integrates the square [x, x+1) x [y, y+1), so
-the continuous coordinate of a pixel centre is (x + 0.5, y + 0.5).
+Pixel convention: the renderer integrates pixel (x, y) over the square
+[x, x+1) x [y, y+1).  Ground-truth centres are reported in the detector's
+coordinates, where the value of pixel (x, y) sits at (x, y) (the bilinear
+sampler of Identification.hpp:197-222), i.e. H.(0,0,1) - (0.5, 0.5).
 
 Frame k of configuration c uses seed 1000*c + k.  Everything is numpy with a
 PCG64 generator, so a frame is byte-identical on any machine with the same
@@ -32,7 +34,7 @@ WHITE = 235.0
 @dataclasses.dataclass
 class MarkerGT:
     id: int
-    center: tuple          # ground-truth imaged centre H.(0,0,1), corner convention
+    center: tuple          # ground-truth imaged centre, detector coordinates
     radius: float          # outer radius in px at zero tilt
     H: np.ndarray          # 3x3 plane -> image
     occluded: bool = False
@@ -255,7 +257,7 @@ def make_frame(cfg: FrameConfig, seed: int, n_markers: int | None = None):
             xa, xb = int(max(ox - half, 0)), int(min(ox + half, w))
             ya, yb = int(max(oy - half, 0)), int(min(oy + half, h))
             img[ya:yb, xa:xb] = 25.0
-        gts.append(MarkerGT(mid, (cx, cy), r, H, occluded))
+        gts.append(MarkerGT(mid, (cx - 0.5, cy - 0.5), r, H, occluded))
     sigma = rng.uniform(*cfg.blur)
     if sigma > 0:
         img = ndimage.gaussian_filter(img, sigma, mode="nearest")

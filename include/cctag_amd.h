@@ -107,6 +107,30 @@ typedef struct cctagx_level_view {
 
 int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* out);
 
+/* ---- full detection ------------------------------------------------------
+ * One marker as cctag::ICCTag exposes it (ICCTag.hpp:28-82): imaged centre
+ * x(), y(), id(), getStatus() (1 = identified; negative codes of
+ * CCTag.hpp:416-426), rescaledOuterEllipse(), plus quality, pyramid level,
+ * scale and the image->marker homography. */
+typedef struct cctagx_marker {
+  float x, y;
+  int32_t id;
+  int32_t status;
+  float quality;
+  int32_t pyramid_level;
+  float scale;
+  float ellipse[5];      /* rescaled outer ellipse: cx, cy, a, b, angle */
+  float homography[9];   /* row-major */
+} cctagx_marker;
+
+/* Replaces cctag::cctagDetection(markers, pipeId, frame, graySrc, params,
+ * durations, bank) (ICCTag.hpp:124-131, Detection.cpp:773-999): front end and
+ * identification on the GPU, candidate stage on the host.  Writes up to
+ * max_out markers sorted by id and their total count in *n_out (which may
+ * exceed max_out).  `frame` is informational, as in the reference. */
+int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
+                  cctagx_marker* out, int max_out, int* n_out);
+
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
  * events on the library stream; names are static strings. */

@@ -89,3 +89,73 @@ double oracle_time_front_end(const uint8_t* img, int w, int h, int n_levels, int
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Full CPU detection: the oracle front end and identification around the
+// shared host candidate stage.
+#include "../cctag_amd/csrc/host/cabi.hpp"
+#include "../cctag_amd/csrc/host/pipeline.hpp"
+#include "oracle_ident.hpp"
+
+namespace {
+std::vector<cctagx::LevelEdges> level_edges(const std::vector<oracle::LevelResult>& levels) {
+  std::vector<cctagx::LevelEdges> out;
+  for (const oracle::LevelResult& L : levels) {
+    cctagx::LevelEdges e;
+    e.w = L.w; e.h = L.h; e.shape_w = L.W0; e.shape_h = L.H0;
+    e.map = L.edge_map.data();
+    e.n = (int)L.ex.size();
+    e.x = L.ex.data(); e.y = L.ey.data(); e.gx = L.edx.data(); e.gy = L.edy.data();
+    e.before = L.before.data(); e.after = L.after.data();
+    e.vindex = L.voters_index.data(); e.vlist = L.voters_list.data();
+    e.flow = L.flow_length.data(); e.ismax = L.is_max.data();
+    e.n_seeds = (int)L.seeds.size(); e.seeds = L.seeds.data();
+    out.push_back(e);
+  }
+  return out;
+}
+
+oracle::Params front_params(const cctagx_params& c) {
+  oracle::Params p;
+  p.canny_low = c.canny_thr_low;
+  p.canny_high = c.canny_thr_high;
+  p.dist_search = c.dist_search;
+  p.thr_grad_vote = c.thr_gradient_mag_in_vote;
+  p.angle_voting = c.angle_voting;
+  p.ratio_voting = c.ratio_voting;
+  p.n_crowns = c.n_crowns;
+  p.min_votes = c.min_votes_to_select_candidate;
+  p.n_levels = c.number_of_processed_multires_layers;
+  return p;
+}
+}  // namespace
+
+extern "C" {
+
+// cctagDetection on the CPU; returns the marker count or -1.
+int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
+                  cctagx_marker* out, int max_out) {
+  try {
+    std::vector<oracle::LevelResult> levels;
+    oracle::front_end(img, w, h, front_params(*params), levels);
+    const auto bank = cctagx::marker_bank(params->n_crowns);
+    oracle::CpuIdentBackend backend(levels[0].img.data(), w, h, bank);
+    std::vector<cctagx::Marker> markers;
+    cctagx::detect_markers(level_edges(levels), cctagx::host_params(*params), bank, &backend, markers);
+    for (size_t i = 0; i < markers.size() && (int)i < max_out; ++i) cctagx::to_c_marker(markers[i], out[i]);
+    return (int)markers.size();
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Wall time of the full CPU detection (front end + candidates + identification).
+double oracle_time_detect(const cctagx_params* params, const uint8_t* img, int w, int h, int iters) {
+  std::vector<cctagx_marker> out(256);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < iters; ++i) oracle_detect(params, img, w, h, out.data(), (int)out.size());
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count() / iters;
+}
+
+}  // extern "C"

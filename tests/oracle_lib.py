@@ -37,6 +37,11 @@ def load():
     o.oracle_downsample.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
     o.oracle_time_front_end.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_front_end.restype = ctypes.c_double
+    o.oracle_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(_lib.Marker), ctypes.c_int]
+    o.oracle_time_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+    o.oracle_time_detect.restype = ctypes.c_double
     _o = o
     return o
 
@@ -113,3 +118,24 @@ def downsample(img: np.ndarray) -> np.ndarray:
     if rc != 0:
         raise ValueError("not an exact 2x level")
     return out
+
+
+def detect(img: np.ndarray, params=None) -> list:
+    """Full CPU detection (cctagDetection) -> list of marker dicts, sorted by id."""
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    prm = params if params is not None else _lib.default_params(3)
+    buf = (_lib.Marker * 512)()
+    n = o.oracle_detect(ctypes.byref(prm), img.ctypes.data, w, h, buf, 512)
+    if n < 0:
+        raise RuntimeError("oracle detection failed")
+    return [buf[i].as_dict() for i in range(min(n, 512))]
+
+
+def time_detect(img: np.ndarray, iters: int = 1, params=None) -> float:
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    prm = params if params is not None else _lib.default_params(3)
+    return o.oracle_time_detect(ctypes.byref(prm), img.ctypes.data, w, h, iters)
