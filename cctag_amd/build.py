@@ -2,9 +2,10 @@
 
     python -m cctag_amd.build          # or __graft_entry__.build()
 
-hipcc compiles every .hip source of csrc/ for gfx950 and links one shared
-library next to this file.  Float contraction is off so the kernels keep the
-reference's rounding order; packed FP32 instructions are disabled.
+hipcc compiles every .hip source of csrc/ for gfx950; the shared host stages
+(csrc/host/*.cpp) are plain C++ built with the ROCm clang++.  Float
+contraction is off everywhere, so kernels and host code keep the reference's
+rounding order; packed FP32 instructions are disabled in device code.
 """
 from __future__ import annotations
 
@@ -21,26 +22,31 @@ LIB = os.path.join(HERE, "libcctag_amd.so")
 ARCH = os.environ.get("CCTAGX_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    "-ffp-contract=off", "-fno-fast-math",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas",
-]
+CLANGXX = "/opt/rocm/llvm/bin/clang++" if os.path.exists("/opt/rocm/llvm/bin/clang++") else "clang++"
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+          "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas",
+          "-Wno-sign-compare"]
+HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}",
+                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    out = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    host = os.path.join(CSRC, "host")
+    out += [os.path.join(host, f) for f in sorted(os.listdir(host)) if f.endswith(".cpp")]
+    return out
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [HIPCC, *FLAGS, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
-    if src.endswith(".cpp"):
-        cmd[1:1] = ["-x", "hip"]
+    rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
+    obj = os.path.join(BUILD, rel + ".o")
+    if src.endswith(".hip"):
+        cmd = [HIPCC, *HIP_FLAGS, "-I", CSRC, "-c", src, "-o", obj]
+    else:
+        cmd = [CLANGXX, *COMMON, "-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s%s" % (src, r.stdout, r.stderr))
+        raise RuntimeError("compile failed for %s:\n%s%s" % (src, r.stdout, r.stderr))
     return obj
 
 

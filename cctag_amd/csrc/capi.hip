@@ -1,6 +1,7 @@
 // C ABI of the library (include/cctag_amd.h).
 #include "../../include/cctag_amd.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -9,6 +10,9 @@
 
 #include "device.hpp"
 #include "frontend_host.hpp"
+#include "host/cabi.hpp"
+#include "host/pipeline.hpp"
+#include "ident_device.hpp"
 
 using namespace cctagx;
 
@@ -37,8 +41,16 @@ const T* ptr(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
 
 }  // namespace
 
+// Pinned host copy of the candidate stage's inputs for one level.
+struct StagedLevel {
+  PinnedBuffer emap, ex, ey, egx, egy, before, after, vindex, vlist, flow, ismax, seeds;
+  LevelEdges view;
+};
+
 struct cctagx_handle {
   cctagx_params params{};
+  HostParams host{};
+  std::vector<std::vector<float>> bank;
   int width = 0, height = 0;
   hipStream_t stream = nullptr;
   FrontEndDevice fe;
@@ -46,6 +58,11 @@ struct cctagx_handle {
   bool have_front_end = false;
   std::vector<HostLevel> levels;
   std::vector<bool> level_loaded;
+  std::vector<StagedLevel> staged;
+  PinnedBuffer counts;
+  GpuIdentBackend ident;
+  std::vector<Marker> markers;
+  double host_ms = 0;  // last detection: host wall time after the front end
 };
 
 static int validate(const cctagx_params* p) {
@@ -55,6 +72,67 @@ static int validate(const cctagx_params* p) {
   if (p->number_of_processed_multires_layers < 1 || p->number_of_processed_multires_layers > 4)
     return CCTAGX_ERR_UNSUPPORTED;
   if (p->dist_search < 0 || p->min_votes_to_select_candidate < 1) return CCTAGX_ERR_ARG;
+  // fixed bounds of the identification kernels and of the candidate stage's
+  // 64-bit per-candidate masks
+  if (p->imaged_center_n_grid_sample < 3 || p->imaged_center_n_grid_sample > kMaxGrid ||
+      p->num_cuts_in_ident_step < 1 || p->num_cuts_in_ident_step > kMaxCuts ||
+      p->sample_cut_length < 31 || p->sample_cut_length > kMaxSamples ||
+      p->num_samples_outer_edge_points_refinement < 9 ||
+      p->num_samples_outer_edge_points_refinement > kMaxRefineSamples ||
+      p->n_samples_outer_ellipse < 5 || p->n_samples_outer_ellipse > kMaxOuterPoints ||
+      p->maximum_nb_candidates_loop_two < 0 || p->maximum_nb_candidates_loop_two > 64)
+    return CCTAGX_ERR_UNSUPPORTED;
+  return CCTAGX_OK;
+}
+
+// Copies what the candidate stage reads of every level into pinned memory:
+// counts first (one synchronization), then the exact-size arrays.
+static int stage_levels(cctagx_handle* h) {
+  const PlaneSet& ps = h->fe.planes();
+  const FrontEndBuffers& b = h->fe.buffers();
+  const int np = (int)ps.planes.size();
+  hipStream_t s = h->stream;
+  if (h->counts.ensure(sizeof(PlaneCounts) * np)) return CCTAGX_ERR_HIP;
+  PlaneCounts* cnt = h->counts.as<PlaneCounts>();
+  CCTAGX_HIP(hipMemcpyAsync(cnt, b.counts, sizeof(PlaneCounts) * np, hipMemcpyDeviceToHost, s));
+  CCTAGX_HIP(hipStreamSynchronize(s));
+  h->staged.resize(np);
+  for (int l = 0; l < np; ++l) {
+    const PlaneDesc& d = ps.planes[l];
+    StagedLevel& S = h->staged[l];
+    const size_t N = (size_t)d.w * d.h, n = cnt[l].n_edges, nv = cnt[l].n_voters,
+                 nsd = cnt[l].n_seeds;
+    auto get = [&](PinnedBuffer& dst, const void* src, size_t bytes) -> int {
+      if (dst.ensure(std::max<size_t>(bytes, 64))) return CCTAGX_ERR_HIP;
+      if (bytes) CCTAGX_HIP(hipMemcpyAsync(dst.as<void>(), src, bytes, hipMemcpyDeviceToHost, s));
+      return 0;
+    };
+    int rc = 0;
+    rc |= get(S.emap, b.emap + d.pix, 4 * N);
+    rc |= get(S.ex, b.ex + d.pix, 2 * n);
+    rc |= get(S.ey, b.ey + d.pix, 2 * n);
+    rc |= get(S.egx, b.egx + d.pix, 2 * n);
+    rc |= get(S.egy, b.egy + d.pix, 2 * n);
+    rc |= get(S.before, b.before + d.pix, 4 * n);
+    rc |= get(S.after, b.after + d.pix, 4 * n);
+    rc |= get(S.vindex, b.vindex + d.pix + l, 4 * (n + 1));
+    rc |= get(S.vlist, b.vlist + d.pix, 4 * nv);
+    rc |= get(S.flow, b.flow + d.pix, 4 * n);
+    rc |= get(S.ismax, b.ismax + d.pix, 4 * n);
+    rc |= get(S.seeds, b.seeds + d.pix, 4 * nsd);
+    if (rc) return CCTAGX_ERR_HIP;
+    LevelEdges& v = S.view;
+    v.w = d.w; v.h = d.h; v.shape_w = d.shape_w; v.shape_h = d.shape_h;
+    v.map = S.emap.as<int32_t>();
+    v.n = (int)n;
+    v.x = S.ex.as<int16_t>(); v.y = S.ey.as<int16_t>();
+    v.gx = S.egx.as<int16_t>(); v.gy = S.egy.as<int16_t>();
+    v.before = S.before.as<int32_t>(); v.after = S.after.as<int32_t>();
+    v.vindex = S.vindex.as<int32_t>(); v.vlist = S.vlist.as<int32_t>();
+    v.flow = S.flow.as<float>(); v.ismax = S.ismax.as<int32_t>();
+    v.n_seeds = (int)nsd; v.seeds = S.seeds.as<int32_t>();
+  }
+  CCTAGX_HIP(hipStreamSynchronize(s));
   return CCTAGX_OK;
 }
 
@@ -138,7 +216,55 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   if (int rc = h->fe.init(ps, front_end_params(*params))) return rc;
   h->levels.resize(ps.planes.size());
   h->level_loaded.assign(ps.planes.size(), false);
+  h->host = host_params(*params);
+  h->bank = marker_bank(params->n_crowns);
+  try {
+    if (int rc = h->ident.init(h->bank, h->host, h->stream)) return rc;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_HIP;
+  }
   *out = h.release();
+  return CCTAGX_OK;
+}
+
+int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
+                  cctagx_marker* out, int max_out, int* n_out) {
+  (void)frame;
+  if (!h || !gray || stride < (size_t)h->width || (max_out > 0 && !out) || !n_out)
+    return CCTAGX_ERR_ARG;
+  *n_out = 0;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  h->timer.reset();
+  if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
+  h->have_front_end = true;
+  h->level_loaded.assign(h->level_loaded.size(), false);
+  if (int rc = stage_levels(h)) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    std::vector<LevelEdges> views;
+    for (const StagedLevel& S : h->staged) views.push_back(S.view);
+    const PlaneDesc& d0 = h->fe.planes().planes[0];
+    h->ident.set_image(h->fe.buffers().img + d0.pix, d0.w, d0.h);
+    detect_markers(views, h->host, h->bank, &h->ident, h->markers);
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_HIP;
+  }
+  h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const int n = (int)h->markers.size();
+  for (int i = 0; i < n && i < max_out; ++i) to_c_marker(h->markers[i], out[i]);
+  *n_out = n;
+  return CCTAGX_OK;
+}
+
+// Wall time of the host part and device time of the identification kernels
+// of the last cctagx_detect, microseconds.
+int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, float* refine_us) {
+  if (!h || !host_us || !select_us || !refine_us) return CCTAGX_ERR_ARG;
+  *host_us = (float)(h->host_ms * 1000.0);
+  *select_us = h->ident.last_select_us();
+  *refine_us = h->ident.last_refine_us();
   return CCTAGX_OK;
 }
 

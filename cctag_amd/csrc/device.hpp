@@ -63,6 +63,20 @@ class PinnedBuffer {
   PinnedBuffer() = default;
   PinnedBuffer(const PinnedBuffer&) = delete;
   PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  PinnedBuffer(PinnedBuffer&& o) noexcept : p_(o.p_), cap_(o.cap_) {
+    o.p_ = nullptr;
+    o.cap_ = 0;
+  }
+  PinnedBuffer& operator=(PinnedBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      cap_ = o.cap_;
+      o.p_ = nullptr;
+      o.cap_ = 0;
+    }
+    return *this;
+  }
   ~PinnedBuffer() { release(); }
   int ensure(size_t bytes) {
     if (bytes <= cap_) return 0;

@@ -1,0 +1,164 @@
+// GPU identification backend (host side): packs the per-marker jobs the host
+// prepared, launches the identification kernels once per step for all
+// markers and unpacks the results.
+#include <stdexcept>
+#include <string>
+
+#include "ident_device.hpp"
+
+namespace cctagx {
+
+namespace {
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+void copy9(float* dst, const Mat3& m) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) dst[i * 3 + j] = m.m[i][j];
+}
+}  // namespace
+
+int GpuIdentBackend::init(const std::vector<std::vector<float>>& bank, const HostParams& p,
+                          hipStream_t s) {
+  stream_ = s;
+  k_ = ident_constants(p, bank);
+  for (hipEvent_t& e : ev_)
+    if (!e) CCTAGX_HIP(hipEventCreate(&e));
+  ensure(64, 64 * 160);
+  return 0;
+}
+
+void GpuIdentBackend::ensure(int n_markers, int n_points) {
+  if (n_markers <= cap_markers_ && n_points <= cap_points_) return;
+  const int nm = std::max(n_markers, cap_markers_), np = std::max(n_points, cap_points_);
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  dev_.release();
+  dev_.clear_plan();
+  const size_t o_sj = dev_.reserve(sizeof(SelectJobDev) * nm);
+  const size_t o_pts = dev_.reserve(sizeof(float4) * np);
+  const size_t o_so = dev_.reserve(sizeof(SelectOutDev) * nm);
+  const size_t o_rj = dev_.reserve(sizeof(RefineJobDev) * nm);
+  const size_t o_ro = dev_.reserve(sizeof(RefineOutDev) * nm);
+  const size_t o_scr =
+      dev_.reserve(sizeof(float) * (size_t)nm * kMaxGrid * kMaxGrid * kMaxCuts * kMaxSamples);
+  const size_t o_inv = dev_.reserve(sizeof(float) * kMaxBank * kMaxRatios);
+  if (dev_.allocate() != 0) throw std::runtime_error(std::string("hipMalloc: ") + last_error());
+  d_sel_jobs_ = dev_.at<SelectJobDev>(o_sj);
+  d_pts_ = dev_.at<float4>(o_pts);
+  d_sel_out_ = dev_.at<SelectOutDev>(o_so);
+  d_ref_jobs_ = dev_.at<RefineJobDev>(o_rj);
+  d_ref_out_ = dev_.at<RefineOutDev>(o_ro);
+  d_scratch_ = dev_.at<float>(o_scr);
+  d_inv_ratio_ = dev_.at<float>(o_inv);
+  hip_check(hipMemcpy(d_inv_ratio_, &k_.inv_ratio[0][0], sizeof(float) * kMaxBank * kMaxRatios,
+                      hipMemcpyHostToDevice),
+            "hipMemcpy bank");
+  if (h_sel_jobs_.ensure(sizeof(SelectJobDev) * nm) || h_pts_.ensure(sizeof(float4) * np) ||
+      h_sel_out_.ensure(sizeof(SelectOutDev) * nm) || h_ref_jobs_.ensure(sizeof(RefineJobDev) * nm) ||
+      h_ref_out_.ensure(sizeof(RefineOutDev) * nm))
+    throw std::runtime_error(std::string("hipHostMalloc: ") + last_error());
+  cap_markers_ = nm;
+  cap_points_ = np;
+}
+
+void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
+                                  std::vector<CutSelection>& out) {
+  out.assign(jobs.size(), CutSelection());
+  const int n = (int)jobs.size();
+  if (n == 0) return;
+  int npts = 0;
+  for (const CutJob& j : jobs) npts += (int)j.outer_points.size();
+  ensure(n, npts);
+  SelectJobDev* hj = h_sel_jobs_.as<SelectJobDev>();
+  float4* hp = h_pts_.as<float4>();
+  int off = 0;
+  for (int i = 0; i < n; ++i) {
+    const CutJob& j = jobs[i];
+    if ((int)j.outer_points.size() > kMaxOuterPoints)
+      throw std::runtime_error("more outer points than the identification kernels accept");
+    hj[i] = SelectJobDev{j.cx, j.cy, j.ca, j.sa, j.ea, j.eb, j.scale, (int)j.outer_points.size(), off};
+    for (const Pt2& q : j.outer_points) hp[off++] = make_float4(q.x, q.y, q.dx, q.dy);
+  }
+  hipStream_t s = stream_;
+  hip_check(hipMemcpyAsync(d_sel_jobs_, hj, sizeof(SelectJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
+  hip_check(hipMemcpyAsync(d_pts_, hp, sizeof(float4) * npts, hipMemcpyHostToDevice, s), "H2D");
+  hipEventRecord(ev_[0], s);
+  if (launch_select_cuts(d_sel_jobs_, n, d_pts_, img_, w_, h_, k_.start_sig, k_.n_samples,
+                         k_.n_refine_samples, p.num_cuts_in_ident_step, d_sel_out_, s) != 0)
+    throw std::runtime_error("k_select_cuts launch failed");
+  hipEventRecord(ev_[1], s);
+  SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
+  hip_check(hipMemcpyAsync(ho, d_sel_out_, sizeof(SelectOutDev) * n, hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipStreamSynchronize(s), "sync select");
+  hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
+  select_us_ *= 1000.f;
+  for (int i = 0; i < n; ++i) {
+    out[i].n_collected = ho[i].n_collected;
+    for (int k = 0; k < ho[i].n_selected; ++k) {
+      const float4 v = ho[i].stops[k];
+      out[i].stops.emplace_back(v.x, v.y, v.z, v.w);
+    }
+  }
+}
+
+void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParams& p,
+                             std::vector<RefineResult>& out) {
+  out.assign(jobs.size(), RefineResult());
+  const int n = (int)jobs.size();
+  if (n == 0) return;
+  ensure(n, 0);
+  RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
+  for (int i = 0; i < n; ++i) {
+    const RefineJob& r = jobs[i];
+    RefineJobDev& d = hj[i];
+    d.Q11 = r.canonic.m[0][0];
+    d.Q22 = r.canonic.m[1][1];
+    d.Q33 = r.canonic.m[2][2];
+    copy9(d.t_can, r.t_can);
+    copy9(d.t_inv_can, r.t_inv_can);
+    copy9(d.cond, r.cond);
+    copy9(d.cond_inv, r.cond_inv);
+    d.grid_scale = r.grid_scale;
+    d.neighbour0 = r.neighbour0;
+    d.begin_sig = r.begin_sig;
+    d.end_sig = r.end_sig;
+    d.n_iter = r.n_iter;
+    d.n_cuts = (int)r.stops.size();
+    d.c0x = r.center0.x;
+    d.c0y = r.center0.y;
+    for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
+  }
+  hipStream_t s = stream_;
+  hip_check(hipMemcpyAsync(d_ref_jobs_, hj, sizeof(RefineJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
+  hipEventRecord(ev_[2], s);
+  if (launch_refine(d_ref_jobs_, n, img_, w_, h_, k_.n_samples, p.imaged_center_n_grid_sample,
+                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_scratch_, d_ref_out_, s) != 0)
+    throw std::runtime_error("k_refine launch failed");
+  hipEventRecord(ev_[3], s);
+  RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
+  hip_check(hipMemcpyAsync(ho, d_ref_out_, sizeof(RefineOutDev) * n, hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipStreamSynchronize(s), "sync refine");
+  hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
+  refine_us_ *= 1000.f;
+  const int ns = k_.n_samples, nb = k_.bank_rows;
+  for (int i = 0; i < n; ++i) {
+    const RefineOutDev& o = ho[i];
+    RefineResult& r = out[i];
+    r.ok = o.ok != 0;
+    if (!r.ok) continue;
+    r.center = Pt2(o.cx, o.cy);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) r.H.m[a][b] = o.H[a * 3 + b];
+    r.min_res = o.min_res;
+    const int nc = (int)jobs[i].stops.size();
+    r.oob.assign(o.oob, o.oob + nc);
+    r.signals.resize((size_t)nc * ns);
+    r.id_dist.assign((size_t)nc * nb, 0.f);
+    for (int k = 0; k < nc; ++k) {
+      for (int ii = 0; ii < ns; ++ii) r.signals[(size_t)k * ns + ii] = o.signals[k * kMaxSamples + ii];
+      for (int c = 0; c < nb; ++c) r.id_dist[(size_t)k * nb + c] = o.id_dist[k * kMaxBankRows + c];
+    }
+  }
+}
+
+}  // namespace cctagx

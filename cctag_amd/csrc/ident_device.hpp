@@ -1,0 +1,94 @@
+// Device-side identification: job / result layouts of the identification
+// kernels and the GPU implementation of the host's IdentBackend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device.hpp"
+#include "host/ident.hpp"
+
+namespace cctagx {
+
+constexpr int kMaxGrid = 5;        // imagedCenterNGridSample upper bound accepted
+constexpr int kMaxBankRows = 128;  // rows of the 4-crown bank
+constexpr int kMaxRefineSamples = 32;  // numSamplesOuterEdgePointsRefinement bound
+
+struct SelectJobDev {
+  float cx, cy;
+  double ca, sa;
+  float ea, eb, scale;
+  int n_outer, pt_offset;
+};
+
+struct SelectOutDev {
+  int n_collected, n_selected;
+  float4 stops[kMaxCuts];  // x, y, dx, dy
+};
+
+struct RefineJobDev {
+  float Q11, Q22, Q33;
+  float t_can[9], t_inv_can[9], cond[9], cond_inv[9];
+  float grid_scale, neighbour0, begin_sig, end_sig;
+  int n_iter, n_cuts;
+  float c0x, c0y;
+  float2 stops[kMaxCuts];
+};
+
+struct RefineOutDev {
+  int ok;
+  float cx, cy;
+  float H[9];
+  float min_res;
+  uint8_t oob[kMaxCuts];
+  float signals[kMaxCuts * kMaxSamples];
+  float id_dist[kMaxCuts * kMaxBankRows];
+};
+
+int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, const uint8_t* img,
+                       int w, int h, float start_sig, int n_samples, int n_ref, int select_max,
+                       SelectOutDev* out, hipStream_t s);
+int launch_refine(const RefineJobDev* jobs, int n_jobs, const uint8_t* img, int w, int h,
+                  int n_samples, int grid_n, const float* inv_ratio, int bank_rows, int bank_cols,
+                  float* scratch, RefineOutDev* out, hipStream_t s);
+
+// IdentBackend on the GPU: jobs packed into pinned memory, one launch per
+// step for all markers of the frame.  Errors are thrown as std::runtime_error.
+class GpuIdentBackend : public IdentBackend {
+ public:
+  int init(const std::vector<std::vector<float>>& bank, const HostParams& p, hipStream_t s);
+  void set_image(const uint8_t* dev_img, int w, int h) {
+    img_ = dev_img;
+    w_ = w;
+    h_ = h;
+  }
+  void select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
+                   std::vector<CutSelection>& out) override;
+  void refine(const std::vector<RefineJob>& jobs, const HostParams& p,
+              std::vector<RefineResult>& out) override;
+  // Device time of the last select / refine launches, microseconds.
+  float last_select_us() const { return select_us_; }
+  float last_refine_us() const { return refine_us_; }
+
+ private:
+  void ensure(int n_markers, int n_points);
+  hipStream_t stream_ = nullptr;
+  const uint8_t* img_ = nullptr;
+  int w_ = 0, h_ = 0;
+  IdentConstants k_{};
+  int cap_markers_ = 0, cap_points_ = 0;
+  DeviceArena dev_;
+  PinnedBuffer h_sel_jobs_, h_pts_, h_sel_out_, h_ref_jobs_, h_ref_out_;
+  SelectJobDev* d_sel_jobs_ = nullptr;
+  float4* d_pts_ = nullptr;
+  SelectOutDev* d_sel_out_ = nullptr;
+  RefineJobDev* d_ref_jobs_ = nullptr;
+  RefineOutDev* d_ref_out_ = nullptr;
+  float* d_scratch_ = nullptr;
+  float* d_inv_ratio_ = nullptr;
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  float select_us_ = 0, refine_us_ = 0;
+};
+
+}  // namespace cctagx

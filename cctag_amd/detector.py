@@ -54,6 +54,22 @@ class Detector:
         _lib.check(self._lib.cctagx_front_end(self._h, img.ctypes.data, img.strides[0]),
                    "cctagx_front_end")
 
+    def detect(self, img: np.ndarray, frame: int = 0, max_markers: int = 1024) -> list:
+        """cctagDetection (ICCTag.hpp:124-131): every marker found, sorted by
+        id; only status == 1 markers are identified."""
+        img = self._gray(img)
+        buf = (_lib.Marker * max_markers)()
+        n = ctypes.c_int()
+        _lib.check(self._lib.cctagx_detect(self._h, img.ctypes.data, img.strides[0], frame, buf,
+                                           max_markers, ctypes.byref(n)), "cctagx_detect")
+        return [buf[i].as_dict() for i in range(min(n.value, max_markers))]
+
+    def detect_times(self) -> dict:
+        host, sel, ref = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        _lib.check(self._lib.cctagx_detect_times(self._h, ctypes.byref(host), ctypes.byref(sel),
+                                                 ctypes.byref(ref)), "cctagx_detect_times")
+        return {"host_us": host.value, "select_us": sel.value, "refine_us": ref.value}
+
     def level(self, level: int) -> dict:
         v = _lib.LevelView()
         _lib.check(self._lib.cctagx_get_level(self._h, level, ctypes.byref(v)),

@@ -131,6 +131,12 @@ typedef struct cctagx_marker {
 int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
                   cctagx_marker* out, int max_out, int* n_out);
 
+/* Breakdown of the last cctagx_detect: host wall time after the front end
+ * (staging, candidate stage, identification control) and device time of the
+ * two identification kernels, microseconds.  The reference's logtime::Mgmt
+ * probes (Detection.cpp:789-964) report the same stage boundaries. */
+int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, float* refine_us);
+
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
  * events on the library stream; names are static strings. */

@@ -1,0 +1,75 @@
+"""End-to-end detection through the C ABI (GPU front end + host candidate
+stage + GPU identification) against the CPU oracle pipeline on the same
+frames: the same markers with identical IDs and statuses, imaged centres
+within 0.05 px, and the synthetic ground truth recovered."""
+import math
+
+import numpy as np
+import pytest
+
+from cctag_amd.detector import Detector
+from tests import frames, oracle_lib
+
+CENTER_TOL_PX = 0.05  # GPU vs CPU oracle, imaged centre
+
+
+def match(gpu, ref):
+    """Pairs markers by list position after checking the lists agree."""
+    assert len(gpu) == len(ref), (len(gpu), len(ref))
+    for g, r in zip(gpu, ref):
+        assert g["id"] == r["id"] and g["status"] == r["status"], (g, r)
+        assert g["pyramid_level"] == r["pyramid_level"], (g, r)
+        if r["status"] == 1:
+            d = math.hypot(g["x"] - r["x"], g["y"] - r["y"])
+            assert d <= CENTER_TOL_PX, ("centre", d, g, r)
+
+
+def gt_errors(markers, gts):
+    errs, ids_ok = [], 0
+    for m in markers:
+        if m["status"] != 1:
+            continue
+        d, g = min((math.hypot(m["x"] - g.center[0], m["y"] - g.center[1]), g) for g in gts)
+        errs.append(d)
+        ids_ok += int(g.id == m["id"])
+    return errs, ids_ok
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", [1, 2])
+def test_detect_matches_oracle(config):
+    img, gts = frames.config_frame(config)
+    print("detect parity config", config, img.shape, flush=True)
+    ref = oracle_lib.detect(img)
+    h, w = img.shape
+    with Detector(w, h) as det:
+        got = det.detect(img)
+    match(got, ref)
+    errs, ids_ok = gt_errors(got, gts)
+    assert ids_ok == len(errs) and len(errs) >= len(gts) - 1
+    assert max(errs) < 0.5
+
+
+@pytest.mark.gpu
+def test_detect_small_frame_and_repeat():
+    img, gts = frames.small_marker_frame()
+    h, w = img.shape
+    ref = oracle_lib.detect(img)
+    with Detector(w, h) as det:
+        a = det.detect(img)
+        b = det.detect(img)
+    match(a, ref)
+    assert a == b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["blank", "noise", "checker"])
+def test_detect_edge_cases(name):
+    img = frames.edge_case(name)
+    h, w = img.shape
+    ref = oracle_lib.detect(img)
+    with Detector(w, h) as det:
+        got = det.detect(img)
+    match(got, ref)
+    if name == "blank":
+        assert got == []
