@@ -155,6 +155,9 @@ def load() -> ctypes.CDLL:
     lib.cctagx_time_front_end.argtypes = [vp, ctypes.c_int, P(c_f32)]
     lib.cctagx_detect.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_uint64, P(Marker),
                                   ctypes.c_int, P(ctypes.c_int)]
+    lib.cctagx_load.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.cctagx_detect_loaded.argtypes = [vp, ctypes.c_uint64, P(Marker), ctypes.c_int,
+                                         P(ctypes.c_int)]
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p

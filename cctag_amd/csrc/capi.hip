@@ -63,6 +63,7 @@ struct cctagx_handle {
   GpuIdentBackend ident;
   std::vector<Marker> markers;
   double host_ms = 0;  // last detection: host wall time after the front end
+  bool have_frame = false;
 };
 
 static int validate(const cctagx_params* p) {
@@ -228,13 +229,27 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   return CCTAGX_OK;
 }
 
+int cctagx_load(cctagx_handle* h, const uint8_t* gray, size_t stride) {
+  if (!h || !gray || stride < (size_t)h->width) return CCTAGX_ERR_ARG;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  h->have_frame = true;
+  return CCTAGX_OK;
+}
+
 int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
                   cctagx_marker* out, int max_out, int* n_out) {
-  (void)frame;
   if (!h || !gray || stride < (size_t)h->width || (max_out > 0 && !out) || !n_out)
     return CCTAGX_ERR_ARG;
+  if (int rc = cctagx_load(h, gray, stride)) return rc;
+  return cctagx_detect_loaded(h, frame, out, max_out, n_out);
+}
+
+int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
+                         int* n_out) {
+  (void)frame;
+  if (!h || (max_out > 0 && !out) || !n_out) return CCTAGX_ERR_ARG;
+  if (!h->have_frame) return CCTAGX_ERR_STATE;
   *n_out = 0;
-  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
   h->timer.reset();
   if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
   h->have_front_end = true;
@@ -281,6 +296,7 @@ int cctagx_destroy(cctagx_handle* h) {
 int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, size_t stride) {
   if (!h || !gray || stride < (size_t)h->width) return CCTAGX_ERR_ARG;
   if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  h->have_frame = true;
   h->timer.reset();
   if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
   CCTAGX_HIP(hipStreamSynchronize(h->stream));

@@ -146,7 +146,8 @@ void identify_markers(std::vector<Marker>& markers, const HostParams& p,
     float ns = p.imaged_center_neighbour_size;
     int it = 0;
     const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
-    while (ns * r.max_semi_axis > 0.02) {
+    // (bounded: a non-finite semi-axis would never end the reference's loop)
+    while (ns * r.max_semi_axis > 0.02 && it < 64) {
       ++it;
       ns /= div;
     }

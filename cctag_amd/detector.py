@@ -64,6 +64,20 @@ class Detector:
                                            max_markers, ctypes.byref(n)), "cctagx_detect")
         return [buf[i].as_dict() for i in range(min(n.value, max_markers))]
 
+    def load(self, img: np.ndarray) -> None:
+        """TagPipe::load (cuda/tag.cu:154-159): the frame stays resident."""
+        img = self._gray(img)
+        _lib.check(self._lib.cctagx_load(self._h, img.ctypes.data, img.strides[0]),
+                   "cctagx_load")
+
+    def detect_loaded(self, frame: int = 0, max_markers: int = 1024) -> list:
+        """Detection on the resident frame (no host-to-device copy)."""
+        buf = (_lib.Marker * max_markers)()
+        n = ctypes.c_int()
+        _lib.check(self._lib.cctagx_detect_loaded(self._h, frame, buf, max_markers,
+                                                  ctypes.byref(n)), "cctagx_detect_loaded")
+        return [buf[i].as_dict() for i in range(min(n.value, max_markers))]
+
     def detect_times(self) -> dict:
         host, sel, ref = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         _lib.check(self._lib.cctagx_detect_times(self._h, ctypes.byref(host), ctypes.byref(sel),

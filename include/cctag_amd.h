@@ -131,6 +131,15 @@ typedef struct cctagx_marker {
 int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
                   cctagx_marker* out, int max_out, int* n_out);
 
+/* The two halves of cctagx_detect.  cctagx_load is TagPipe::load
+ * (cuda/tag.cu:154-159): the frame is copied into device memory and stays
+ * resident.  cctagx_detect_loaded runs the detection on the resident frame
+ * (TagPipe::tagframe onwards), so a caller can time or repeat detection
+ * without the host-to-device copy. */
+int cctagx_load(cctagx_handle* h, const uint8_t* gray, size_t stride);
+int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
+                         int* n_out);
+
 /* Breakdown of the last cctagx_detect: host wall time after the front end
  * (staging, candidate stage, identification control) and device time of the
  * two identification kernels, microseconds.  The reference's logtime::Mgmt
