@@ -93,6 +93,19 @@ class Marker(ctypes.Structure):
                 "homography": list(self.homography)}
 
 
+class DetectStats(ctypes.Structure):
+    """cctagx_detect_stats."""
+    _fields_ = [
+        ("front_end_us", c_f32), ("stage_us", c_f32), ("host_us", c_f32),
+        ("select_us", c_f32), ("refine_us", c_f32),
+        ("n_select", c_i32), ("n_refine", c_i32),
+        ("select_samples", ctypes.c_int64), ("refine_samples", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _arr(p, n, dtype):
     if n == 0 or not p:
         return np.zeros(0, dtype=dtype)
@@ -159,6 +172,7 @@ def load() -> ctypes.CDLL:
     lib.cctagx_detect_loaded.argtypes = [vp, ctypes.c_uint64, P(Marker), ctypes.c_int,
                                          P(ctypes.c_int)]
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]
+    lib.cctagx_get_detect_stats.argtypes = [vp, P(DetectStats)]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int

@@ -63,6 +63,7 @@ struct cctagx_handle {
   GpuIdentBackend ident;
   std::vector<Marker> markers;
   double host_ms = 0;  // last detection: host wall time after the front end
+  double stage_ms = 0; // last detection: wall time of the level staging copies
   bool have_frame = false;
 };
 
@@ -254,8 +255,11 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
   h->have_front_end = true;
   h->level_loaded.assign(h->level_loaded.size(), false);
+  const auto ts = std::chrono::steady_clock::now();
   if (int rc = stage_levels(h)) return rc;
   const auto t0 = std::chrono::steady_clock::now();
+  h->stage_ms = std::chrono::duration<double, std::milli>(t0 - ts).count();
+  h->ident.clear_stats();
   try {
     std::vector<LevelEdges> views;
     for (const StagedLevel& S : h->staged) views.push_back(S.view);
@@ -280,6 +284,25 @@ int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, floa
   *host_us = (float)(h->host_ms * 1000.0);
   *select_us = h->ident.last_select_us();
   *refine_us = h->ident.last_refine_us();
+  return CCTAGX_OK;
+}
+
+int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
+  if (!h || !st) return CCTAGX_ERR_ARG;
+  *st = cctagx_detect_stats{};
+  if (h->timer.used >= 2) {
+    float ms = 0.f;
+    CCTAGX_HIP(hipEventElapsedTime(&ms, h->timer.events[0], h->timer.events[h->timer.used - 1]));
+    st->front_end_us = ms * 1000.f;
+  }
+  st->stage_us = (float)(h->stage_ms * 1000.0);
+  st->host_us = (float)(h->host_ms * 1000.0);
+  st->select_us = h->ident.last_select_us();
+  st->refine_us = h->ident.last_refine_us();
+  st->n_select = h->ident.last_select_jobs();
+  st->n_refine = h->ident.last_refine_jobs();
+  st->select_samples = h->ident.last_select_samples();
+  st->refine_samples = h->ident.last_refine_samples();
   return CCTAGX_OK;
 }
 

@@ -896,10 +896,12 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   k_thin2<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.tmp, b.edges);
   mark("thin2");
   k_compact_count<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.tcount);
+  mark("compact_count");
   k_compact_scan<<<np, 1024, 0, s>>>(P, np, b.tcount, b.counts);
+  mark("compact_scan");
   k_compact_write<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.dx, b.dy, b.tcount, b.emap,
                                                b.ex, b.ey, b.egx, b.egy, b.vcount, b.seedmark);
-  mark("compact");
+  mark("compact_write");
   const dim3 g(L.edge_blocks, np);
   k_descent<<<g, 256, 0, s>>>(P, b.counts, b.emap, b.ex, b.ey, b.egx, b.egy,
                               L.prm.dist_search, L.prm.thr_grad, b.before, b.after);
@@ -909,11 +911,14 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
                            b.tdist, b.vcount);
   mark("vote");
   k_voters_scan<<<np, 1024, 0, s>>>(P, b.counts, b.vcount, b.vindex, b.ismax);
+  mark("voters_scan");
   k_voters_fill<<<g, 256, 0, s>>>(P, b.counts, b.chosen, b.ismax, b.vlist);
+  mark("voters_fill");
   k_voters_finish<<<g, 256, 0, s>>>(P, b.counts, b.vindex, b.tdist, L.prm.min_votes, b.vlist,
                                     b.flow, b.ismax, b.seedmark);
+  mark("voters_finish");
   k_seeds<<<np, 1024, 0, s>>>(P, b.counts, b.seedmark, b.seeds);
-  mark("voters_seeds");
+  mark("seeds");
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -69,6 +69,8 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
   int npts = 0;
   for (const CutJob& j : jobs) npts += (int)j.outer_points.size();
   ensure(n, npts);
+  select_jobs_ = n;
+  select_samples_ = (long long)npts * (k_.n_samples + k_.n_refine_samples);
   SelectJobDev* hj = h_sel_jobs_.as<SelectJobDev>();
   float4* hp = h_pts_.as<float4>();
   int off = 0;
@@ -107,6 +109,11 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   const int n = (int)jobs.size();
   if (n == 0) return;
   ensure(n, 0);
+  refine_jobs_ = n;
+  refine_samples_ = 0;
+  const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
+  for (const RefineJob& r : jobs)
+    refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
   RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
   for (int i = 0; i < n; ++i) {
     const RefineJob& r = jobs[i];

@@ -70,6 +70,16 @@ class GpuIdentBackend : public IdentBackend {
   // Device time of the last select / refine launches, microseconds.
   float last_select_us() const { return select_us_; }
   float last_refine_us() const { return refine_us_; }
+  // Work of the last launches: markers and bilinear image samples.
+  int last_select_jobs() const { return select_jobs_; }
+  int last_refine_jobs() const { return refine_jobs_; }
+  long long last_select_samples() const { return select_samples_; }
+  long long last_refine_samples() const { return refine_samples_; }
+  void clear_stats() {
+    select_us_ = refine_us_ = 0;
+    select_jobs_ = refine_jobs_ = 0;
+    select_samples_ = refine_samples_ = 0;
+  }
 
  private:
   void ensure(int n_markers, int n_points);
@@ -89,6 +99,8 @@ class GpuIdentBackend : public IdentBackend {
   float* d_inv_ratio_ = nullptr;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   float select_us_ = 0, refine_us_ = 0;
+  int select_jobs_ = 0, refine_jobs_ = 0;
+  long long select_samples_ = 0, refine_samples_ = 0;
 };
 
 }  // namespace cctagx

@@ -84,6 +84,13 @@ class Detector:
                                                  ctypes.byref(ref)), "cctagx_detect_times")
         return {"host_us": host.value, "select_us": sel.value, "refine_us": ref.value}
 
+    def detect_stats(self) -> dict:
+        """Time and work of the last detection (cctagx_get_detect_stats)."""
+        st = _lib.DetectStats()
+        _lib.check(self._lib.cctagx_get_detect_stats(self._h, ctypes.byref(st)),
+                   "cctagx_get_detect_stats")
+        return st.as_dict()
+
     def level(self, level: int) -> dict:
         v = _lib.LevelView()
         _lib.check(self._lib.cctagx_get_level(self._h, level, ctypes.byref(v)),

@@ -146,6 +146,25 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
  * probes (Detection.cpp:789-964) report the same stage boundaries. */
 int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, float* refine_us);
 
+/* Where the time and work of the last detection went: device time of the
+ * front end (first to last kernel), host wall time of staging the level
+ * outputs and of the host part after it, device time of the two
+ * identification kernels, the markers each received and the bilinear image
+ * samples each evaluated.  Counterpart of the reference's logtime::Mgmt
+ * stage probes (Detection.cpp:789-964). */
+typedef struct cctagx_detect_stats {
+  float front_end_us;
+  float stage_us;
+  float host_us;
+  float select_us;
+  float refine_us;
+  int32_t n_select;
+  int32_t n_refine;
+  int64_t select_samples;
+  int64_t refine_samples;
+} cctagx_detect_stats;
+int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);
+
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
  * events on the library stream; names are static strings. */
