@@ -202,7 +202,10 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   if (!out || width <= 0 || height <= 0) return CCTAGX_ERR_ARG;
   *out = nullptr;
   if (int rc = validate(params)) return rc;
-  if (cctagx_device_count() <= 0) return CCTAGX_ERR_NO_DEVICE;
+  if (cctagx_device_count() <= 0) {
+    set_error("no HIP device visible");
+    return CCTAGX_ERR_NO_DEVICE;
+  }
   std::unique_ptr<cctagx_handle> h(new (std::nothrow) cctagx_handle());
   if (!h) return CCTAGX_ERR_ARG;
   h->params = *params;
