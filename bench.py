@@ -101,6 +101,7 @@ def main():
     img, gts = synth.config_frame(2, 0)
     h, w = img.shape
     det = Detector(w, h)
+    host_threads = det.host_threads()
     det.load(img)
     markers = []
     for _ in range(args.warmup):
@@ -199,7 +200,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain 1920x1080 uint8 frame, 16 three-ring markers, "
                                "4-level pyramid, default parameters, frame resident in HBM",
-                   "width": w, "height": h, "markers": len(gts), "pipes": 1},
+                   "width": w, "height": h, "markers": len(gts), "pipes": 1,
+                   "host_threads": host_threads},
         "roofline": roof,
         "cpu_baseline": cpu,
         "vs_cpu": (fps / cpu["value"]) if cpu else None,

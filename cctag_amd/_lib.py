@@ -173,6 +173,8 @@ def load() -> ctypes.CDLL:
                                          P(ctypes.c_int)]
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]
     lib.cctagx_get_detect_stats.argtypes = [vp, P(DetectStats)]
+    lib.cctagx_set_host_threads.argtypes = [vp, ctypes.c_int]
+    lib.cctagx_get_host_threads.argtypes = [vp]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int

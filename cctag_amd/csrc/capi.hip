@@ -62,6 +62,7 @@ struct cctagx_handle {
   PinnedBuffer counts;
   GpuIdentBackend ident;
   std::vector<Marker> markers;
+  std::unique_ptr<TaskPool> pool;  // host candidate-stage workers
   double host_ms = 0;  // last detection: host wall time after the front end
   double stage_ms = 0; // last detection: wall time of the level staging copies
   bool have_frame = false;
@@ -225,6 +226,8 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   h->bank = marker_bank(params->n_crowns);
   try {
     if (int rc = h->ident.init(h->bank, h->host, h->stream)) return rc;
+    const int threads = TaskPool::default_threads();
+    if (threads > 1) h->pool = std::make_unique<TaskPool>(threads);
   } catch (const std::exception& e) {
     set_error(e.what());
     return CCTAGX_ERR_HIP;
@@ -268,7 +271,7 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
     for (const StagedLevel& S : h->staged) views.push_back(S.view);
     const PlaneDesc& d0 = h->fe.planes().planes[0];
     h->ident.set_image(h->fe.buffers().img + d0.pix, d0.w, d0.h);
-    detect_markers(views, h->host, h->bank, &h->ident, h->markers);
+    detect_markers(views, h->host, h->bank, &h->ident, h->markers, h->pool.get());
   } catch (const std::exception& e) {
     set_error(e.what());
     return CCTAGX_ERR_HIP;
@@ -288,6 +291,24 @@ int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, floa
   *select_us = h->ident.last_select_us();
   *refine_us = h->ident.last_refine_us();
   return CCTAGX_OK;
+}
+
+int cctagx_set_host_threads(cctagx_handle* h, int threads) {
+  if (!h || threads < 0) return CCTAGX_ERR_ARG;
+  if (threads == 0) threads = TaskPool::default_threads();
+  try {
+    h->pool.reset();
+    if (threads > 1) h->pool = std::make_unique<TaskPool>(threads);
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_ARG;
+  }
+  return CCTAGX_OK;
+}
+
+int cctagx_get_host_threads(cctagx_handle* h) {
+  if (!h) return CCTAGX_ERR_ARG;
+  return h->pool ? h->pool->threads() : 1;
 }
 
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {

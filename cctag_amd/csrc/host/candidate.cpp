@@ -1,7 +1,10 @@
-// Candidate stage (see candidate.hpp).  Serial restatement: the reference
-// runs three tbb::parallel_for loops here (Detection.cpp:616,644,680); run in
-// index order with one rand_5_k stream they give the result of its
-// CCTAG_SERIALIZE build (Detection.cpp:618,646,682).
+// Candidate stage (see candidate.hpp).  The reference runs its three loops
+// with tbb::parallel_for (Detection.cpp:616,644,680) over one shared
+// EdgePointCollection and a thread_local rand_5_k stream (Statistic.cpp:18),
+// so its result depends on the schedule.  Here loop one keeps the serial
+// seed order (a seed already linked into an earlier segment is skipped), and
+// every loop-two / loop-three task owns its random stream and scratch flags,
+// so tasks are independent and the result is the same on any thread count.
 #include "candidate.hpp"
 
 #include <algorithm>
@@ -33,6 +36,43 @@ inline float dist2d(float ax, float ay, float bx, float by) {
 }
 
 inline Pt2 pt_of(const EdgePoint& e) { return Pt2((float)e.x, (float)e.y, e.gx, e.gy); }
+
+// Per-candidate bits of EdgePoint::processed (EllipseGrowing.cpp:332): tasks
+// of different candidates touch different bits of the same words.
+inline void bits_set(uint64_t& w, uint64_t m) { __atomic_fetch_or(&w, m, __ATOMIC_RELAXED); }
+inline void bits_clear(uint64_t& w, uint64_t m) { __atomic_fetch_and(&w, ~m, __ATOMIC_RELAXED); }
+inline bool bits_test(const uint64_t& w, uint64_t m) {
+  return (__atomic_load_n(&w, __ATOMIC_RELAXED) & m) != 0;
+}
+
+// The auxiliary "processed" flags of one addCandidateFlow call
+// (Types.hpp:142-155), private to the calling thread and cleared on every
+// exit.  Testing a null point throws std::out_of_range like the reference.
+class AuxFlags {
+ public:
+  explicit AuxFlags(size_t n) : f_(storage()) {
+    if (f_.size() < n) f_.resize(n, 0);
+  }
+  ~AuxFlags() {
+    for (int q : touched_) f_[q] = 0;
+  }
+  bool test(int i) const {
+    if (i < 0) throw std::out_of_range("EdgePointCollection: null point");
+    return f_[i] != 0;
+  }
+  void set(int i) {
+    f_[i] = 1;
+    touched_.push_back(i);
+  }
+
+ private:
+  static std::vector<uint8_t>& storage() {
+    static thread_local std::vector<uint8_t> v;
+    return v;
+  }
+  std::vector<uint8_t>& f_;
+  std::vector<int> touched_;
+};
 }  // namespace
 
 void EdgeSet::load(const LevelEdges& L) {
@@ -59,7 +99,6 @@ void EdgeSet::load(const LevelEdges& L) {
     e.n_segment_out = -1;
   }
   pin_.assign(L.n, 0);
-  paux_.assign(L.n, 0);
   seeds.assign(L.seeds, L.seeds + L.n_seeds);
 }
 
@@ -384,7 +423,7 @@ static void connected_point(std::vector<int>& pts, uint64_t mask, EdgeSet& es, c
     int x, y, i;
   };
   std::vector<Frame> stack;
-  es.pts[es.at(x0, y0)].processed |= mask;
+  bits_set(es.pts[es.at(x0, y0)].processed, mask);
   stack.push_back({x0, y0, 0});
   while (!stack.empty()) {
     Frame& f = stack.back();
@@ -398,11 +437,11 @@ static void connected_point(std::vector<int>& pts, uint64_t mask, EdgeSet& es, c
     const int ie = es.at(sx, sy);
     if (ie < 0) continue;
     EdgePoint& e = es.pts[ie];
-    if (!is_in_hull(qin, qout, (float)e.x, (float)e.y) || (e.processed & mask)) continue;
+    if (!is_in_hull(qin, qout, (float)e.x, (float)e.y) || bits_test(e.processed, mask)) continue;
     const float ex = qin.cx() - e.x, ey = qin.cy() - e.y;
     if (e.gx * ex + e.gy * ey < 0) {
       pts.push_back(ie);
-      e.processed |= mask;
+      bits_set(e.processed, mask);
       stack.push_back({sx, sy, 0});
     }
   }
@@ -437,7 +476,7 @@ static void ellipse_growing2(EdgeSet& es, const std::vector<int>& fc, std::vecto
   outer.reserve(fc.size() * 3);
   for (int c : fc) {
     outer.push_back(c);
-    es.pts[c].processed |= mask;
+    bits_set(es.pts[c].processed, mask);
   }
   int lastSize = 0;
   int nIter = 0;
@@ -466,8 +505,8 @@ static void ellipse_growing2(EdgeSet& es, const std::vector<int>& fc, std::vecto
     outer = sets[nIterMax];
     ellipse = ellipses[nIterMax];
     for (auto& s : sets)
-      for (int i : s) es.pts[i].processed &= ~mask;
-    for (int i : outer) es.pts[i].processed |= mask;
+      for (int i : s) bits_clear(es.pts[i].processed, mask);
+    for (int i : outer) bits_set(es.pts[i].processed, mask);
   }
   lastSize = 0;
   fit_ellipse(as_points(es, outer), ellipse);
@@ -479,16 +518,13 @@ static void ellipse_growing2(EdgeSet& es, const std::vector<int>& fc, std::vecto
 }
 
 // EllipseGrowing.cpp:80-203.
-static bool add_candidate_flow(EdgeSet& es, const std::vector<int>& fc,
+static bool add_candidate_flow(const EdgeSet& es, const std::vector<int>& fc,
                                const std::vector<int>& outer_pts, const Ellipse& outer,
                                std::vector<std::vector<Pt2>>& cctagPoints, size_t numCircles) {
   cctagPoints.resize(numCircles);
   for (int e : outer_pts) cctagPoints.back().push_back(pt_of(es.pts[e]));
-  std::vector<int> processed;
+  AuxFlags aux(es.pts.size());
   size_t nGradientOut = 0, nAddedPoint = 0;
-  auto release = [&]() {
-    for (int q : processed) es.set_processed_aux(q, false);
-  };
   for (int p0 : fc) {
     int dir = -1;
     int p = p0;
@@ -498,11 +534,9 @@ static bool add_candidate_flow(EdgeSet& es, const std::vector<int>& fc,
     const Vec3 line{{a, b, -a * outer.cx() - b * outer.cy()}};
     for (size_t j = 1; j < numCircles; ++j) {
       p = dir == -1 ? es.before(p) : es.after(p);
-      // A missing link throws out of here with the flags still set, as the
-      // reference's test_processed_aux(nullptr) does (Types.hpp:150-155).
-      if (!es.test_processed_aux(p)) {
-        es.set_processed_aux(p, true);
-        processed.push_back(p);
+      // a missing link throws out of here (Types.hpp:150-155)
+      if (!aux.test(p)) {
+        aux.set(p);
         const EdgePoint& ep = es.pts[p];
         const float normGrad = std::sqrt(ep.gx * ep.gx + ep.gy * ep.gy);
         const float ge0 = ep.gx / normGrad, ge1 = ep.gy / normGrad;
@@ -517,14 +551,12 @@ static bool add_candidate_flow(EdgeSet& es, const std::vector<int>& fc,
           if (j >= numCircles - 2) ++nAddedPoint;
         } else {
           cctagPoints.clear();
-          release();
           return false;
         }
       }
       dir = -dir;
     }
   }
-  release();
   if (float(nGradientOut) / float(nAddedPoint) > 0.5f) {
     cctagPoints.clear();
     return false;
@@ -546,7 +578,7 @@ struct Candidate {
 }  // namespace
 
 // Vote.cpp:578-742.
-static bool is_another_segment(EdgeSet& es, Ellipse& outerEllipse, std::vector<int>& outerPts,
+static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::vector<int>& outerPts,
                                const Candidate& another,
                                std::vector<std::vector<Pt2>>& cctagPoints, size_t numCircles,
                                float thrMedian, Pcg32& rng) {
@@ -639,10 +671,21 @@ static void construct_flow_component(int seed, EdgeSet& es,
   loopOne.insert(it, std::move(cand));
 }
 
-// Detection.cpp:115-267.
-static void complete_flow_component(Candidate& cand, EdgeSet& es, std::vector<Candidate>& loopTwo,
-                                    size_t& nSegmentOut, size_t run_id, const HostParams& p,
-                                    Pcg32& rng) {
+// Detection.cpp:115-267, up to the segment label: what one loop-two task
+// computes.  `labeled` is set when the reference would have reached the
+// label assignment (enough filtered points, no exception before it).
+namespace {
+struct LoopTwoTask {
+  Candidate cand;
+  bool labeled = false;
+  bool accepted = false;
+};
+}  // namespace
+
+static void complete_flow_component(LoopTwoTask& t, EdgeSet& es, size_t run_id,
+                                    const HostParams& p) {
+  Candidate& cand = t.cand;
+  Pcg32 rng(271828u);
   try {
     std::vector<int> children;
     children_of(es, cand.convex_segment, children);
@@ -656,43 +699,50 @@ static void complete_flow_component(Candidate& cand, EdgeSet& es, std::vector<Ca
                     kInvGradWeight, 60, rng);
     TRACE("    filtered %zu SmFinal %g\n", filtered.size(), SmFinal);
     if (filtered.size() < 5) return;
-    size_t nLabel = (size_t)-1;
-    {
-      long nSegmentCommon = -1;
-      for (int q : filtered)
-        if (es.pts[q].n_segment_out != -1) {
-          nSegmentCommon = es.pts[q].n_segment_out;
-          break;
-        }
-      if (nSegmentCommon == -1) nLabel = nSegmentOut++;
-      else nLabel = (size_t)nSegmentCommon;
-      for (int q : filtered) es.pts[q].n_segment_out = (int32_t)nLabel;
-    }
+    t.labeled = true;
     std::vector<int>& outer = cand.outer_points;
     Ellipse& outerEllipse = cand.outer;
     const bool goodInit = ellipse_growing_init(es, filtered, outerEllipse);
     ellipse_growing2(es, filtered, outer, outerEllipse, p.ellipse_growing_elliptic_hull_width,
                      run_id, goodInit);
-    cand.label = nLabel;
     std::vector<float> vDist;
     for (int q : outer) vDist.push_back(distance_point_ellipse((float)es.pts[q].x, (float)es.pts[q].y, outerEllipse));
     SmFinal = median_ref(vDist);
-    TRACE("    grown c=(%.1f,%.1f) a=%.1f b=%.1f n=%zu Sm %g quality %g\n", outerEllipse.cx(), outerEllipse.cy(),
-          outerEllipse.a(), outerEllipse.b(), outer.size(), SmFinal,
-          (float)outer.size() / (float)rasterize_ellipse_perimeter(outerEllipse));
     if (SmFinal > p.thr_median_distance_ellipse) return;
     const float quality = (float)outer.size() / (float)rasterize_ellipse_perimeter(outerEllipse);
     if (quality > 1.1) return;
     const float ratio = outerEllipse.a() / outerEllipse.b();
     if ((ratio < 0.05) || (ratio > 20)) return;
-    loopTwo.push_back(cand);
+    t.accepted = true;
   } catch (const std::exception& ex) {
     TRACE("    exception %s\n", ex.what());
   }
 }
 
+// Detection.cpp:164-180: the segment label of each labelled candidate, in
+// candidate order -- the first label already carried by one of its
+// filtered points, else a new one.
+static void assign_labels(EdgeSet& es, std::vector<LoopTwoTask>& tasks,
+                          std::vector<Candidate>& loopTwo) {
+  size_t nSegmentOut = 0;
+  for (LoopTwoTask& t : tasks) {
+    if (!t.labeled) continue;
+    const std::vector<int>& filtered = t.cand.filtered_children;
+    long nSegmentCommon = -1;
+    for (int q : filtered)
+      if (es.pts[q].n_segment_out != -1) {
+        nSegmentCommon = es.pts[q].n_segment_out;
+        break;
+      }
+    const size_t nLabel = nSegmentCommon == -1 ? nSegmentOut++ : (size_t)nSegmentCommon;
+    for (int q : filtered) es.pts[q].n_segment_out = (int32_t)nLabel;
+    t.cand.label = nLabel;
+    if (t.accepted) loopTwo.push_back(std::move(t.cand));
+  }
+}
+
 // Detection.cpp:278-377.
-static void flow_component_assembling(EdgeSet& es, float& quality, size_t ic,
+static void flow_component_assembling(const EdgeSet& es, float& quality, size_t ic,
                                       const std::vector<Candidate>& loopTwo, Ellipse& outerEllipse,
                                       std::vector<int>& outerPts,
                                       std::vector<std::vector<Pt2>>& cctagPoints,
@@ -725,7 +775,7 @@ static void flow_component_assembling(EdgeSet& es, float& quality, size_t ic,
 }
 
 // Detection.cpp:379-566.
-static void loop_two_iteration(std::vector<Marker>& markers, EdgeSet& es,
+static void loop_two_iteration(std::vector<Marker>& markers, const EdgeSet& es,
                                const std::vector<Candidate>& loopTwo, size_t ic, int level,
                                float scale, const HostParams& p, Pcg32& rng) {
   const Candidate& cand = loopTwo[ic];
@@ -763,34 +813,54 @@ static void loop_two_iteration(std::vector<Marker>& markers, EdgeSet& es,
   }
 }
 
-void detection_from_edges(std::vector<Marker>& markers, EdgeSet& es, int level_h,
-                          int pyramid_level, float scale, const HostParams& p, Pcg32& rng) {
-  size_t nSegmentOut = 0;
-  if (es.seeds.empty()) return;
-  // Multiresolution.cpp:239-242: seeds by received votes, std::sort (its
-  // order among equal votes is the library's; the input order is the order in
-  // which the points reached min votes, as on the CPU).
-  if (es.seeds.size() > 1)
-    std::sort(es.seeds.begin(), es.seeds.end(),
-              [&](int a, int b) { return es.pts[a].is_max > es.pts[b].is_max; });
-  const size_t nMaxSeeds = (size_t)std::max(level_h / 2, p.maximum_nb_seeds);
-  const size_t nSeeds = std::min(es.seeds.size(), nMaxSeeds);
-  std::vector<std::unique_ptr<Candidate>> loopOne;
-  for (size_t i = 0; i < nSeeds; ++i) construct_flow_component(es.seeds[i], es, loopOne, p);
-  TRACE("level %d: %zu seeds, %zu loop-one candidates\n", pyramid_level, es.seeds.size(), loopOne.size());
-  const size_t nLoopTwo = std::min(loopOne.size(), (size_t)p.maximum_nb_candidates_loop_two);
-  std::vector<Candidate> loopTwo;
-  loopTwo.reserve(nLoopTwo);
-  for (size_t i = 0; i < nLoopTwo; ++i)
-    complete_flow_component(*loopOne[i], es, loopTwo, nSegmentOut, i, p, rng);
-  TRACE("level %d: %zu loop-two candidates\n", pyramid_level, loopTwo.size());
-  for (size_t i = 0; i < loopTwo.size(); ++i) {
-    TRACE("  cand %zu seed (%d,%d) ellipse c=(%.1f,%.1f) a=%.1f b=%.1f n=%zu\n", i, es.pts[loopTwo[i].seed].x, es.pts[loopTwo[i].seed].y,
-          loopTwo[i].outer.cx(), loopTwo[i].outer.cy(), loopTwo[i].outer.a(), loopTwo[i].outer.b(), loopTwo[i].outer_points.size());
-    [[maybe_unused]] const size_t before = markers.size();
-    loop_two_iteration(markers, es, loopTwo, i, pyramid_level, scale, p, rng);
-    TRACE("    -> %s\n", markers.size() > before ? "marker" : "rejected");
+void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
+                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers) {
+  const int nl = (int)levels.size();
+  // Loop one per level, in seed order (Detection.cpp:603-627).
+  std::vector<std::vector<std::unique_ptr<Candidate>>> loopOne(nl);
+  parallel_for(pool, nl, [&](int l) {
+    EdgeSet& es = levels[l];
+    if (es.seeds.empty()) return;
+    // Multiresolution.cpp:239-242: seeds by received votes, std::sort (its
+    // order among equal votes is the library's; the input order is the order
+    // in which the points reached min votes, as on the CPU).
+    if (es.seeds.size() > 1)
+      std::sort(es.seeds.begin(), es.seeds.end(),
+                [&](int a, int b) { return es.pts[a].is_max > es.pts[b].is_max; });
+    const size_t nMaxSeeds = (size_t)std::max(level_h[l] / 2, p.maximum_nb_seeds);
+    const size_t nSeeds = std::min(es.seeds.size(), nMaxSeeds);
+    for (size_t i = 0; i < nSeeds; ++i) construct_flow_component(es.seeds[i], es, loopOne[l], p);
+    TRACE("level %d: %zu seeds, %zu loop-one candidates\n", l, es.seeds.size(), loopOne[l].size());
+  });
+  // Loop two: one task per (level, candidate) (Detection.cpp:629-658).
+  std::vector<std::vector<LoopTwoTask>> two(nl);
+  std::vector<std::pair<int, int>> jobs;
+  for (int l = 0; l < nl; ++l) {
+    const size_t n = std::min(loopOne[l].size(), (size_t)p.maximum_nb_candidates_loop_two);
+    two[l].resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      two[l][i].cand = std::move(*loopOne[l][i]);
+      jobs.emplace_back(l, (int)i);
+    }
   }
+  parallel_for(pool, (int)jobs.size(), [&](int j) {
+    const int l = jobs[j].first, i = jobs[j].second;
+    complete_flow_component(two[l][i], levels[l], (size_t)i, p);
+  });
+  std::vector<std::vector<Candidate>> loopTwo(nl);
+  for (int l = 0; l < nl; ++l) assign_labels(levels[l], two[l], loopTwo[l]);
+  // Loop three: one task per (level, candidate) (Detection.cpp:675-688).
+  jobs.clear();
+  for (int l = nl - 1; l >= 0; --l)
+    for (size_t i = 0; i < loopTwo[l].size(); ++i) jobs.emplace_back(l, (int)i);
+  std::vector<std::vector<Marker>> found(jobs.size());
+  parallel_for(pool, (int)jobs.size(), [&](int j) {
+    const int l = jobs[j].first, i = jobs[j].second;
+    Pcg32 rng(271828u);
+    loop_two_iteration(found[j], levels[l], loopTwo[l], (size_t)i, l, (float)std::pow(2.0, l), p, rng);
+  });
+  for (std::vector<Marker>& f : found)
+    for (Marker& m : f) markers.push_back(std::move(m));
 }
 
 // Multiresolution.cpp:43-170.
@@ -825,9 +895,12 @@ static bool intersect_line_two_ellipses(int y, const Ellipse& qIn, const Ellipse
 }
 
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
-                     Pcg32& rng) {
-  for (Marker& m : markers) {
+                     TaskPool* pool) {
+  (void)p;
+  parallel_for(pool, (int)markers.size(), [&](int im) {
+    Marker& m = markers[im];
     if (m.pyramid_level > 0) {
+      Pcg32 rng(271828u);
       const float scale = m.scale;
       Ellipse rescaled = m.rescaled_outer_ellipse;
       std::vector<int> hull;
@@ -840,11 +913,11 @@ void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const 
         if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
       for (int y = minY; y >= 0; --y)
         if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
-      if (hull.size() < 5) continue;
+      if (hull.size() < 5) return;
       std::vector<int> refit;
       float SmFinal = 1e+10f;
       outlier_removal(level0, hull, refit, SmFinal, 20.0f, kNoWeight, 60, rng);
-      if (refit.size() < 5) continue;
+      if (refit.size() < 5) return;
       try {
         fit_ellipse(as_points(level0, refit), rescaled);
         std::vector<Pt2> pts;
@@ -857,7 +930,7 @@ void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const 
     } else {
       m.rescaled_outer_ellipse_points = m.points.back();
     }
-  }
+  });
 }
 
 void update_markers(std::vector<Marker>& markers, const Marker& add) {

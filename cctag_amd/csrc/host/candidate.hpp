@@ -1,7 +1,8 @@
 // Candidate stage: from one level's edge points, links, votes and seeds to
 // marker candidates (outer ellipse + ring points), then the level-0 re-fit of
-// markers found at coarser levels.  Serial host code; the front end (GPU, or
-// the oracle's CPU restatement) provides its input.
+// markers found at coarser levels.  Host code, task-parallel over seeds'
+// flow components, candidates and markers of all levels at once; the front
+// end (GPU, or the oracle's CPU restatement) provides its input.
 //
 // Reference: Detection.cpp:69-693, Vote.cpp:249-742, EllipseGrowing.cpp,
 // Fitting.cpp:228-364, Multiresolution.cpp:43-193,268-413, CCTag.{hpp,cpp}.
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "geometry.hpp"
+#include "pool.hpp"
 
 namespace cctagx {
 
@@ -57,8 +59,6 @@ class EdgeSet {
   // Types.hpp:142-155: an invalid index throws std::out_of_range.
   void set_processed_in(int i, bool f) { pin_.at(check(i)) = f; }
   bool test_processed_in(int i) const { return pin_.at(check(i)); }
-  void set_processed_aux(int i, bool f) { paux_.at(check(i)) = f; }
-  bool test_processed_aux(int i) const { return paux_.at(check(i)); }
 
   std::vector<EdgePoint> pts;
   std::vector<int32_t> seeds;
@@ -71,7 +71,7 @@ class EdgeSet {
   int w_ = 0, h_ = 0, shape_w_ = 0, shape_h_ = 0;
   const int32_t* map_ = nullptr;
   const int32_t *before_ = nullptr, *after_ = nullptr, *vindex_ = nullptr, *vlist_ = nullptr;
-  std::vector<uint8_t> pin_, paux_;
+  std::vector<uint8_t> pin_;
 };
 
 // The detection parameters the host stages read (Params.hpp:108-199).
@@ -118,14 +118,21 @@ struct Marker {
   bool is_equal(const Marker& other) const;
 };
 
-// cctagDetectionFromEdges (Detection.cpp:568-693), serial; rng is the single
-// rand_5_k stream of the call.
-void detection_from_edges(std::vector<Marker>& markers, EdgeSet& es, int level_h,
-                          int pyramid_level, float scale, const HostParams& p, Pcg32& rng);
+// cctagDetectionFromEdges (Detection.cpp:568-693) for every level at once,
+// appending markers coarse level first.  Loop one (seed -> flow component)
+// runs per level in seed order; loop two (completeFlowComponent) and the
+// final candidate loop run as independent tasks over all levels on `pool`
+// (serially when pool is null), each task with its own rand_5_k stream
+// seeded 271828 and its own auxiliary point flags; the segment labels of
+// loop two are assigned afterwards in candidate order.  The result does not
+// depend on the number of threads.
+void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
+                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers);
 
-// Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges.
+// Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges,
+// one task per marker (own rand_5_k stream seeded 271828).
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
-                     Pcg32& rng);
+                     TaskPool* pool);
 
 // Multiresolution.cpp:172-193.
 void update_markers(std::vector<Marker>& markers, const Marker& m);

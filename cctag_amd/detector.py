@@ -84,6 +84,13 @@ class Detector:
                                                  ctypes.byref(ref)), "cctagx_detect_times")
         return {"host_us": host.value, "select_us": sel.value, "refine_us": ref.value}
 
+    def set_host_threads(self, threads: int) -> None:
+        """Candidate-stage host threads (0 = default); results do not change."""
+        _lib.check(self._lib.cctagx_set_host_threads(self._h, threads), "cctagx_set_host_threads")
+
+    def host_threads(self) -> int:
+        return self._lib.cctagx_get_host_threads(self._h)
+
     def detect_stats(self) -> dict:
         """Time and work of the last detection (cctagx_get_detect_stats)."""
         st = _lib.DetectStats()

@@ -140,6 +140,13 @@ int cctagx_load(cctagx_handle* h, const uint8_t* gray, size_t stride);
 int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
                          int* n_out);
 
+/* Host threads of the candidate stage, the counterpart of the reference's
+ * tbb::parallel_for loops (Detection.cpp:616,644,680).  A handle starts with
+ * min(16, hardware threads) or $CCTAGX_HOST_THREADS; 0 restores that default,
+ * 1 runs the stage on the calling thread.  Results do not depend on it. */
+int cctagx_set_host_threads(cctagx_handle* h, int threads);
+int cctagx_get_host_threads(cctagx_handle* h);
+
 /* Breakdown of the last cctagx_detect: host wall time after the front end
  * (staging, candidate stage, identification control) and device time of the
  * two identification kernels, microseconds.  The reference's logtime::Mgmt

@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <exception>
+#include <memory>
 #include <vector>
 
 namespace {
@@ -132,21 +133,31 @@ oracle::Params front_params(const cctagx_params& c) {
 
 extern "C" {
 
-// cctagDetection on the CPU; returns the marker count or -1.
-int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
-                  cctagx_marker* out, int max_out) {
+// cctagDetection on the CPU; returns the marker count or -1.  `threads` > 1
+// runs the shared host candidate stage on a pool of that many threads (the
+// product's configuration), 1 serially.
+int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w, int h,
+                          cctagx_marker* out, int max_out, int threads) {
   try {
     std::vector<oracle::LevelResult> levels;
     oracle::front_end(img, w, h, front_params(*params), levels);
     const auto bank = cctagx::marker_bank(params->n_crowns);
     oracle::CpuIdentBackend backend(levels[0].img.data(), w, h, bank);
     std::vector<cctagx::Marker> markers;
-    cctagx::detect_markers(level_edges(levels), cctagx::host_params(*params), bank, &backend, markers);
+    std::unique_ptr<cctagx::TaskPool> pool;
+    if (threads > 1) pool = std::make_unique<cctagx::TaskPool>(threads);
+    cctagx::detect_markers(level_edges(levels), cctagx::host_params(*params), bank, &backend,
+                           markers, pool.get());
     for (size_t i = 0; i < markers.size() && (int)i < max_out; ++i) cctagx::to_c_marker(markers[i], out[i]);
     return (int)markers.size();
   } catch (const std::exception&) {
     return -1;
   }
+}
+
+int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
+                  cctagx_marker* out, int max_out) {
+  return oracle_detect_threads(params, img, w, h, out, max_out, 1);
 }
 
 // Wall time of the full CPU detection (front end + candidates + identification).

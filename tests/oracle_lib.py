@@ -39,6 +39,9 @@ def load():
     o.oracle_time_front_end.restype = ctypes.c_double
     o.oracle_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(_lib.Marker), ctypes.c_int]
+    o.oracle_detect_threads.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(_lib.Marker), ctypes.c_int,
+                                        ctypes.c_int]
     o.oracle_time_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
@@ -120,14 +123,15 @@ def downsample(img: np.ndarray) -> np.ndarray:
     return out
 
 
-def detect(img: np.ndarray, params=None) -> list:
-    """Full CPU detection (cctagDetection) -> list of marker dicts, sorted by id."""
+def detect(img: np.ndarray, params=None, threads: int = 1) -> list:
+    """Full CPU detection (cctagDetection) -> list of marker dicts, sorted by
+    id; threads > 1 runs the shared candidate stage on a thread pool."""
     o = load()
     img = np.ascontiguousarray(img, dtype=np.uint8)
     h, w = img.shape
     prm = params if params is not None else _lib.default_params(3)
     buf = (_lib.Marker * 512)()
-    n = o.oracle_detect(ctypes.byref(prm), img.ctypes.data, w, h, buf, 512)
+    n = o.oracle_detect_threads(ctypes.byref(prm), img.ctypes.data, w, h, buf, 512, threads)
     if n < 0:
         raise RuntimeError("oracle detection failed")
     return [buf[i].as_dict() for i in range(min(n, 512))]

@@ -73,3 +73,16 @@ def test_detect_edge_cases(name):
     match(got, ref)
     if name == "blank":
         assert got == []
+
+
+@pytest.mark.gpu
+def test_detect_host_threads_do_not_change_result():
+    img, _ = frames.config_frame(2)
+    h, w = img.shape
+    with Detector(w, h) as det:
+        print("host threads", det.host_threads(), flush=True)
+        a = det.detect(img)
+        det.set_host_threads(1)
+        assert det.host_threads() == 1
+        b = det.detect(img)
+    assert a == b

@@ -269,3 +269,11 @@ def test_oracle_detection_recovers_ground_truth():
     for m in ms:
         d, g = min((math.hypot(m["x"] - g.center[0], m["y"] - g.center[1]), g) for g in gts)
         assert d < 0.5 and g.id == m["id"], (d, g.id, m["id"])
+
+
+@pytest.mark.parametrize("threads", [2, 5])
+def test_candidate_stage_independent_of_thread_count(threads):
+    """The host candidate stage runs as independent tasks; the product runs
+    it on a thread pool, the oracle serially: both must agree exactly."""
+    img, _ = frames.config_frame(1)
+    assert oracle_lib.detect(img, threads=threads) == oracle_lib.detect(img)
