@@ -43,7 +43,7 @@ const T* ptr(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
 
 // Pinned host copy of the candidate stage's inputs for one level.
 struct StagedLevel {
-  PinnedBuffer emap, ex, ey, egx, egy, before, after, vindex, vlist, flow, ismax, seeds;
+  PinnedBuffer ebits, ex, ey, egx, egy, before, after, vindex, vlist, flow, ismax, seeds;
   LevelEdges view;
 };
 
@@ -111,7 +111,7 @@ static int stage_levels(cctagx_handle* h) {
       return 0;
     };
     int rc = 0;
-    rc |= get(S.emap, b.emap + d.pix, 4 * N);
+    rc |= get(S.ebits, b.ebits + (d.pix >> 5), 4 * ((N + 31) / 32));
     rc |= get(S.ex, b.ex + d.pix, 2 * n);
     rc |= get(S.ey, b.ey + d.pix, 2 * n);
     rc |= get(S.egx, b.egx + d.pix, 2 * n);
@@ -126,7 +126,7 @@ static int stage_levels(cctagx_handle* h) {
     if (rc) return CCTAGX_ERR_HIP;
     LevelEdges& v = S.view;
     v.w = d.w; v.h = d.h; v.shape_w = d.shape_w; v.shape_h = d.shape_h;
-    v.map = S.emap.as<int32_t>();
+    v.bits = S.ebits.as<uint32_t>();
     v.n = (int)n;
     v.x = S.ex.as<int16_t>(); v.y = S.ey.as<int16_t>();
     v.gx = S.egx.as<int16_t>(); v.gy = S.egy.as<int16_t>();

@@ -459,26 +459,33 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
 }
 
 // a7: raster-order compaction of edges == 255 (Canny.cpp:24-33) in 4096-px
-// chunks of the plane's linear pixel order.
+// chunks of the plane's linear pixel order.  Also writes the plane's edge
+// bitmask (32 pixels per word), which is what the host reads to locate
+// points by position.
 __global__ __launch_bounds__(256) void k_compact_count(const PlaneDesc* __restrict__ P, int np,
                                                         const uint8_t* __restrict__ edges,
-                                                        int32_t* __restrict__ tcount) {
+                                                        int32_t* __restrict__ tcount,
+                                                        uint32_t* __restrict__ ebits) {
   const int p = plane_of<F_TILE_C>(P, np, blockIdx.x);
   const PlaneDesc d = P[p];
   const int c = blockIdx.x - d.tile_c;
   const int n = d.w * d.h;
   const int b = c * kChunk + threadIdx.x * 16;
-  int cnt = 0;
+  uint32_t m16 = 0;  // bit q = pixel b + q
   if (b + 16 <= n) {
     const uint4 v = *reinterpret_cast<const uint4*>(edges + d.pix + b);
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) cnt += ((w[k] >> (8 * q)) & 0xffu) == 255u;
+      for (int q = 0; q < 4; ++q) m16 |= (uint32_t)(((w[k] >> (8 * q)) & 0xffu) == 255u) << (4 * k + q);
   } else {
-    for (int k = b; k < n && k < b + 16; ++k) cnt += edges[d.pix + k] == 255;
+    for (int k = b; k < n && k < b + 16; ++k) m16 |= (uint32_t)(edges[d.pix + k] == 255) << (k - b);
   }
+  const int cnt = __builtin_popcount(m16);
+  // even lanes own a 32-pixel word: their 16 bits and the next lane's
+  const uint32_t hi = __shfl_down(m16, 1, 64);
+  if ((threadIdx.x & 1) == 0 && b < n) ebits[(d.pix >> 5) + (b >> 5)] = m16 | (hi << 16);
   int total;
   block_exclusive_scan(cnt, &total);
   if (threadIdx.x == 0) tcount[blockIdx.x] = total;
@@ -895,7 +902,7 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   mark("thin1");
   k_thin2<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.tmp, b.edges);
   mark("thin2");
-  k_compact_count<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.tcount);
+  k_compact_count<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.tcount, b.ebits);
   mark("compact_count");
   k_compact_scan<<<np, 1024, 0, s>>>(P, np, b.tcount, b.counts);
   mark("compact_scan");

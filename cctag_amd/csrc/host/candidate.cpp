@@ -80,7 +80,14 @@ void EdgeSet::load(const LevelEdges& L) {
   h_ = L.h;
   shape_w_ = L.shape_w;
   shape_h_ = L.shape_h;
-  map_ = L.map;
+  bits_ = L.bits;
+  const size_t nw = ((size_t)L.w * L.h + 31) / 32;
+  rank_.resize(nw);
+  int32_t acc = 0;
+  for (size_t j = 0; j < nw; ++j) {
+    rank_[j] = acc;
+    acc += __builtin_popcount(bits_[j]);
+  }
   before_ = L.before;
   after_ = L.after;
   vindex_ = L.vindex;

@@ -20,7 +20,9 @@ namespace cctagx {
 struct LevelEdges {
   int w = 0, h = 0;              // level size
   int shape_w = 0, shape_h = 0;  // EdgePointCollection shape (level-0 size)
-  const int32_t* map = nullptr;  // w*h, point index or -1
+  // ceil(w*h/32) words, bit i of word j set when pixel 32j+i (raster order)
+  // is an edge point; a point's index is its rank among the set bits.
+  const uint32_t* bits = nullptr;
   int n = 0;
   const int16_t *x = nullptr, *y = nullptr, *gx = nullptr, *gy = nullptr;
   const int32_t *before = nullptr, *after = nullptr;
@@ -45,8 +47,14 @@ struct EdgePoint {
 class EdgeSet {
  public:
   void load(const LevelEdges& L);
-  int at(int x, int y) const {  // caller guarantees 0 <= x < shape_w, 0 <= y < shape_h
-    return (x < w_ && y < h_) ? map_[(size_t)y * w_ + x] : -1;
+  // Point index at (x, y) or -1; the caller guarantees 0 <= x < shape_w and
+  // 0 <= y < shape_h (the collection's shape is the level-0 size).
+  int at(int x, int y) const {
+    if (x >= w_ || y >= h_) return -1;
+    const size_t i = (size_t)y * w_ + x;
+    const uint32_t word = bits_[i >> 5], bit = 1u << (i & 31);
+    if (!(word & bit)) return -1;
+    return rank_[i >> 5] + __builtin_popcount(word & (bit - 1));
   }
   int shape_w() const { return shape_w_; }
   int shape_h() const { return shape_h_; }
@@ -69,7 +77,8 @@ class EdgeSet {
     return i;
   }
   int w_ = 0, h_ = 0, shape_w_ = 0, shape_h_ = 0;
-  const int32_t* map_ = nullptr;
+  const uint32_t* bits_ = nullptr;
+  std::vector<int32_t> rank_;  // set bits before each word
   const int32_t *before_ = nullptr, *after_ = nullptr, *vindex_ = nullptr, *vlist_ = nullptr;
   std::vector<uint8_t> pin_;
 };

@@ -90,7 +90,12 @@ struct MeanVar {
 };
 
 // Identification.cpp:396-459 for a cut whose signal is only consumed
-// sample by sample; returns false when a sample falls outside the image.
+// sample by sample; returns false when a sample falls outside the image
+// (the caller then discards what f saw).  Sample positions follow the
+// reference's running sums; the image reads of kWalkBatch samples are issued
+// together so their latencies overlap.
+constexpr int kWalkBatch = 8;
+
 template <class F>
 __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, float begin, float end,
                          int n, F&& f) {
@@ -111,12 +116,26 @@ __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, 
     ye = ey;
   }
   const float stepX = (xe - xs) / ((float)n - 1.f), stepY = (ye - ys) / ((float)n - 1.f);
+  const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
   float x = xs, y = ys;
-  for (int i = 0; i < n; ++i) {
-    if (x >= 1.f && x < (float)(im.w - 1) && y >= 1.f && y < (float)(im.h - 1)) f(i, im.bilinear(x, y));
-    else return false;
-    x += stepX;
-    y += stepY;
+  for (int i0 = 0; i0 < n; i0 += kWalkBatch) {
+    float px[kWalkBatch], py[kWalkBatch];
+    bool inside = true;
+#pragma unroll
+    for (int t = 0; t < kWalkBatch; ++t) {
+      px[t] = x;
+      py[t] = y;
+      if (i0 + t < n) inside &= (x >= 1.f && x < xmax && y >= 1.f && y < ymax);
+      x += stepX;
+      y += stepY;
+    }
+    if (!inside) return false;
+    float v[kWalkBatch];
+#pragma unroll
+    for (int t = 0; t < kWalkBatch; ++t) v[t] = i0 + t < n ? im.bilinear(px[t], py[t]) : 0.f;
+#pragma unroll
+    for (int t = 0; t < kWalkBatch; ++t)
+      if (i0 + t < n) f(i0 + t, v[t]);
   }
   return true;
 }
@@ -249,6 +268,7 @@ __global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restr
 
 // ---------------------------------------------------------------------------
 constexpr int kRefineThreads = 1024;
+constexpr int kRefineChunk = 10;  // samples per sampling work item
 
 __global__ __launch_bounds__(kRefineThreads) void k_refine(
     const RefineJobDev* __restrict__ jobs, Img im, int n_samples, int grid_n,
@@ -257,7 +277,7 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
   constexpr int G2MAX = kMaxGrid * kMaxGrid;
   __shared__ float sH[G2MAX][9], sHi[G2MAX][9];
   __shared__ float gx[G2MAX], gy[G2MAX];
-  __shared__ uint8_t oob_here[G2MAX][kMaxCuts];
+  __shared__ int oob_here[G2MAX][kMaxCuts];
   __shared__ uint8_t valid[G2MAX][kMaxCuts];
   __shared__ uint8_t sticky[kMaxCuts];
   __shared__ int mvalid[G2MAX];
@@ -318,9 +338,15 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
       }
     }
     __syncthreads();
-    // extractSignalUsingHomography for every (grid point, cut)
-    for (int pk = tid; pk < G2 * nc; pk += kRefineThreads) {
-      const int g = pk / nc, k = pk - g * nc;
+    // extractSignalUsingHomography for every (grid point, cut), split into
+    // chunks of kRefineChunk samples so that all lanes gather at once; a
+    // chunk replays the running sums of the positions before it.
+    for (int q = tid; q < G2 * nc; q += kRefineThreads) oob_here[q / nc][q % nc] = 0;
+    __syncthreads();
+    const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
+    for (int pk = tid; pk < G2 * nc * nchunk; pk += kRefineThreads) {
+      const int c = pk % nchunk, gk = pk / nchunk;
+      const int g = gk / nc, k = gk - g * nc;
       float bx, by;
       apply_h(bx, by, sHi[g], J.stops[k].x, J.stops[k].y);
       float xs, ys, xe, ye;
@@ -340,19 +366,31 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
       }
       const float stepX = (xe - xs) / ((float)n_samples - 1.f), stepY = (ye - ys) / ((float)n_samples - 1.f);
       float x = xs, y = ys;
-      bool oob = false;
-      float* dst = scr + ((size_t)g * kMaxCuts + k) * kMaxSamples;
-      for (int i = 0; i < n_samples; ++i) {
-        float xr, yr;
-        apply_h(xr, yr, sH[g], x, y);
-        if (xr >= 0.f && xr < (float)(im.w - 1) && yr >= 0.f && yr < (float)(im.h - 1))
-          dst[i] = im.bilinear(xr, yr);
-        else
-          oob = true;
+      const int i0 = c * kRefineChunk;
+      for (int i = 0; i < i0; ++i) {
         x += stepX;
         y += stepY;
       }
-      oob_here[g][k] = oob;
+      const float* H = sH[g];
+      const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
+      float v[kRefineChunk];
+      bool in[kRefineChunk];
+      bool oob = false;
+#pragma unroll
+      for (int t = 0; t < kRefineChunk; ++t) {
+        float xr, yr;
+        apply_h(xr, yr, H, x, y);
+        in[t] = i0 + t < n_samples && xr >= 0.f && xr < xmax && yr >= 0.f && yr < ymax;
+        oob |= (i0 + t < n_samples) && !in[t];
+        v[t] = in[t] ? im.bilinear(xr, yr) : 0.f;
+        x += stepX;
+        y += stepY;
+      }
+      float* dst = scr + ((size_t)g * kMaxCuts + k) * kMaxSamples + i0;
+#pragma unroll
+      for (int t = 0; t < kRefineChunk; ++t)
+        if (in[t]) dst[t] = v[t];
+      if (oob) atomicOr(&oob_here[g][k], 1);
     }
     __syncthreads();
     // sticky out-of-bounds flags, in grid-point evaluation order

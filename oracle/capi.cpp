@@ -104,7 +104,7 @@ std::vector<cctagx::LevelEdges> level_edges(const std::vector<oracle::LevelResul
   for (const oracle::LevelResult& L : levels) {
     cctagx::LevelEdges e;
     e.w = L.w; e.h = L.h; e.shape_w = L.W0; e.shape_h = L.H0;
-    e.map = L.edge_map.data();
+    e.bits = L.edge_bits.data();
     e.n = (int)L.ex.size();
     e.x = L.ex.data(); e.y = L.ey.data(); e.gx = L.edx.data(); e.gy = L.edy.data();
     e.before = L.before.data(); e.after = L.after.data();

@@ -373,12 +373,14 @@ static float neg_dot(const LevelResult& L, int a, int b) {  // -grad(a).grad(b)
 void edges_and_vote(LevelResult& L, const Params& p) {
   const int w = L.w, h = L.h;
   L.edge_map.assign((size_t)w * h, -1);
+  L.edge_bits.assign(((size_t)w * h + 31) / 32, 0u);
   L.ex.clear(); L.ey.clear(); L.edx.clear(); L.edy.clear();
   for (int y = 0; y < h; ++y)
     for (int x = 0; x < w; ++x) {
       const size_t o = (size_t)y * w + x;
       if (L.edges[o] == 255) {
         L.edge_map[o] = (int)L.ex.size();
+        L.edge_bits[o >> 5] |= 1u << (o & 31);
         L.ex.push_back((int16_t)x);
         L.ey.push_back((int16_t)y);
         L.edx.push_back(L.dx[o]);

@@ -36,6 +36,7 @@ struct LevelResult {
   std::vector<int16_t> dx, dy;
   std::vector<uint8_t> edges;       // thinned Canny map, {0,255}
   std::vector<int32_t> edge_map;    // w*h, point index or -1
+  std::vector<uint32_t> edge_bits;  // ceil(w*h/32): bit i of word j = pixel 32j+i is an edge
   // edge points in raster order (Canny.cpp:24-33)
   std::vector<int16_t> ex, ey, edx, edy;
   std::vector<int32_t> before, after;        // Vote.cpp:79-87
