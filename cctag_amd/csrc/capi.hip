@@ -13,6 +13,7 @@
 #include "host/cabi.hpp"
 #include "host/pipeline.hpp"
 #include "ident_device.hpp"
+#include "staging.hpp"
 
 using namespace cctagx;
 
@@ -41,12 +42,6 @@ const T* ptr(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
 
 }  // namespace
 
-// Pinned host copy of the candidate stage's inputs for one level.
-struct StagedLevel {
-  PinnedBuffer ebits, ex, ey, egx, egy, before, after, vindex, vlist, flow, ismax, seeds;
-  LevelEdges view;
-};
-
 struct cctagx_handle {
   cctagx_params params{};
   HostParams host{};
@@ -58,8 +53,8 @@ struct cctagx_handle {
   bool have_front_end = false;
   std::vector<HostLevel> levels;
   std::vector<bool> level_loaded;
-  std::vector<StagedLevel> staged;
-  PinnedBuffer counts;
+  LevelStager stager;
+  std::vector<LevelEdges> views;  // candidate-stage inputs of the last detection
   GpuIdentBackend ident;
   std::vector<Marker> markers;
   std::unique_ptr<TaskPool> pool;  // host candidate-stage workers
@@ -88,55 +83,10 @@ static int validate(const cctagx_params* p) {
   return CCTAGX_OK;
 }
 
-// Copies what the candidate stage reads of every level into pinned memory:
-// counts first (one synchronization), then the exact-size arrays.
+// Copies what the candidate stage reads of every level into pinned memory.
 static int stage_levels(cctagx_handle* h) {
-  const PlaneSet& ps = h->fe.planes();
-  const FrontEndBuffers& b = h->fe.buffers();
-  const int np = (int)ps.planes.size();
-  hipStream_t s = h->stream;
-  if (h->counts.ensure(sizeof(PlaneCounts) * np)) return CCTAGX_ERR_HIP;
-  PlaneCounts* cnt = h->counts.as<PlaneCounts>();
-  CCTAGX_HIP(hipMemcpyAsync(cnt, b.counts, sizeof(PlaneCounts) * np, hipMemcpyDeviceToHost, s));
-  CCTAGX_HIP(hipStreamSynchronize(s));
-  h->staged.resize(np);
-  for (int l = 0; l < np; ++l) {
-    const PlaneDesc& d = ps.planes[l];
-    StagedLevel& S = h->staged[l];
-    const size_t N = (size_t)d.w * d.h, n = cnt[l].n_edges, nv = cnt[l].n_voters,
-                 nsd = cnt[l].n_seeds;
-    auto get = [&](PinnedBuffer& dst, const void* src, size_t bytes) -> int {
-      if (dst.ensure(std::max<size_t>(bytes, 64))) return CCTAGX_ERR_HIP;
-      if (bytes) CCTAGX_HIP(hipMemcpyAsync(dst.as<void>(), src, bytes, hipMemcpyDeviceToHost, s));
-      return 0;
-    };
-    int rc = 0;
-    rc |= get(S.ebits, b.ebits + (d.pix >> 5), 4 * ((N + 31) / 32));
-    rc |= get(S.ex, b.ex + d.pix, 2 * n);
-    rc |= get(S.ey, b.ey + d.pix, 2 * n);
-    rc |= get(S.egx, b.egx + d.pix, 2 * n);
-    rc |= get(S.egy, b.egy + d.pix, 2 * n);
-    rc |= get(S.before, b.before + d.pix, 4 * n);
-    rc |= get(S.after, b.after + d.pix, 4 * n);
-    rc |= get(S.vindex, b.vindex + d.pix + l, 4 * (n + 1));
-    rc |= get(S.vlist, b.vlist + d.pix, 4 * nv);
-    rc |= get(S.flow, b.flow + d.pix, 4 * n);
-    rc |= get(S.ismax, b.ismax + d.pix, 4 * n);
-    rc |= get(S.seeds, b.seeds + d.pix, 4 * nsd);
-    if (rc) return CCTAGX_ERR_HIP;
-    LevelEdges& v = S.view;
-    v.w = d.w; v.h = d.h; v.shape_w = d.shape_w; v.shape_h = d.shape_h;
-    v.bits = S.ebits.as<uint32_t>();
-    v.n = (int)n;
-    v.x = S.ex.as<int16_t>(); v.y = S.ey.as<int16_t>();
-    v.gx = S.egx.as<int16_t>(); v.gy = S.egy.as<int16_t>();
-    v.before = S.before.as<int32_t>(); v.after = S.after.as<int32_t>();
-    v.vindex = S.vindex.as<int32_t>(); v.vlist = S.vlist.as<int32_t>();
-    v.flow = S.flow.as<float>(); v.ismax = S.ismax.as<int32_t>();
-    v.n_seeds = (int)nsd; v.seeds = S.seeds.as<int32_t>();
-  }
-  CCTAGX_HIP(hipStreamSynchronize(s));
-  return CCTAGX_OK;
+  const int rc = h->stager.stage(h->fe.planes(), h->fe.buffers(), h->stream, h->views);
+  return rc == 0 ? CCTAGX_OK : (rc == -1 ? CCTAGX_ERR_ARG : CCTAGX_ERR_HIP);
 }
 
 static FrontEndParams front_end_params(const cctagx_params& p) {
@@ -267,11 +217,9 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   h->stage_ms = std::chrono::duration<double, std::milli>(t0 - ts).count();
   h->ident.clear_stats();
   try {
-    std::vector<LevelEdges> views;
-    for (const StagedLevel& S : h->staged) views.push_back(S.view);
     const PlaneDesc& d0 = h->fe.planes().planes[0];
     h->ident.set_image(h->fe.buffers().img + d0.pix, d0.w, d0.h);
-    detect_markers(views, h->host, h->bank, &h->ident, h->markers, h->pool.get());
+    detect_markers(h->views, h->host, h->bank, &h->ident, h->markers, h->pool.get());
   } catch (const std::exception& e) {
     set_error(e.what());
     return CCTAGX_ERR_HIP;

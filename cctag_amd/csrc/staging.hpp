@@ -1,0 +1,72 @@
+// Device-to-host staging of what the host candidate stage reads of every
+// plane: the counterpart of TagPipe::convertToHost (cuda/frame_export.cu:
+// 23-165).  One counts read-back, then one gather kernel packs every
+// plane's arrays (exact sizes) into a single device buffer and one copy
+// brings it to pinned host memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device.hpp"
+#include "frontend_launch.hpp"
+#include "host/candidate.hpp"
+
+namespace cctagx {
+
+// Device buffer that only grows.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  int ensure(size_t bytes) {
+    if (bytes <= cap_) return 0;
+    release();
+    CCTAGX_HIP(hipMalloc(&p_, bytes));
+    cap_ = bytes;
+    return 0;
+  }
+  void release() {
+    if (p_) hipFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p_); }
+
+ private:
+  void* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
+// One contiguous piece to gather: `bytes` from `src` to packed offset `dst`
+// (a multiple of 16).
+struct CopySeg {
+  const void* src;
+  uint64_t dst;
+  uint64_t bytes;
+};
+
+int launch_pack(const CopySeg* segs, int n_segs, uint8_t* dst, hipStream_t s);
+
+class LevelStager {
+ public:
+  // Stages planes [first, first + count) of the plane set (all when count
+  // < 0) and fills one view per plane.  Synchronizes the stream twice.
+  int stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t s,
+            std::vector<LevelEdges>& views, int first = 0, int count = -1);
+  // Counts of the planes last staged (indexed like the views).
+  const std::vector<PlaneCounts>& counts() const { return counts_; }
+  size_t last_bytes() const { return last_bytes_; }
+
+ private:
+  PinnedBuffer h_counts_, h_segs_, h_packed_;
+  DeviceBuffer d_segs_, d_packed_;
+  std::vector<PlaneCounts> counts_;
+  size_t last_bytes_ = 0;
+};
+
+}  // namespace cctagx
