@@ -67,170 +67,205 @@ static float compute_median(const std::vector<float>& vec) {
   return (float)((out[s - 1] + out[s - 2]) / 2.0);
 }
 
-void identify_markers(std::vector<Marker>& markers, const HostParams& p,
-                      const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
-  const size_t n = markers.size();
-  std::vector<int> detected(n, -1);
+namespace {
+// Working state of one frame between the backend calls.
+struct FrameWork {
+  std::vector<Marker>* markers;
+  int image;
+  std::vector<int> detected;
+  std::vector<size_t> job_marker;   // select job -> marker
+  std::vector<std::vector<Pt2>> stops;
+  std::vector<size_t> rmarker;      // refine job -> marker
+};
+}  // namespace
+
+void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
+                     const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
+  std::vector<FrameWork> fw;
+  for (IdentFrame& f : frames) fw.push_back(FrameWork{f.markers, f.image, {}, {}, {}, {}});
   // --- identify_step_1 (Identification.cpp:1172-1301) ----------------------
   std::vector<CutJob> jobs;
-  std::vector<size_t> job_marker;
-  for (size_t i = 0; i < n; ++i) {
-    const Marker& m = markers[i];
-    const Ellipse& e = m.rescaled_outer_ellipse;
-    CutJob job;
-    sorted_outer_points(e, m.rescaled_outer_ellipse_points, job.outer_points,
-                        (size_t)p.n_samples_outer_ellipse);
-    if (job.outer_points.size() < 5) {
-      detected[i] = status::too_few_outer_points;
-      continue;
+  for (FrameWork& F : fw) {
+    std::vector<Marker>& markers = *F.markers;
+    const size_t n = markers.size();
+    F.detected.assign(n, -1);
+    F.stops.assign(n, {});
+    for (size_t i = 0; i < n; ++i) {
+      const Marker& m = markers[i];
+      const Ellipse& e = m.rescaled_outer_ellipse;
+      CutJob job;
+      job.image = F.image;
+      sorted_outer_points(e, m.rescaled_outer_ellipse_points, job.outer_points,
+                          (size_t)p.n_samples_outer_ellipse);
+      if (job.outer_points.size() < 5) {
+        F.detected[i] = status::too_few_outer_points;
+        continue;
+      }
+      job.cx = e.cx();
+      job.cy = e.cy();
+      job.ca = std::cos((double)e.angle());
+      job.sa = std::sin((double)e.angle());
+      job.ea = e.a();
+      job.eb = e.b();
+      job.scale = m.scale;
+      jobs.push_back(std::move(job));
+      F.job_marker.push_back(i);
     }
-    job.cx = e.cx();
-    job.cy = e.cy();
-    job.ca = std::cos((double)e.angle());
-    job.sa = std::sin((double)e.angle());
-    job.ea = e.a();
-    job.eb = e.b();
-    job.scale = m.scale;
-    jobs.push_back(std::move(job));
-    job_marker.push_back(i);
   }
   std::vector<CutSelection> sel;
   backend.select_cuts(jobs, p, sel);
-  std::vector<std::vector<Pt2>> stops(n);
-  for (size_t j = 0; j < jobs.size(); ++j) {
-    const size_t i = job_marker[j];
-    if (sel[j].n_collected == 0) detected[i] = status::no_collected_cuts;
-    else if (sel[j].stops.empty()) detected[i] = status::no_selected_cuts;
-    else {
-      detected[i] = status::id_reliable;
-      stops[i] = sel[j].stops;
+  size_t j = 0;
+  for (FrameWork& F : fw)
+    for (size_t i : F.job_marker) {
+      const CutSelection& c = sel[j++];
+      if (c.n_collected == 0) F.detected[i] = status::no_collected_cuts;
+      else if (c.stops.empty()) F.detected[i] = status::no_selected_cuts;
+      else {
+        F.detected[i] = status::id_reliable;
+        F.stops[i] = c.stops;
+      }
     }
-  }
   // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
   std::vector<RefineJob> rjobs;
-  std::vector<size_t> rmarker;
-  bool have_mean_ab = false;
-  float mean_ab = 0.f;
   const float sqrt2 = std::sqrt(2.f);
   const IdentConstants kc = ident_constants(p, bank);
-  for (size_t i = 0; i < n; ++i) {
-    if (detected[i] != status::id_reliable) continue;
-    const Marker& m = markers[i];
-    const Ellipse& e = m.rescaled_outer_ellipse;
-    RefineJob r;
-    e.canonic_form(r.canonic, r.t_can, r.t_inv_can);
-    // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
-    // first ellipse it sees.
-    if (!have_mean_ab) {
-      mean_ab = (e.a() + e.b()) / 2.f;
-      have_mean_ab = true;
+  for (FrameWork& F : fw) {
+    std::vector<Marker>& markers = *F.markers;
+    bool have_mean_ab = false;
+    float mean_ab = 0.f;
+    for (size_t i = 0; i < markers.size(); ++i) {
+      if (F.detected[i] != status::id_reliable) continue;
+      const Marker& m = markers[i];
+      const Ellipse& e = m.rescaled_outer_ellipse;
+      RefineJob r;
+      r.image = F.image;
+      e.canonic_form(r.canonic, r.t_can, r.t_inv_can);
+      // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
+      // first ellipse it sees.
+      if (!have_mean_ab) {
+        mean_ab = (e.a() + e.b()) / 2.f;
+        have_mean_ab = true;
+      }
+      r.cond = Mat3::zero();
+      r.cond.m[0][0] = sqrt2 / mean_ab;
+      r.cond.m[0][2] = -sqrt2 * e.cx() / mean_ab;
+      r.cond.m[1][1] = sqrt2 / mean_ab;
+      r.cond.m[1][2] = -sqrt2 * e.cy() / mean_ab;
+      r.cond.m[2][2] = 1.f;
+      r.cond_inv = inverse3(r.cond);
+      float gs = 0.f;
+      try {
+        const Ellipse te(mul(mul(transpose(r.cond_inv), e.matrix()), r.cond_inv));
+        gs = std::max(te.a(), te.b());
+      } catch (...) {
+        F.detected[i] = status::opti_has_diverged;
+        continue;
+      }
+      r.grid_scale = gs;
+      r.max_semi_axis = std::max(e.a(), e.b());
+      r.neighbour0 = p.imaged_center_neighbour_size;
+      float ns = p.imaged_center_neighbour_size;
+      int it = 0;
+      const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
+      // (bounded: a non-finite semi-axis would never end the reference's loop)
+      while (ns * r.max_semi_axis > 0.02 && it < 64) {
+        ++it;
+        ns /= div;
+      }
+      r.n_iter = it;
+      r.center0 = m.center_img;
+      r.stops = F.stops[i];
+      r.begin_sig = kc.start_sig;
+      r.end_sig = 1.f;
+      rjobs.push_back(std::move(r));
+      F.rmarker.push_back(i);
     }
-    r.cond = Mat3::zero();
-    r.cond.m[0][0] = sqrt2 / mean_ab;
-    r.cond.m[0][2] = -sqrt2 * e.cx() / mean_ab;
-    r.cond.m[1][1] = sqrt2 / mean_ab;
-    r.cond.m[1][2] = -sqrt2 * e.cy() / mean_ab;
-    r.cond.m[2][2] = 1.f;
-    r.cond_inv = inverse3(r.cond);
-    float gs = 0.f;
-    try {
-      const Ellipse te(mul(mul(transpose(r.cond_inv), e.matrix()), r.cond_inv));
-      gs = std::max(te.a(), te.b());
-    } catch (...) {
-      detected[i] = status::opti_has_diverged;
-      continue;
-    }
-    r.grid_scale = gs;
-    r.max_semi_axis = std::max(e.a(), e.b());
-    r.neighbour0 = p.imaged_center_neighbour_size;
-    float ns = p.imaged_center_neighbour_size;
-    int it = 0;
-    const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
-    // (bounded: a non-finite semi-axis would never end the reference's loop)
-    while (ns * r.max_semi_axis > 0.02 && it < 64) {
-      ++it;
-      ns /= div;
-    }
-    r.n_iter = it;
-    r.center0 = m.center_img;
-    r.stops = stops[i];
-    r.begin_sig = kc.start_sig;
-    r.end_sig = 1.f;
-    rjobs.push_back(std::move(r));
-    rmarker.push_back(i);
   }
   std::vector<RefineResult> res;
   backend.refine(rjobs, p, res);
   const int nb = (int)bank.size();
-  for (size_t j = 0; j < rjobs.size(); ++j) {
-    const size_t i = rmarker[j];
-    Marker& m = markers[i];
-    const RefineResult& r = res[j];
-    if (!r.ok) {
-      m.quality = 1.f / FLT_MAX;
-      m.center_img = Pt2(0.f, 0.f);
-      detected[i] = status::opti_has_diverged;
-      continue;
-    }
-    m.center_img = r.center;
-    m.homography = r.H;
-    const size_t ncut = rjobs[j].stops.size();
-    const size_t ns = r.signals.size() / ncut;
-    std::vector<size_t> correct;
-    for (size_t k = 0; k < ncut; ++k)
-      if (!r.oob[k]) correct.push_back(k);
-    std::vector<float> bar(ns), along(correct.size());
-    for (size_t s = 0; s < ns; ++s) {
-      for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
-      bar[s] = compute_median(along);
-    }
-    const float vMin = *std::min_element(bar.begin(), bar.end());
-    const float vMax = *std::max_element(bar.begin(), bar.end());
-    const float magnitude = vMax - vMin;
-    const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
-    m.quality = 1.f / residual;
-    if (residual > 2.7f) {
-      detected[i] = status::opti_has_diverged;
-      continue;
-    }
-    // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
-    std::vector<std::list<float>> vScore(nb);
-    for (size_t k = 0; k < ncut; ++k) {
-      if (r.oob[k]) continue;
-      std::map<float, int> sortedId;
-      for (int idc = 0; idc < nb; ++idc) {
-        const float v = std::exp(-r.id_dist[k * nb + idc]);
-        sortedId[v] = idc;
+  j = 0;
+  for (FrameWork& F : fw) {
+    std::vector<Marker>& markers = *F.markers;
+    std::vector<int>& detected = F.detected;
+    for (size_t i : F.rmarker) {
+      const RefineJob& rj = rjobs[j];
+      const RefineResult& r = res[j++];
+      Marker& m = markers[i];
+      if (!r.ok) {
+        m.quality = 1.f / FLT_MAX;
+        m.center_img = Pt2(0.f, 0.f);
+        detected[i] = status::opti_has_diverged;
+        continue;
       }
-      const auto best = sortedId.rbegin();
-      vScore[best->second].push_back(best->first);
-    }
-    size_t maxSize = 0;
-    int iMax = 0;
-    for (int c = 0; c < nb; ++c)
-      if (vScore[c].size() > maxSize) {
-        iMax = c;
-        maxSize = vScore[c].size();
+      m.center_img = r.center;
+      m.homography = r.H;
+      const size_t ncut = rj.stops.size();
+      const size_t ns = r.signals.size() / ncut;
+      std::vector<size_t> correct;
+      for (size_t k = 0; k < ncut; ++k)
+        if (!r.oob[k]) correct.push_back(k);
+      std::vector<float> bar(ns), along(correct.size());
+      for (size_t s = 0; s < ns; ++s) {
+        for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
+        bar[s] = compute_median(along);
       }
-    float score = 0;
-    for (float v : vScore[iMax]) score += v;
-    score /= vScore[iMax].size();
-    m.id = iMax;
-    m.radius_ratios = bank[iMax];
-    try {
-      const Mat3 inv = inverse3(m.homography);
-      for (float rr : m.radius_ratios) {
-        const Ellipse circle(0.f, 0.f, 1.f / rr, 1.f / rr, 0.f);
-        const Ellipse ring(mul(mul(transpose(inv), circle.matrix()), inv));
-        (void)ring;
+      const float vMin = *std::min_element(bar.begin(), bar.end());
+      const float vMax = *std::max_element(bar.begin(), bar.end());
+      const float magnitude = vMax - vMin;
+      const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
+      m.quality = 1.f / residual;
+      if (residual > 2.7f) {
+        detected[i] = status::opti_has_diverged;
+        continue;
       }
-    } catch (...) {
-      detected[i] = status::degenerate;
-      continue;
+      // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
+      std::vector<std::list<float>> vScore(nb);
+      for (size_t k = 0; k < ncut; ++k) {
+        if (r.oob[k]) continue;
+        std::map<float, int> sortedId;
+        for (int idc = 0; idc < nb; ++idc) {
+          const float v = std::exp(-r.id_dist[k * nb + idc]);
+          sortedId[v] = idc;
+        }
+        const auto best = sortedId.rbegin();
+        vScore[best->second].push_back(best->first);
+      }
+      size_t maxSize = 0;
+      int iMax = 0;
+      for (int c = 0; c < nb; ++c)
+        if (vScore[c].size() > maxSize) {
+          iMax = c;
+          maxSize = vScore[c].size();
+        }
+      float score = 0;
+      for (float v : vScore[iMax]) score += v;
+      score /= vScore[iMax].size();
+      m.id = iMax;
+      m.radius_ratios = bank[iMax];
+      try {
+        const Mat3 inv = inverse3(m.homography);
+        for (float rr : m.radius_ratios) {
+          const Ellipse circle(0.f, 0.f, 1.f / rr, 1.f / rr, 0.f);
+          const Ellipse ring(mul(mul(transpose(inv), circle.matrix()), inv));
+          (void)ring;
+        }
+      } catch (...) {
+        detected[i] = status::degenerate;
+        continue;
+      }
+      detected[i] = (score > p.min_ident_proba) ? status::id_reliable : status::id_not_reliable;
     }
-    detected[i] = (score > p.min_ident_proba) ? status::id_reliable : status::id_not_reliable;
+    for (size_t i = 0; i < markers.size(); ++i) markers[i].status = detected[i];
   }
-  for (size_t i = 0; i < n; ++i) markers[i].status = detected[i];
+}
+
+void identify_markers(std::vector<Marker>& markers, const HostParams& p,
+                      const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
+  std::vector<IdentFrame> f(1);
+  f[0].markers = &markers;
+  f[0].image = 0;
+  identify_frames(f, p, bank, backend);
 }
 
 }  // namespace cctagx

@@ -34,6 +34,7 @@ constexpr int degenerate = -5;
 
 // Input of identify_step_1 for one marker after the host's getSortedOuterPoints.
 struct CutJob {
+  int image = 0;                     // index of the frame's image in the backend
   float cx = 0, cy = 0;              // outer ellipse centre (cut start)
   std::vector<Pt2> outer_points;     // sorted, subsampled outer points (cut stops, with gradients)
   double ca = 1, sa = 0;             // cos / sin of the ellipse angle (double C functions)
@@ -50,6 +51,7 @@ struct CutSelection {
 // Per-marker constants of the grid search, precomputed on the host (the
 // only transcendental functions of the search live here).
 struct RefineJob {
+  int image = 0;                     // index of the frame's image in the backend
   Mat3 canonic, t_can, t_inv_can;    // Ellipse::getCanonicForm
   Mat3 cond, cond_inv;               // conditionerFromEllipse and its inverse
   float grid_scale = 0;              // max(a, b) of the conditioned ellipse
@@ -96,10 +98,20 @@ struct IdentConstants {
 IdentConstants ident_constants(const HostParams& p, const std::vector<std::vector<float>>& bank);
 
 // Runs identification on every marker (Detection.cpp:873-965) and sets
-// status, id, centre, homography, quality.  `conditioner_mean_ab` is the
-// (a+b)/2 that conditionerFromEllipse caches on its first call; it is taken
-// from the first marker refined in this call.
+// status, id, centre, homography, quality.  The (a+b)/2 that
+// conditionerFromEllipse caches on its first call is taken from the first
+// marker refined in this call.
 void identify_markers(std::vector<Marker>& markers, const HostParams& p,
                       const std::vector<std::vector<float>>& bank, IdentBackend& backend);
+
+// The same for several frames at once: one select_cuts and one refine call
+// of the backend for all their markers; each frame's markers come out as
+// identify_markers would leave them (the conditioner cache is per frame).
+struct IdentFrame {
+  std::vector<Marker>* markers = nullptr;
+  int image = 0;                     // backend image index of the frame
+};
+void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
+                     const std::vector<std::vector<float>>& bank, IdentBackend& backend);
 
 }  // namespace cctagx

@@ -61,6 +61,11 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   cap_points_ = np;
 }
 
+const DeviceImage& GpuIdentBackend::image(int i) const {
+  if (i < 0 || i >= (int)images_.size()) throw std::runtime_error("identification job names no image");
+  return images_[i];
+}
+
 void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
                                   std::vector<CutSelection>& out) {
   out.assign(jobs.size(), CutSelection());
@@ -78,14 +83,16 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
     const CutJob& j = jobs[i];
     if ((int)j.outer_points.size() > kMaxOuterPoints)
       throw std::runtime_error("more outer points than the identification kernels accept");
-    hj[i] = SelectJobDev{j.cx, j.cy, j.ca, j.sa, j.ea, j.eb, j.scale, (int)j.outer_points.size(), off};
+    const DeviceImage& im = image(j.image);
+    hj[i] = SelectJobDev{im.p, im.w, im.h, j.cx, j.cy, j.ca, j.sa, j.ea, j.eb, j.scale,
+                         (int)j.outer_points.size(), off};
     for (const Pt2& q : j.outer_points) hp[off++] = make_float4(q.x, q.y, q.dx, q.dy);
   }
   hipStream_t s = stream_;
   hip_check(hipMemcpyAsync(d_sel_jobs_, hj, sizeof(SelectJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
   hip_check(hipMemcpyAsync(d_pts_, hp, sizeof(float4) * npts, hipMemcpyHostToDevice, s), "H2D");
   hipEventRecord(ev_[0], s);
-  if (launch_select_cuts(d_sel_jobs_, n, d_pts_, img_, w_, h_, k_.start_sig, k_.n_samples,
+  if (launch_select_cuts(d_sel_jobs_, n, d_pts_, k_.start_sig, k_.n_samples,
                          k_.n_refine_samples, p.num_cuts_in_ident_step, d_sel_out_, s) != 0)
     throw std::runtime_error("k_select_cuts launch failed");
   hipEventRecord(ev_[1], s);
@@ -118,6 +125,10 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   for (int i = 0; i < n; ++i) {
     const RefineJob& r = jobs[i];
     RefineJobDev& d = hj[i];
+    const DeviceImage& im = image(r.image);
+    d.img = im.p;
+    d.iw = im.w;
+    d.ih = im.h;
     d.Q11 = r.canonic.m[0][0];
     d.Q22 = r.canonic.m[1][1];
     d.Q33 = r.canonic.m[2][2];
@@ -138,7 +149,7 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   hipStream_t s = stream_;
   hip_check(hipMemcpyAsync(d_ref_jobs_, hj, sizeof(RefineJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
   hipEventRecord(ev_[2], s);
-  if (launch_refine(d_ref_jobs_, n, img_, w_, h_, k_.n_samples, p.imaged_center_n_grid_sample,
+  if (launch_refine(d_ref_jobs_, n, k_.n_samples, p.imaged_center_n_grid_sample,
                     d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_scratch_, d_ref_out_, s) != 0)
     throw std::runtime_error("k_refine launch failed");
   hipEventRecord(ev_[3], s);

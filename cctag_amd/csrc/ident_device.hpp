@@ -16,6 +16,8 @@ constexpr int kMaxBankRows = 128;  // rows of the 4-crown bank
 constexpr int kMaxRefineSamples = 32;  // numSamplesOuterEdgePointsRefinement bound
 
 struct SelectJobDev {
+  const uint8_t* img;  // the marker's frame (level 0), row-major, width iw
+  int iw, ih;
   float cx, cy;
   double ca, sa;
   float ea, eb, scale;
@@ -28,6 +30,8 @@ struct SelectOutDev {
 };
 
 struct RefineJobDev {
+  const uint8_t* img;  // the marker's frame (level 0), row-major, width iw
+  int iw, ih;
   float Q11, Q22, Q33;
   float t_can[9], t_inv_can[9], cond[9], cond_inv[9];
   float grid_scale, neighbour0, begin_sig, end_sig;
@@ -46,23 +50,26 @@ struct RefineOutDev {
   float id_dist[kMaxCuts * kMaxBankRows];
 };
 
-int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, const uint8_t* img,
-                       int w, int h, float start_sig, int n_samples, int n_ref, int select_max,
-                       SelectOutDev* out, hipStream_t s);
-int launch_refine(const RefineJobDev* jobs, int n_jobs, const uint8_t* img, int w, int h,
-                  int n_samples, int grid_n, const float* inv_ratio, int bank_rows, int bank_cols,
-                  float* scratch, RefineOutDev* out, hipStream_t s);
+int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
+                       int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s);
+int launch_refine(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
+                  const float* inv_ratio, int bank_rows, int bank_cols, float* scratch,
+                  RefineOutDev* out, hipStream_t s);
+
+// A device image the identification jobs sample.
+struct DeviceImage {
+  const uint8_t* p;
+  int w, h;
+};
 
 // IdentBackend on the GPU: jobs packed into pinned memory, one launch per
 // step for all markers of the frame.  Errors are thrown as std::runtime_error.
 class GpuIdentBackend : public IdentBackend {
  public:
   int init(const std::vector<std::vector<float>>& bank, const HostParams& p, hipStream_t s);
-  void set_image(const uint8_t* dev_img, int w, int h) {
-    img_ = dev_img;
-    w_ = w;
-    h_ = h;
-  }
+  // Images that CutJob::image / RefineJob::image index.
+  void set_images(const std::vector<DeviceImage>& images) { images_ = images; }
+  void set_image(const uint8_t* dev_img, int w, int h) { images_.assign(1, DeviceImage{dev_img, w, h}); }
   void select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
                    std::vector<CutSelection>& out) override;
   void refine(const std::vector<RefineJob>& jobs, const HostParams& p,
@@ -84,8 +91,8 @@ class GpuIdentBackend : public IdentBackend {
  private:
   void ensure(int n_markers, int n_points);
   hipStream_t stream_ = nullptr;
-  const uint8_t* img_ = nullptr;
-  int w_ = 0, h_ = 0;
+  std::vector<DeviceImage> images_;
+  const DeviceImage& image(int i) const;
   IdentConstants k_{};
   int cap_markers_ = 0, cap_points_ = 0;
   DeviceArena dev_;

@@ -146,7 +146,7 @@ __constant__ float kRefineKernels[3][9] = {
     {-0.0366f, -0.1113f, -0.1801f, -0.1594f, .0f, 0.1594f, 0.1801f, 0.1113f, 0.0366f}};
 
 __global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restrict__ jobs,
-                                                      const float4* __restrict__ pts, Img im,
+                                                      const float4* __restrict__ pts,
                                                       float start_sig, int n_samples, int n_ref,
                                                       int select_max, SelectOutDev* __restrict__ out) {
   __shared__ float var[kMaxOuterPoints];
@@ -155,6 +155,7 @@ __global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restr
   __shared__ float4 stop[kMaxOuterPoints];
   __shared__ float var_max;
   const SelectJobDev J = jobs[blockIdx.x];
+  const Img im{J.img, J.iw, J.ih};
   const int n = J.n_outer;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const float4 op = pts[J.pt_offset + t];
@@ -271,7 +272,7 @@ constexpr int kRefineThreads = 1024;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
 
 __global__ __launch_bounds__(kRefineThreads) void k_refine(
-    const RefineJobDev* __restrict__ jobs, Img im, int n_samples, int grid_n,
+    const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
     const float* __restrict__ inv_ratio, int bank_rows, int bank_cols,
     float* __restrict__ scratch, RefineOutDev* __restrict__ out) {
   constexpr int G2MAX = kMaxGrid * kMaxGrid;
@@ -288,6 +289,7 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
 
   const RefineJobDev& J = jobs[blockIdx.x];
+  const Img im{J.img, J.iw, J.ih};
   const int nc = J.n_cuts, G2 = grid_n * grid_n;
   float* scr = scratch + (size_t)blockIdx.x * G2MAX * kMaxCuts * kMaxSamples;
   const int tid = threadIdx.x;
@@ -530,21 +532,19 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
 
 }  // namespace
 
-int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, const uint8_t* img,
-                       int w, int h, float start_sig, int n_samples, int n_ref, int select_max,
-                       SelectOutDev* out, hipStream_t s) {
+int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
+                       int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_select_cuts<<<n_jobs, 256, 0, s>>>(jobs, pts, Img{img, w, h}, start_sig, n_samples, n_ref,
-                                       select_max, out);
+  k_select_cuts<<<n_jobs, 256, 0, s>>>(jobs, pts, start_sig, n_samples, n_ref, select_max, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_refine(const RefineJobDev* jobs, int n_jobs, const uint8_t* img, int w, int h,
-                  int n_samples, int grid_n, const float* inv_ratio, int bank_rows, int bank_cols,
-                  float* scratch, RefineOutDev* out, hipStream_t s) {
+int launch_refine(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
+                  const float* inv_ratio, int bank_rows, int bank_cols, float* scratch,
+                  RefineOutDev* out, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_refine<<<n_jobs, kRefineThreads, 0, s>>>(jobs, Img{img, w, h}, n_samples, grid_n, inv_ratio,
-                                             bank_rows, bank_cols, scratch, out);
+  k_refine<<<n_jobs, kRefineThreads, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
+                                             bank_cols, scratch, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
