@@ -106,6 +106,19 @@ class DetectStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(ctypes.Structure):
+    """cctagx_batch_stats."""
+    _fields_ = [
+        ("front_end_us", c_f32), ("stage_us", c_f32), ("candidates_us", c_f32),
+        ("ident_us", c_f32), ("select_us", c_f32), ("refine_us", c_f32), ("tune_us", c_f32),
+        ("filter_variant", c_i32), ("edge_blocks", c_i32), ("graph", c_i32),
+        ("n_markers", c_i32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _arr(p, n, dtype):
     if n == 0 or not p:
         return np.zeros(0, dtype=dtype)
@@ -175,6 +188,12 @@ def load() -> ctypes.CDLL:
     lib.cctagx_get_detect_stats.argtypes = [vp, P(DetectStats)]
     lib.cctagx_set_host_threads.argtypes = [vp, ctypes.c_int]
     lib.cctagx_get_host_threads.argtypes = [vp]
+    lib.cctagx_batch_create.argtypes = [P(Params), ctypes.c_int, P(c_i32), P(c_i32), P(vp)]
+    lib.cctagx_batch_destroy.argtypes = [vp]
+    lib.cctagx_batch_load.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    lib.cctagx_batch_set_partition.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.cctagx_batch_detect.argtypes = [vp, P(Marker), ctypes.c_int, P(c_i32), P(ctypes.c_int)]
+    lib.cctagx_batch_get_stats.argtypes = [vp, P(BatchStats)]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int

@@ -14,6 +14,7 @@
 #include "host/pipeline.hpp"
 #include "ident_device.hpp"
 #include "staging.hpp"
+#include "capi_common.hpp"
 
 using namespace cctagx;
 
@@ -63,46 +64,10 @@ struct cctagx_handle {
   bool have_frame = false;
 };
 
-static int validate(const cctagx_params* p) {
-  if (!p) return CCTAGX_ERR_ARG;
-  if (p->n_crowns != 3 && p->n_crowns != 4) return CCTAGX_ERR_UNSUPPORTED;
-  if (p->angle_voting != 0.0f) return CCTAGX_ERR_ARG;  // Vote.cpp:95-98 throws
-  if (p->number_of_processed_multires_layers < 1 || p->number_of_processed_multires_layers > 4)
-    return CCTAGX_ERR_UNSUPPORTED;
-  if (p->dist_search < 0 || p->min_votes_to_select_candidate < 1) return CCTAGX_ERR_ARG;
-  // fixed bounds of the identification kernels and of the candidate stage's
-  // 64-bit per-candidate masks
-  if (p->imaged_center_n_grid_sample < 3 || p->imaged_center_n_grid_sample > kMaxGrid ||
-      p->num_cuts_in_ident_step < 1 || p->num_cuts_in_ident_step > kMaxCuts ||
-      p->sample_cut_length < 31 || p->sample_cut_length > kMaxSamples ||
-      p->num_samples_outer_edge_points_refinement < 9 ||
-      p->num_samples_outer_edge_points_refinement > kMaxRefineSamples ||
-      p->n_samples_outer_ellipse < 5 || p->n_samples_outer_ellipse > kMaxOuterPoints ||
-      p->maximum_nb_candidates_loop_two < 0 || p->maximum_nb_candidates_loop_two > 64)
-    return CCTAGX_ERR_UNSUPPORTED;
-  return CCTAGX_OK;
-}
-
 // Copies what the candidate stage reads of every level into pinned memory.
 static int stage_levels(cctagx_handle* h) {
   const int rc = h->stager.stage(h->fe.planes(), h->fe.buffers(), h->stream, h->views);
   return rc == 0 ? CCTAGX_OK : (rc == -1 ? CCTAGX_ERR_ARG : CCTAGX_ERR_HIP);
-}
-
-static FrontEndParams front_end_params(const cctagx_params& p) {
-  // Level.cpp:206-207 passes thr * 256; cvRecode.cpp:67,107-108 orders and floors them.
-  float lo = p.canny_thr_low * 256, hi = p.canny_thr_high * 256;
-  if (lo > hi) std::swap(lo, hi);
-  FrontEndParams f{};
-  f.low = (int)std::floor(lo);
-  f.high = (int)std::floor(hi);
-  f.dist_search = p.dist_search;
-  f.thr_grad = p.thr_gradient_mag_in_vote;
-  f.ratio_voting = p.ratio_voting;
-  f.angle_voting = p.angle_voting;
-  f.n_crowns = p.n_crowns;
-  f.min_votes = p.min_votes_to_select_candidate;
-  return f;
 }
 
 extern "C" {
@@ -152,7 +117,7 @@ void cctagx_params_default(cctagx_params* p, int n_crowns) {
 int cctagx_create(const cctagx_params* params, int width, int height, cctagx_handle** out) {
   if (!out || width <= 0 || height <= 0) return CCTAGX_ERR_ARG;
   *out = nullptr;
-  if (int rc = validate(params)) return rc;
+  if (int rc = validate_params(params)) return rc;
   if (cctagx_device_count() <= 0) {
     set_error("no HIP device visible");
     return CCTAGX_ERR_NO_DEVICE;

@@ -23,7 +23,7 @@ struct PlaneDesc {
   int32_t parent;           // plane of the next-finer level, -1 for level 0
   int32_t level;            // pyramid level
   int32_t frame;            // frame index within a batch
-  int32_t tile_f;           // first block of the filter grid (64x32 tiles)
+  int32_t tile_f;           // first block of the default filter grid (64x32 tiles)
   int32_t tile_s;           // first block of the stencil grids (64x16 tiles)
   int32_t tile_c;           // first block of the compaction grid (4096-px chunks)
   int32_t tile_p;           // first block of the pyramid grid (level-0 planes only)
@@ -39,6 +39,10 @@ struct PlaneCounts {
 
 // Tile geometry shared by host launch code and kernels.
 constexpr int kFilterTileW = 64, kFilterTileH = 32;
+// Filter tile heights the launch can choose from (64 wide, 256 threads: a
+// thread owns one column of TH/4 rows); variant 1 is the default 64x32.
+constexpr int kFilterVariants = 3;
+constexpr int kFilterTileHs[kFilterVariants] = {16, 32, 64};
 constexpr int kStencilTileW = 64, kStencilTileH = 16;
 constexpr int kChunk = 4096;        // compaction chunk, pixels
 constexpr int kPyrTile = 64;        // pyramid tile at level 0 (-> 32, 16, 8)

@@ -43,6 +43,10 @@ bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_leve
       d.frame = (int)f;
       d.tile_f = tf;
       tf += cdiv(lw, kFilterTileW) * cdiv(lh, kFilterTileH);
+      for (int v = 0; v < kFilterVariants; ++v) {
+        out.filter_starts[v].push_back(out.n_tiles_fv[v]);
+        out.n_tiles_fv[v] += cdiv(lw, kFilterTileW) * cdiv(lh, kFilterTileHs[v]);
+      }
       d.tile_s = ts;
       ts += cdiv(lw, kStencilTileW) * cdiv(lh, kStencilTileH);
       d.tile_c = tc;
@@ -95,6 +99,8 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t o_seeds = arena_.reserve(4 * N);
   const size_t o_counts = arena_.reserve(sizeof(PlaneCounts) * NP);
   const size_t o_planes = arena_.reserve(sizeof(PlaneDesc) * NP);
+  size_t o_fstart[kFilterVariants];
+  for (int v = 0; v < kFilterVariants; ++v) o_fstart[v] = arena_.reserve(sizeof(int32_t) * NP);
   if (int rc = arena_.allocate()) return rc;
   FrontEndBuffers& b = launch_.buf;
   b.img = arena_.at<uint8_t>(o_img);
@@ -131,6 +137,12 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   launch_.n_levels = ps.n_levels;
   launch_.n_tiles_p = ps.n_tiles_p;
   launch_.n_tiles_f = ps.n_tiles_f;
+  for (int v = 0; v < kFilterVariants; ++v) {
+    int32_t* fs = arena_.at<int32_t>(o_fstart[v]);
+    CCTAGX_HIP(hipMemcpy(fs, ps.filter_starts[v].data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice));
+    launch_.dev_filter_starts[v] = fs;
+    launch_.n_tiles_fv[v] = ps.n_tiles_fv[v];
+  }
   launch_.n_tiles_s = ps.n_tiles_s;
   launch_.n_tiles_c = ps.n_tiles_c;
   // Enough blocks per plane to cover a level-0 plane's edges with one

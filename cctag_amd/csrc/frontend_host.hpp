@@ -17,6 +17,12 @@ class FrontEndDevice {
   const PlaneSet& planes() const { return ps_; }
   const FrontEndBuffers& buffers() const { return launch_.buf; }
   const FrontEndLaunch& launch() const { return launch_; }
+  // Work partition of the launch sequence (filter tile variant, blocks per
+  // plane of the edge-indexed kernels); every choice gives the same output.
+  void set_partition(int filter_variant, int edge_blocks) {
+    launch_.filter_variant = filter_variant;
+    launch_.edge_blocks = edge_blocks;
+  }
 
  private:
   PlaneSet ps_;

@@ -30,6 +30,17 @@ __device__ __forceinline__ int plane_of(const PlaneDesc* P, int np, int blk) {
   return lo;
 }
 constexpr int F_TILE_F = 8, F_TILE_S = 9, F_TILE_C = 10, F_TILE_P = 11;
+
+// The same lookup over a plain array of per-plane first blocks.
+__device__ __forceinline__ int plane_of_starts(const int32_t* a, int np, int blk) {
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 static_assert(sizeof(PlaneDesc) == 12 * sizeof(int32_t), "PlaneDesc layout");
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -131,15 +142,18 @@ __host__ __device__ constexpr float kdx(int i, int j) {
   return j == 4 ? 0.0f : (j < 4 ? -tap(i <= 4 ? i : 8 - i, j) : tap(i <= 4 ? i : 8 - i, 8 - j));
 }
 
+// RPT rows per thread: a 64 x 4*RPT tile per 256-thread block.
+template <int RPT>
 __global__ __launch_bounds__(256) void k_filter(const PlaneDesc* __restrict__ P, int np,
+                                                 const int32_t* __restrict__ starts,
                                                  const uint8_t* __restrict__ img,
                                                  int16_t* __restrict__ dxo,
                                                  int16_t* __restrict__ dyo) {
-  constexpr int TW = kFilterTileW, TH = kFilterTileH, LW = TW + 8, LH = TH + 8;
+  constexpr int TW = kFilterTileW, TH = 4 * RPT, LW = TW + 8, LH = TH + 8;
   __shared__ float s[LH][LW + 1];
-  const int p = plane_of<F_TILE_F>(P, np, blockIdx.x);
+  const int p = plane_of_starts(starts, np, blockIdx.x);
   const PlaneDesc d = P[p];
-  const int t = blockIdx.x - d.tile_f;
+  const int t = blockIdx.x - starts[p];
   const int tx_n = (d.w + TW - 1) / TW;
   const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
   const uint8_t* src = img + d.pix;
@@ -149,17 +163,17 @@ __global__ __launch_bounds__(256) void k_filter(const PlaneDesc* __restrict__ P,
     s[ly][lx] = (float)src[gy * d.w + gx];
   }
   __syncthreads();
-  const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) * 8;
-  float ax[8], ay[8];
+  const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) * RPT;
+  float ax[RPT], ay[RPT];
 #pragma unroll
-  for (int o = 0; o < 8; ++o) { ax[o] = 0.0f; ay[o] = 0.0f; }
+  for (int o = 0; o < RPT; ++o) { ax[o] = 0.0f; ay[o] = 0.0f; }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
+  for (int r = 0; r < RPT + 8; ++r) {
     float v[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) v[j] = s[ty + r][tx + j];
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
+    for (int o = 0; o < RPT; ++o) {
       const int i = r - o;
       if (i >= 0 && i < 9) {
 #pragma unroll
@@ -173,7 +187,7 @@ __global__ __launch_bounds__(256) void k_filter(const PlaneDesc* __restrict__ P,
   const int x = x0 + tx;
   if (x < d.w) {
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
+    for (int o = 0; o < RPT; ++o) {
       const int y = y0 + ty + o;
       if (y < d.h) {
         const int g = d.pix + y * d.w + x;
@@ -889,7 +903,14 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
     k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img);
     mark("pyramid");
   }
-  k_filter<<<L.n_tiles_f, 256, 0, s>>>(P, np, b.img, b.dx, b.dy);
+  {
+    const int v = L.filter_variant;
+    const int32_t* fs = L.dev_filter_starts[v];
+    const int nt = L.n_tiles_fv[v];
+    if (kFilterTileHs[v] == 16) k_filter<4><<<nt, 256, 0, s>>>(P, np, fs, b.img, b.dx, b.dy);
+    else if (kFilterTileHs[v] == 64) k_filter<16><<<nt, 256, 0, s>>>(P, np, fs, b.img, b.dx, b.dy);
+    else k_filter<8><<<nt, 256, 0, s>>>(P, np, fs, b.img, b.dx, b.dy);
+  }
   mark("filter");
   k_nms<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.dx, b.dy, L.prm.low, L.prm.high, b.cls, b.lab,
                                     b.root_strong);

@@ -38,7 +38,10 @@ struct FrontEndLaunch {
   int n_planes = 0;
   int n_levels = 0;
   int n_tiles_p = 0;   // pyramid blocks (level-0 64x64 tiles)
-  int n_tiles_f = 0;   // filter blocks
+  int n_tiles_f = 0;   // filter blocks of the default variant
+  int filter_variant = 1;                     // index into kFilterTileHs
+  const int32_t* dev_filter_starts[kFilterVariants] = {};  // per variant: first block of each plane
+  int n_tiles_fv[kFilterVariants] = {};       // per variant: filter blocks
   int n_tiles_s = 0;   // stencil blocks
   int n_tiles_c = 0;   // compaction chunks
   int edge_blocks = 0; // blocks per plane of the edge-indexed kernels
@@ -54,6 +57,8 @@ struct PlaneSet {
   int n_levels = 0;
   int total_pix = 0;   // pixel / edge slots, plane starts aligned to 256
   int n_tiles_p = 0, n_tiles_f = 0, n_tiles_s = 0, n_tiles_c = 0;
+  std::vector<int32_t> filter_starts[kFilterVariants];  // per variant, one entry per plane
+  int n_tiles_fv[kFilterVariants] = {};
 };
 
 // Returns false when a level is not an exact half of its parent (the fast

@@ -18,4 +18,11 @@ void detect_markers(const std::vector<LevelEdges>& levels, const HostParams& p,
                     const std::vector<std::vector<float>>& bank, IdentBackend* ident,
                     std::vector<Marker>& out, TaskPool* pool = nullptr);
 
+// The stages of detect_markers, for callers that batch identification over
+// frames: candidates of one frame (all levels, level-0 re-fit), then the
+// double de-duplication and the sort by id (Detection.cpp:966-999).
+void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p,
+                      TaskPool* pool, std::vector<Marker>& markers);
+void finish_markers(const std::vector<Marker>& markers, std::vector<Marker>& out);
+
 }  // namespace cctagx

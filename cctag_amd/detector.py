@@ -119,3 +119,68 @@ class Detector:
         _lib.check(self._lib.cctagx_time_front_end(self._h, iters, ctypes.byref(us)),
                    "cctagx_time_front_end")
         return us.value
+
+
+class BatchDetector:
+    """Batched detection of many frames of uneven sizes (cctagx_batch_*):
+    one captured graph launch for the front end of every frame, the
+    candidate stage parallel over frames, identification batched."""
+
+    def __init__(self, sizes, params: _lib.Params | None = None, n_crowns: int = 3):
+        self._lib = _lib.load()
+        self.params = params if params is not None else _lib.default_params(n_crowns)
+        self.sizes = [(int(w), int(h)) for w, h in sizes]
+        n = len(self.sizes)
+        ws = (_lib.c_i32 * n)(*[w for w, _ in self.sizes])
+        hs = (_lib.c_i32 * n)(*[h for _, h in self.sizes])
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.cctagx_batch_create(ctypes.byref(self.params), n, ws, hs,
+                                                 ctypes.byref(h)), "cctagx_batch_create")
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.cctagx_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, frame: int, img: np.ndarray) -> None:
+        img = np.ascontiguousarray(img)
+        w, h = self.sizes[frame]
+        if img.dtype != np.uint8 or img.shape != (h, w):
+            raise ValueError("frame %d must be a %dx%d uint8 image" % (frame, w, h))
+        _lib.check(self._lib.cctagx_batch_load(self._h, frame, img.ctypes.data, img.strides[0]),
+                   "cctagx_batch_load")
+
+    def set_partition(self, filter_variant: int = -1, edge_blocks: int = 0) -> None:
+        _lib.check(self._lib.cctagx_batch_set_partition(self._h, filter_variant, edge_blocks),
+                   "cctagx_batch_set_partition")
+
+    def detect(self, max_markers: int = 1 << 14) -> list:
+        """Markers of every frame: a list (one per frame) of marker dicts."""
+        n = len(self.sizes)
+        buf = (_lib.Marker * max_markers)()
+        first = (_lib.c_i32 * (n + 1))()
+        tot = ctypes.c_int()
+        _lib.check(self._lib.cctagx_batch_detect(self._h, buf, max_markers, first,
+                                                 ctypes.byref(tot)), "cctagx_batch_detect")
+        if tot.value > max_markers:
+            raise RuntimeError("more than %d markers in the batch" % max_markers)
+        return [[buf[i].as_dict() for i in range(first[f], first[f + 1])] for f in range(n)]
+
+    def stats(self) -> dict:
+        st = _lib.BatchStats()
+        _lib.check(self._lib.cctagx_batch_get_stats(self._h, ctypes.byref(st)),
+                   "cctagx_batch_get_stats")
+        return st.as_dict()

@@ -172,6 +172,43 @@ typedef struct cctagx_detect_stats {
 } cctagx_detect_stats;
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);
 
+/* ---- batched detection ---------------------------------------------------
+ * Many frames of uneven sizes detected together: the front end of every
+ * frame and level runs as one captured hipGraph launch over a single plane
+ * set, the candidate stage of the frames runs in parallel on the host, and
+ * identification runs one launch per step for the markers of all frames.
+ * Each frame's markers are exactly what cctagx_detect returns for it.  No
+ * counterpart in the reference, whose pipes take one frame per call
+ * (Detection.cpp:732-760). */
+typedef struct cctagx_batch cctagx_batch;
+int cctagx_batch_create(const cctagx_params* params, int n_frames, const int32_t* widths,
+                        const int32_t* heights, cctagx_batch** out);
+int cctagx_batch_destroy(cctagx_batch* b);
+/* Copies frame `frame` (row-major 8-bit, stride in bytes) to the device. */
+int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, size_t stride);
+/* Work partition of the batched front end: filter tile variant (0, 1, 2 =
+ * 64x16, 64x32, 64x64 tiles) and blocks per plane of the edge-indexed
+ * kernels.  -1 / 0 leave the choice to the autotuner, which times the
+ * candidates on the first detection's frames before capturing the graph. */
+int cctagx_batch_set_partition(cctagx_batch* b, int filter_variant, int edge_blocks);
+/* Detects every loaded frame.  Writes up to max_out markers, frame by frame
+ * (each frame sorted by id); frame_first[f] .. frame_first[f+1] (n_frames + 1
+ * entries) index frame f's markers, and *n_out is the total, which may
+ * exceed max_out. */
+int cctagx_batch_detect(cctagx_batch* b, cctagx_marker* out, int max_out, int32_t* frame_first,
+                        int* n_out);
+typedef struct cctagx_batch_stats {
+  float front_end_us;   /* device time of the graph launch */
+  float stage_us;       /* host wall time of staging every plane */
+  float candidates_us;  /* host wall time of all frames' candidate stages */
+  float ident_us;       /* host wall time of the batched identification */
+  float select_us, refine_us; /* device time of the two identification kernels */
+  float tune_us;        /* autotuning time spent in this call (0 after the first) */
+  int32_t filter_variant, edge_blocks, graph;
+  int32_t n_markers;
+} cctagx_batch_stats;
+int cctagx_batch_get_stats(cctagx_batch* b, cctagx_batch_stats* out);
+
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
  * events on the library stream; names are static strings. */

@@ -37,3 +37,17 @@ def edge_case(name: str) -> np.ndarray:
         r = np.hypot(x - 127.3, y - 121.8)
         return np.where((r.astype(int) // 9) % 2 == 0, 40, 210).astype(np.uint8)
     raise KeyError(name)
+
+
+@functools.lru_cache(maxsize=None)
+def small_batch():
+    """Five frames of four sizes for the batched path: a 640x480 frame of the
+    uneven-size batch (markers + distractors), the 1280x720 four-marker
+    frame, the small two-marker frame, concentric rings and a blank frame."""
+    cfg = synth.FrameConfig(640, 480, (6, 6), (15, 80), 60.0, (0.5, 2.0), 6.0,
+                            occlusion_frac=0.2, distractors=(20, 20))
+    out = [synth.make_frame(cfg, 4242)[0], config_frame(1)[0], small_marker_frame()[0],
+           edge_case("rings"), edge_case("blank")]
+    for img in out:
+        img.setflags(write=False)
+    return out

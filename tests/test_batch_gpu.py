@@ -1,0 +1,48 @@
+"""Batched detection through the C ABI: every frame of an uneven-size batch
+gets exactly the markers the plain path finds for it alone, under every
+work partition the autotuner can choose, and graph replays repeat."""
+import pytest
+
+from cctag_amd.detector import BatchDetector, Detector
+from tests import frames
+
+
+def plain(imgs):
+    out = []
+    for img in imgs:
+        h, w = img.shape
+        with Detector(w, h) as det:
+            out.append(det.detect(img))
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_matches_plain_per_frame():
+    imgs = frames.small_batch()
+    ref = plain(imgs)
+    print("batch of", [im.shape for im in imgs], flush=True)
+    with BatchDetector([(im.shape[1], im.shape[0]) for im in imgs]) as b:
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+        st = b.stats()
+        print("stats", st, flush=True)
+        assert st["graph"] == 1 and st["tune_us"] > 0
+        again = b.detect()
+        assert b.stats()["tune_us"] == 0
+    assert got == ref
+    assert again == got
+    assert sum(len(r) for r in ref) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fv,eb", [(0, 64), (1, 512), (2, 1024), (2, 128)])
+def test_batch_partitions_agree(fv, eb):
+    imgs = frames.small_batch()
+    ref = plain(imgs)
+    with BatchDetector([(im.shape[1], im.shape[0]) for im in imgs]) as b:
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        b.set_partition(fv, eb)
+        print("partition", fv, eb, flush=True)
+        assert b.detect() == ref
