@@ -1,6 +1,7 @@
 // GPU identification backend (host side): packs the per-marker jobs the host
 // prepared, launches the identification kernels once per step for all
 // markers and unpacks the results.
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -39,8 +40,9 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   const size_t o_so = dev_.reserve(sizeof(SelectOutDev) * nm);
   const size_t o_rj = dev_.reserve(sizeof(RefineJobDev) * nm);
   const size_t o_ro = dev_.reserve(sizeof(RefineOutDev) * nm);
-  const size_t o_scr =
-      dev_.reserve(sizeof(float) * (size_t)nm * kMaxGrid * kMaxGrid * kMaxCuts * kMaxSamples);
+  const size_t o_st = dev_.reserve(sizeof(RefineStateDev) * nm);
+  const size_t o_gr = dev_.reserve(sizeof(RefineGridDev) * (size_t)nm * kMaxGrid * kMaxGrid);
+  const size_t o_gm = dev_.reserve(sizeof(double) * (size_t)nm * kMaxGrid * kMaxGrid * kMaxPairs);
   const size_t o_inv = dev_.reserve(sizeof(float) * kMaxBank * kMaxRatios);
   if (dev_.allocate() != 0) throw std::runtime_error(std::string("hipMalloc: ") + last_error());
   d_sel_jobs_ = dev_.at<SelectJobDev>(o_sj);
@@ -48,7 +50,9 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   d_sel_out_ = dev_.at<SelectOutDev>(o_so);
   d_ref_jobs_ = dev_.at<RefineJobDev>(o_rj);
   d_ref_out_ = dev_.at<RefineOutDev>(o_ro);
-  d_scratch_ = dev_.at<float>(o_scr);
+  d_work_.state = dev_.at<RefineStateDev>(o_st);
+  d_work_.grid = dev_.at<RefineGridDev>(o_gr);
+  d_work_.gram = dev_.at<double>(o_gm);
   d_inv_ratio_ = dev_.at<float>(o_inv);
   hip_check(hipMemcpy(d_inv_ratio_, &k_.inv_ratio[0][0], sizeof(float) * kMaxBank * kMaxRatios,
                       hipMemcpyHostToDevice),
@@ -149,8 +153,10 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   hipStream_t s = stream_;
   hip_check(hipMemcpyAsync(d_ref_jobs_, hj, sizeof(RefineJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
   hipEventRecord(ev_[2], s);
-  if (launch_refine(d_ref_jobs_, n, k_.n_samples, p.imaged_center_n_grid_sample,
-                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_scratch_, d_ref_out_, s) != 0)
+  int max_iter = 0;
+  for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
+  if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
+                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_, d_ref_out_, s) != 0)
     throw std::runtime_error("k_refine launch failed");
   hipEventRecord(ev_[3], s);
   RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();

@@ -52,8 +52,29 @@ struct RefineOutDev {
 
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
                        int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s);
-int launch_refine(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
-                  const float* inv_ratio, int bank_rows, int bank_cols, float* scratch,
+// Grid-search state of one marker between iteration launches.
+struct RefineStateDev {
+  float cx, cy, neighbour, min_res;
+  int ok, best_g;
+  unsigned done;       // blocks of the current iteration that finished
+  int pad;
+  uint8_t sticky[kMaxCuts];
+};
+// One grid point of the current iteration.
+struct RefineGridDev {
+  float H[9], Hi[9];
+  float gx, gy;
+  uint8_t oob[kMaxCuts];
+};
+constexpr int kMaxPairs = kMaxCuts * (kMaxCuts + 1) / 2;
+struct RefineWork {
+  RefineStateDev* state;  // n_jobs
+  RefineGridDev* grid;    // n_jobs x kMaxGrid^2
+  double* gram;           // n_jobs x kMaxGrid^2 x kMaxPairs
+};
+
+int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,
+                  const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, hipStream_t s);
 
 // A device image the identification jobs sample.
@@ -102,7 +123,7 @@ class GpuIdentBackend : public IdentBackend {
   SelectOutDev* d_sel_out_ = nullptr;
   RefineJobDev* d_ref_jobs_ = nullptr;
   RefineOutDev* d_ref_out_ = nullptr;
-  float* d_scratch_ = nullptr;
+  RefineWork d_work_{};
   float* d_inv_ratio_ = nullptr;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   float select_us_ = 0, refine_us_ = 0;

@@ -4,15 +4,15 @@
 //                (Identification.cpp:396-709, 1172-1301); one block per
 //                marker, one thread per outer point.  Bit-exact with the
 //                reference's serial float arithmetic.
-// k_refine       refineConicFamilyGlob (Identification.cpp:813-1158): every
-//                grid-search iteration of a marker inside one persistent
-//                1024-thread block, then the final signals and the per-cut
-//                distances to every bank profile of orazioDistanceRobust
-//                (:50-160).  The pairwise sum of squared differences uses
-//                sum_{i<j} (s_i - s_j)^2 = m sum s^2 - (sum s)^2 in double
-//                instead of the reference's sequential float sum; everything
-//                else (grid, homographies, sampling, the sticky out-of-bounds
-//                flag of a cut) follows the reference's float operations.
+// k_refine_*     refineConicFamilyGlob (Identification.cpp:813-1158): one
+//                launch per grid-search iteration, a block per (grid point,
+//                marker), then the final signals and the per-cut distances
+//                to every bank profile of orazioDistanceRobust (:50-160).
+//                The pairwise sum of squared differences is taken from the
+//                cuts' Gram matrix in double instead of the reference's
+//                sequential float sum; everything else (grid, homographies,
+//                sampling, the sticky out-of-bounds flag of a cut) follows
+//                the reference's float operations.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -145,44 +145,60 @@ __constant__ float kRefineKernels[3][9] = {
     {-0.0044f, -0.0540f, -0.2376f, -0.3450f, .0f, 0.3450f, 0.2376f, 0.0540f, 0.0044f},
     {-0.0366f, -0.1113f, -0.1801f, -0.1594f, .0f, 0.1594f, 0.1801f, 0.1113f, 0.0366f}};
 
-__global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restrict__ jobs,
-                                                      const float4* __restrict__ pts,
-                                                      float start_sig, int n_samples, int n_ref,
-                                                      int select_max, SelectOutDev* __restrict__ out) {
-  __shared__ float var[kMaxOuterPoints];
-  __shared__ uint8_t valid[kMaxOuterPoints];
-  __shared__ uint8_t cand[kMaxOuterPoints];
+// One thread per outer point (n <= kMaxOuterPoints = blockDim.x).  The
+// counts, the maximum variance and the candidates' ranks come from block
+// reductions and a ballot scan; the uniform-stride pick of
+// selectCutCheapUniform (Identification.cpp:546-601) is one lane per
+// selected cut.
+constexpr int kSelectBlock = 256;
+static_assert(kSelectBlock >= kMaxOuterPoints, "one lane per outer point");
+
+__global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev* __restrict__ jobs,
+                                                              const float4* __restrict__ pts,
+                                                              float start_sig, int n_samples, int n_ref,
+                                                              int select_max, SelectOutDev* __restrict__ out) {
+  constexpr int NW = kSelectBlock / 64;
   __shared__ float4 stop[kMaxOuterPoints];
-  __shared__ float var_max;
+  __shared__ int idx_of_rank[kMaxOuterPoints];
+  __shared__ int w_cnt[NW], w_cand[NW];
+  __shared__ float w_max[NW];
   const SelectJobDev J = jobs[blockIdx.x];
   const Img im{J.img, J.iw, J.ih};
   const int n = J.n_outer;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const float4 op = pts[J.pt_offset + t];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  bool valid = false;
+  float var = 0.f;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (t < n) {
+    s = pts[J.pt_offset + t];
     MeanVar mv;
-    const bool ok = cut_walk(im, J.cx, J.cy, op.x, op.y, start_sig, 1.f, n_samples,
-                             [&](int, float v) { mv.add(v); });
-    valid[t] = ok;
-    var[t] = mv.var;
-    stop[t] = op;
+    valid = cut_walk(im, J.cx, J.cy, s.x, s.y, start_sig, 1.f, n_samples,
+                     [&](int, float v) { mv.add(v); });
+    var = mv.var;
+  }
+  // number of collected cuts and their largest variance
+  {
+    int c = valid ? 1 : 0;
+    float mx = valid ? var : -FLT_MAX;
+    for (int o = 32; o > 0; o >>= 1) {
+      c += __shfl_xor(c, o, 64);
+      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) {
+      w_cnt[wid] = c;
+      w_max[wid] = mx;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = -FLT_MAX;
-    int nc = 0;
-    for (int t = 0; t < n; ++t)
-      if (valid[t]) {
-        m = nc == 0 ? var[t] : fmaxf(m, var[t]);
-        ++nc;
-      }
-    var_max = m;
-    out[blockIdx.x].n_collected = nc;
+  int nc = 0;
+  float var_max = -FLT_MAX;
+  for (int w = 0; w < NW; ++w) {
+    nc += w_cnt[w];
+    var_max = fmaxf(var_max, w_max[w]);
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    cand[t] = 0;
-    if (!valid[t]) continue;
-    float4 s = stop[t];
+  if (t == 0) out[blockIdx.x].n_collected = nc;
+  bool cand = false;
+  if (valid) {
     // pointOnEllipse (EllipseFromPoints.cpp:29-49), double like the C cos/sin
     {
       const float x = s.x - J.cx, y = s.y - J.cy;
@@ -202,7 +218,7 @@ __global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restr
     const float hx = half * gdx, hy = half * gdy;
     const float psx = s.x - hx, psy = s.y - hy;
     const float pex = s.x + half * gdx, pey = s.y + half * gdy;
-    float sig[32];
+    float sig[kMaxRefineSamples];
     const bool ok = cut_walk(im, psx, psy, pex, pey, 0.f, 1.f, n_ref, [&](int i, float v) { sig[i] = v; });
     if (ok) {
       float key[3], loc[3];
@@ -231,261 +247,334 @@ __global__ __launch_bounds__(256) void k_select_cuts(const SelectJobDev* __restr
       const float step = len / ((float)n_ref - 1.f);
       s.x = psx + step * loc[b] * gdx;
       s.y = psy + step * loc[b] * gdy;
-      cand[t] = var[t] / var_max > 0.5f;
+      cand = var / var_max > 0.5f;
     }
-    stop[t] = s;
   }
+  // rank of each candidate among the collected cuts, in outer-point order
+  const uint64_t ball = __ballot(cand);
+  const int in_wave = __popcll(ball & ((1ull << lane) - 1ull));
+  if (lane == 0) w_cand[wid] = __popcll(ball);
+  if (t < n) stop[t] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int nc = 0, ni = 0;
-    for (int t = 0; t < n; ++t) {
-      if (!valid[t]) continue;
-      ++nc;
-      if (cand[t]) ++ni;
-    }
-    const int select = min(select_max, nc);
-    const float step = fmaxf(1.f, (float)ni / (float)select);
+  int before = 0, ni = 0;
+  for (int w = 0; w < NW; ++w) {
+    if (w < wid) before += w_cand[w];
+    ni += w_cand[w];
+  }
+  if (cand) idx_of_rank[before + in_wave] = t;
+  __syncthreads();
+  // uniform stride over the candidates: pick k takes rank (int)(k * step)
+  const int select = min(select_max, nc);
+  const float step = fmaxf(1.f, (float)ni / (float)select);
+  if (t < select) {
+    const int at = (int)((float)t * step);
+    if (at < ni) out[blockIdx.x].stops[t] = stop[idx_of_rank[at]];
+  }
+  if (t == 0) {
     int nsel = 0;
-    for (int k = 0;; ++k) {
-      const int at = (int)((float)k * step);
-      if (at < ni && nsel < select) {
-        // at-th candidate in collected order
-        int seen = 0;
-        for (int t = 0; t < n; ++t)
-          if (valid[t] && cand[t]) {
-            if (seen == at) {
-              out[blockIdx.x].stops[nsel++] = stop[t];
-              break;
-            }
-            ++seen;
-          }
-      } else {
-        break;
-      }
-    }
+    while (nsel < select && (int)((float)nsel * step) < ni) ++nsel;
     out[blockIdx.x].n_selected = nsel;
   }
 }
 
 // ---------------------------------------------------------------------------
-constexpr int kRefineThreads = 1024;
+// Centre refinement, one launch per grid-search iteration.  Block (g, m)
+// evaluates grid point g of marker m: its homography (thread 0), the signals
+// of every selected cut (LDS), their out-of-bounds flags and the Gram matrix
+// C_kl = sum_i s_ki s_li in double.  The last of the G2 blocks of a marker to
+// finish (device-scope counter) applies the sticky out-of-bounds flags in
+// grid-point order and takes the residual of grid point g over its valid
+// cuts as sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl,
+// then moves the marker to the first minimum, as the reference's loop does.
+constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
+constexpr int G2MAX = kMaxGrid * kMaxGrid;
+constexpr int kCostParts = 10;    // partial sums per grid point in the selection
 
-__global__ __launch_bounds__(kRefineThreads) void k_refine(
-    const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
-    const float* __restrict__ inv_ratio, int bank_rows, int bank_cols,
-    float* __restrict__ scratch, RefineOutDev* __restrict__ out) {
-  constexpr int G2MAX = kMaxGrid * kMaxGrid;
-  __shared__ float sH[G2MAX][9], sHi[G2MAX][9];
-  __shared__ float gx[G2MAX], gy[G2MAX];
-  __shared__ int oob_here[G2MAX][kMaxCuts];
+// getNearbyPoints point g + computeHomographyFromEllipseAndImagedCenter
+// (Identification.cpp:739-798, 1071-1109) and the inverse.
+__device__ void grid_homography(const RefineJobDev& J, float cur_x, float cur_y, float neighbour,
+                                int g, int grid_n, float* Hf, float* Hi, float& qx, float& qy) {
+  const int i = g / grid_n, j = g - i * grid_n;
+  const float ns = neighbour * J.grid_scale;
+  float cx = cur_x, cy = cur_y;
+  condition(cx, cy, J.cond);
+  const float halfWidth = ns / 2.f;
+  const float stepSize = ns / (float)(grid_n - 1);
+  qx = cx - halfWidth + (float)i * stepSize;
+  qy = cy - halfWidth + (float)j * stepSize;
+  condition(qx, qy, J.cond_inv);
+  float xc, yc;
+  apply_h(xc, yc, J.t_can, qx, qy);
+  const float Q11 = J.Q11, Q22 = J.Q22, Q33 = J.Q33;
+  float H[9];
+  H[0] = Q33;              H[1] = Q22 * xc * yc;        H[2] = -Q33 * xc;
+  H[3] = 0.f;              H[4] = -Q11 * xc * xc - Q33; H[5] = -Q33 * yc;
+  H[6] = -Q11 * xc;        H[7] = Q22 * yc;             H[8] = -Q33;
+  const float d0 = sqrt_rn((Q22 * Q33 / Q11 * (Q11 * xc * xc + Q22 * yc * yc + Q33)));
+  const float d1 = Q33;
+  const float d2 = sqrt_rn(-Q22 * (Q11 * xc * xc + Q33));
+  for (int r = 0; r < 3; ++r) {
+    H[r * 3 + 0] *= d0;
+    H[r * 3 + 1] *= d1;
+    H[r * 3 + 2] *= d2;
+  }
+  mul3(J.t_inv_can, H, Hf);
+  inverse3(Hf, Hi);
+}
+
+// extractSignalUsingHomography (Identification.cpp:216-280) for samples
+// [c*kRefineChunk, ...) of cut k: the stop back-projected by Hi, samples
+// from begin*stop to end*stop in the marker plane (running sums replayed up
+// to the chunk), mapped by H, half bilinear value; 0 and the flag when out.
+__device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, float2 stop,
+                             float begin_sig, float end_sig, int n_samples, int c, float* out) {
+  float bx, by;
+  apply_h(bx, by, Hi, stop.x, stop.y);
+  float xs, ys, xe, ye;
+  if (begin_sig != 0.f) {
+    xs = bx * begin_sig;
+    ys = by * begin_sig;
+  } else {
+    xs = 0;
+    ys = 0;
+  }
+  if (end_sig != 1.f) {
+    xe = bx * end_sig;
+    ye = by * end_sig;
+  } else {
+    xe = bx;
+    ye = by;
+  }
+  const float stepX = (xe - xs) / ((float)n_samples - 1.f), stepY = (ye - ys) / ((float)n_samples - 1.f);
+  float x = xs, y = ys;
+  const int i0 = c * kRefineChunk;
+  for (int i = 0; i < i0; ++i) {
+    x += stepX;
+    y += stepY;
+  }
+  const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
+  bool oob = false;
+#pragma unroll
+  for (int t = 0; t < kRefineChunk; ++t) {
+    float xr, yr;
+    apply_h(xr, yr, H, x, y);
+    const bool live = i0 + t < n_samples;
+    const bool in = live && xr >= 0.f && xr < xmax && yr >= 0.f && yr < ymax;
+    oob |= live && !in;
+    const float v = in ? im.bilinear(xr, yr) : 0.f;
+    if (live) out[i0 + t] = v;
+    x += stepX;
+    y += stepY;
+  }
+  return oob;
+}
+
+__device__ __forceinline__ int pair_offset(int k, int nc) { return k * nc - k * (k - 1) / 2; }
+
+__global__ __launch_bounds__(256) void k_refine_init(const RefineJobDev* __restrict__ jobs,
+                                                     int n_jobs, RefineStateDev* state) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_jobs) return;
+  RefineStateDev& S = state[m];
+  S.cx = jobs[m].c0x;
+  S.cy = jobs[m].c0y;
+  S.neighbour = jobs[m].neighbour0;
+  S.min_res = FLT_MAX;
+  S.ok = 1;
+  S.best_g = 0;
+  S.done = 0u;
+  for (int k = 0; k < kMaxCuts; ++k) S.sticky[k] = 0;
+}
+
+__global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
+    const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n,
+    RefineStateDev* state, RefineGridDev* grid, double* gram) {
+  __shared__ float sH[9], sHi[9];
+  __shared__ float sig[kMaxCuts][kMaxSamples + 1];
+  __shared__ int oobk[kMaxCuts];
+  __shared__ int is_last;
+  __shared__ uint8_t oobl[G2MAX][kMaxCuts];
   __shared__ uint8_t valid[G2MAX][kMaxCuts];
-  __shared__ uint8_t sticky[kMaxCuts];
+  __shared__ uint8_t sticky_new[kMaxCuts];
   __shared__ int mvalid[G2MAX];
+  __shared__ double part[G2MAX][kCostParts];
   __shared__ float res[G2MAX];
-  __shared__ double contrib[G2MAX][kMaxSamples];
-  __shared__ float cur_x, cur_y, neighbour;
-  __shared__ int best_g, ok;
-  __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
 
-  const RefineJobDev& J = jobs[blockIdx.x];
+  const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x;
+  const int G2 = grid_n * grid_n;
+  const RefineJobDev& J = jobs[m];
+  RefineStateDev& S = state[m];
+  if (it >= J.n_iter || !S.ok) return;
+  const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
-  const int nc = J.n_cuts, G2 = grid_n * grid_n;
-  float* scr = scratch + (size_t)blockIdx.x * G2MAX * kMaxCuts * kMaxSamples;
-  const int tid = threadIdx.x;
-  if (tid < kMaxCuts) sticky[tid] = 0;
+  RefineGridDev& GD = grid[(size_t)m * G2MAX + g];
   if (tid == 0) {
-    cur_x = J.c0x;
-    cur_y = J.c0y;
-    neighbour = J.neighbour0;
-    ok = 1;
-    best_g = 0;
+    float Hf[9], Hi[9], qx, qy;
+    grid_homography(J, S.cx, S.cy, S.neighbour, g, grid_n, Hf, Hi, qx, qy);
+    for (int r = 0; r < 9; ++r) {
+      sH[r] = Hf[r];
+      sHi[r] = Hi[r];
+      GD.H[r] = Hf[r];
+      GD.Hi[r] = Hi[r];
+    }
+    GD.gx = qx;
+    GD.gy = qy;
+  }
+  if (tid < kMaxCuts) oobk[tid] = 0;
+  __syncthreads();
+  const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
+  for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
+    const int k = w / nchunk, c = w - k * nchunk;
+    if (sample_chunk(im, sH, sHi, J.stops[k], J.begin_sig, J.end_sig, n_samples, c, sig[k]))
+      atomicOr(&oobk[k], 1);
   }
   __syncthreads();
-  float min_res = FLT_MAX;
-  for (int it = 0; it < J.n_iter; ++it) {
-    // getNearbyPoints + computeHomographyFromEllipseAndImagedCenter per grid point
-    if (tid < G2) {
-      const int i = tid / grid_n, j = tid - i * grid_n;
-      const float ns = neighbour * J.grid_scale;
-      float cx = cur_x, cy = cur_y;
-      condition(cx, cy, J.cond);
-      const float halfWidth = ns / 2.f;
-      const float stepSize = ns / (float)(grid_n - 1);
-      float qx = cx - halfWidth + (float)i * stepSize, qy = cy - halfWidth + (float)j * stepSize;
-      condition(qx, qy, J.cond_inv);
-      gx[tid] = qx;
-      gy[tid] = qy;
-      float xc, yc;
-      apply_h(xc, yc, J.t_can, qx, qy);
-      const float Q11 = J.Q11, Q22 = J.Q22, Q33 = J.Q33;
-      float H[9];
-      H[0] = Q33;              H[1] = Q22 * xc * yc;        H[2] = -Q33 * xc;
-      H[3] = 0.f;              H[4] = -Q11 * xc * xc - Q33; H[5] = -Q33 * yc;
-      H[6] = -Q11 * xc;        H[7] = Q22 * yc;             H[8] = -Q33;
-      const float d0 = sqrt_rn((Q22 * Q33 / Q11 * (Q11 * xc * xc + Q22 * yc * yc + Q33)));
-      const float d1 = Q33;
-      const float d2 = sqrt_rn(-Q22 * (Q11 * xc * xc + Q33));
-      for (int r = 0; r < 3; ++r) {
-        H[r * 3 + 0] *= d0;
-        H[r * 3 + 1] *= d1;
-        H[r * 3 + 2] *= d2;
-      }
-      float Hf[9], Hi[9];
-      mul3(J.t_inv_can, H, Hf);
-      inverse3(Hf, Hi);
-      for (int r = 0; r < 9; ++r) {
-        sH[tid][r] = Hf[r];
-        sHi[tid][r] = Hi[r];
-      }
+  double* C = gram + ((size_t)m * G2MAX + g) * kMaxPairs;
+  const int npairs = nc * (nc + 1) / 2;
+  for (int q = tid; q < npairs; q += kRefineBlock) {
+    int k = 0, r = q;
+    while (r >= nc - k) {
+      r -= nc - k;
+      ++k;
     }
-    __syncthreads();
-    // extractSignalUsingHomography for every (grid point, cut), split into
-    // chunks of kRefineChunk samples so that all lanes gather at once; a
-    // chunk replays the running sums of the positions before it.
-    for (int q = tid; q < G2 * nc; q += kRefineThreads) oob_here[q / nc][q % nc] = 0;
-    __syncthreads();
-    const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
-    for (int pk = tid; pk < G2 * nc * nchunk; pk += kRefineThreads) {
-      const int c = pk % nchunk, gk = pk / nchunk;
-      const int g = gk / nc, k = gk - g * nc;
-      float bx, by;
-      apply_h(bx, by, sHi[g], J.stops[k].x, J.stops[k].y);
-      float xs, ys, xe, ye;
-      if (J.begin_sig != 0.f) {
-        xs = bx * J.begin_sig;
-        ys = by * J.begin_sig;
-      } else {
-        xs = 0;
-        ys = 0;
-      }
-      if (J.end_sig != 1.f) {
-        xe = bx * J.end_sig;
-        ye = by * J.end_sig;
-      } else {
-        xe = bx;
-        ye = by;
-      }
-      const float stepX = (xe - xs) / ((float)n_samples - 1.f), stepY = (ye - ys) / ((float)n_samples - 1.f);
-      float x = xs, y = ys;
-      const int i0 = c * kRefineChunk;
-      for (int i = 0; i < i0; ++i) {
-        x += stepX;
-        y += stepY;
-      }
-      const float* H = sH[g];
-      const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
-      float v[kRefineChunk];
-      bool in[kRefineChunk];
-      bool oob = false;
-#pragma unroll
-      for (int t = 0; t < kRefineChunk; ++t) {
-        float xr, yr;
-        apply_h(xr, yr, H, x, y);
-        in[t] = i0 + t < n_samples && xr >= 0.f && xr < xmax && yr >= 0.f && yr < ymax;
-        oob |= (i0 + t < n_samples) && !in[t];
-        v[t] = in[t] ? im.bilinear(xr, yr) : 0.f;
-        x += stepX;
-        y += stepY;
-      }
-      float* dst = scr + ((size_t)g * kMaxCuts + k) * kMaxSamples + i0;
-#pragma unroll
-      for (int t = 0; t < kRefineChunk; ++t)
-        if (in[t]) dst[t] = v[t];
-      if (oob) atomicOr(&oob_here[g][k], 1);
-    }
-    __syncthreads();
-    // sticky out-of-bounds flags, in grid-point evaluation order
-    if (tid < nc) {
-      uint8_t run = sticky[tid];
-      for (int g = 0; g < G2; ++g) {
-        run |= oob_here[g][tid];
-        valid[g][tid] = !run;
-      }
-      sticky[tid] = run;
-    }
-    __syncthreads();
-    if (tid < G2) {
-      int m = 0;
-      for (int k = 0; k < nc; ++k) m += valid[tid][k];
-      mvalid[tid] = m;
-    }
-    __syncthreads();
-    // costFunctionGlob: per sample, m sum s^2 - (sum s)^2 over valid cuts
-    __threadfence_block();
-    for (int q = tid; q < G2 * n_samples; q += kRefineThreads) {
-      const int g = q / n_samples, ii = q - g * n_samples;
-      double s1 = 0.0, s2 = 0.0;
-      for (int k = 0; k < nc; ++k)
-        if (valid[g][k]) {
-          const double s = (double)scr[((size_t)g * kMaxCuts + k) * kMaxSamples + ii];
-          s1 += s;
-          s2 += s * s;
-        }
-      contrib[g][ii] = (double)mvalid[g] * s2 - s1 * s1;
-    }
-    __syncthreads();
-    if (tid < G2) {
-      const int m = mvalid[tid];
-      if (m < 2) {
-        res[tid] = FLT_MAX;
-      } else {
-        double acc = 0.0;
-        for (int ii = 0; ii < n_samples; ++ii) acc += contrib[tid][ii];
-        const int pairs = m * (m - 1) / 2;
-        res[tid] = (float)acc / (float)pairs;
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      int b = -1;
-      float mr = FLT_MAX;
-      for (int g = 0; g < G2; ++g)
-        if (mvalid[g] >= 2 && res[g] < mr) {
-          mr = res[g];
-          b = g;
-        }
-      if (b < 0) {
-        ok = 0;
-      } else {
-        best_g = b;
-        cur_x = gx[b];
-        cur_y = gy[b];
-        neighbour /= (float)((grid_n - 1) / 2);
-      }
-    }
-    __syncthreads();
-    if (!ok) break;
-    min_res = res[best_g];
+    const int l = k + r;
+    double acc = 0.0;
+    for (int i = 0; i < n_samples; ++i) acc += (double)sig[k][i] * (double)sig[l][i];
+    C[q] = acc;
   }
-  RefineOutDev& O = out[blockIdx.x];
-  if (!ok || J.n_iter == 0) {
+  if (tid < nc) GD.oob[tid] = oobk[tid] != 0;
+  // the last block of this marker to get here selects the next centre
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) is_last = atomicAdd(&S.done, 1u) == (unsigned)(G2 - 1);
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const RefineGridDev* GM = grid + (size_t)m * G2MAX;
+  for (int q = tid; q < G2 * nc; q += kRefineBlock) {
+    const int gg = q / nc, k = q - gg * nc;
+    oobl[gg][k] = GM[gg].oob[k];
+  }
+  __syncthreads();
+  if (tid < nc) {
+    uint8_t run = S.sticky[tid];
+    for (int gg = 0; gg < G2; ++gg) {
+      run |= oobl[gg][tid];
+      valid[gg][tid] = !run;
+    }
+    sticky_new[tid] = run;
+  }
+  __syncthreads();
+  if (tid < G2) {
+    int cnt = 0;
+    for (int k = 0; k < nc; ++k) cnt += valid[tid][k];
+    mvalid[tid] = cnt;
+  }
+  __syncthreads();
+  if (tid < G2 * kCostParts) {
+    const int gg = tid / kCostParts, pp = tid - gg * kCostParts;
+    const double* Cg = gram + ((size_t)m * G2MAX + gg) * kMaxPairs;
+    const double wdiag = (double)(mvalid[gg] - 1);
+    double acc = 0.0;
+    for (int k = pp; k < nc; k += kCostParts) {
+      if (!valid[gg][k]) continue;
+      const double* row = Cg + pair_offset(k, nc);
+      acc += wdiag * row[0];
+      for (int l = k + 1; l < nc; ++l)
+        if (valid[gg][l]) acc -= 2.0 * row[l - k];
+    }
+    part[gg][pp] = acc;
+  }
+  __syncthreads();
+  if (tid < G2) {
+    const int mv = mvalid[tid];
+    if (mv < 2) {
+      res[tid] = FLT_MAX;
+    } else {
+      double acc = 0.0;
+      for (int pp = 0; pp < kCostParts; ++pp) acc += part[tid][pp];
+      const int pairs = mv * (mv - 1) / 2;
+      res[tid] = (float)acc / (float)pairs;
+    }
+  }
+  __syncthreads();
+  if (tid < nc) S.sticky[tid] = sticky_new[tid];
+  if (tid == 0) {
+    int b = -1;
+    float mr = FLT_MAX;
+    for (int gg = 0; gg < G2; ++gg)
+      if (mvalid[gg] >= 2 && res[gg] < mr) {
+        mr = res[gg];
+        b = gg;
+      }
+    if (b < 0) {
+      S.ok = 0;
+    } else {
+      S.best_g = b;
+      S.cx = GM[b].gx;
+      S.cy = GM[b].gy;
+      S.neighbour /= (float)((grid_n - 1) / 2);
+      S.min_res = res[b];
+    }
+    S.done = 0u;
+  }
+}
+
+// Signals at the last iteration's best grid point, then the per-cut
+// statistics and distances of orazioDistanceRobust (Identification.cpp:
+// 88-160).
+__global__ __launch_bounds__(kRefineBlock) void k_refine_final(
+    const RefineJobDev* __restrict__ jobs, int n_samples, const float* __restrict__ inv_ratio,
+    int bank_rows, int bank_cols, const RefineStateDev* __restrict__ state,
+    const RefineGridDev* __restrict__ grid, RefineOutDev* __restrict__ out) {
+  __shared__ float sig[kMaxCuts][kMaxSamples + 1];
+  __shared__ float sH[9], sHi[9];
+  __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const RefineJobDev& J = jobs[m];
+  const RefineStateDev& S = state[m];
+  RefineOutDev& O = out[m];
+  if (!S.ok || J.n_iter == 0) {
     if (tid == 0) O.ok = 0;
     return;
   }
-  const int bg = best_g;
+  const int nc = J.n_cuts;
+  const Img im{J.img, J.iw, J.ih};
+  const RefineGridDev& GB = grid[(size_t)m * G2MAX + S.best_g];
+  if (tid < 9) {
+    sH[tid] = GB.H[tid];
+    sHi[tid] = GB.Hi[tid];
+  }
+  __syncthreads();
+  const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
+  for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
+    const int k = w / nchunk, c = w - k * nchunk;
+    sample_chunk(im, sH, sHi, J.stops[k], J.begin_sig, J.end_sig, n_samples, c, sig[k]);
+  }
+  __syncthreads();
   if (tid == 0) {
     O.ok = 1;
-    O.cx = cur_x;
-    O.cy = cur_y;
-    for (int r = 0; r < 9; ++r) O.H[r] = sH[bg][r];
-    O.min_res = min_res;
+    O.cx = S.cx;
+    O.cy = S.cy;
+    for (int r = 0; r < 9; ++r) O.H[r] = sH[r];
+    O.min_res = S.min_res;
   }
-  if (tid < nc) O.oob[tid] = sticky[tid];
-  for (int q = tid; q < nc * n_samples; q += kRefineThreads) {
+  if (tid < nc) O.oob[tid] = S.sticky[tid];
+  for (int q = tid; q < nc * n_samples; q += kRefineBlock) {
     const int k = q / n_samples, ii = q - k * n_samples;
-    O.signals[k * kMaxSamples + ii] = scr[((size_t)bg * kMaxCuts + k) * kMaxSamples + ii];
+    O.signals[k * kMaxSamples + ii] = sig[k][ii];
   }
   // orazioDistanceRobust statistics per valid cut (Identification.cpp:88-121)
-  if (tid < nc && !sticky[tid]) {
-    const float* s = scr + ((size_t)bg * kMaxCuts + tid) * kMaxSamples;
+  if (tid < nc && !S.sticky[tid]) {
+    const float* sg = sig[tid];
     MeanVar mv;
-    for (int i = 30; i < n_samples; ++i) mv.add(s[i]);
+    for (int i = 30; i < n_samples; ++i) mv.add(sg[i]);
     float sumInf = 0, sumSup = 0;
     unsigned nInf = 0, nSup = 0;
     bool acc = false;
     for (int i = 0; i < n_samples; ++i) {
-      const float v = s[i];
+      const float v = sg[i];
       if (!acc && v < mv.mean) acc = true;
       if (acc) {
         if (v < mv.mean) {
@@ -503,10 +592,10 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
   }
   __syncthreads();
   // distance of each valid cut to each bank profile (:124-160)
-  for (int q = tid; q < nc * bank_rows; q += kRefineThreads) {
+  for (int q = tid; q < nc * bank_rows; q += kRefineBlock) {
     const int k = q / bank_rows, idc = q - k * bank_rows;
-    if (sticky[k]) continue;
-    const float* s = scr + ((size_t)bg * kMaxCuts + k) * kMaxSamples;
+    if (S.sticky[k]) continue;
+    const float* sg = sig[k];
     const float stepX = (J.end_sig - J.begin_sig) / ((float)n_samples - 1.f);
     const float mub = stat_mub[k], muw = stat_muw[k], v2 = 2.0f * stat_var[k];
     float x = J.begin_sig;
@@ -517,10 +606,10 @@ __global__ __launch_bounds__(kRefineThreads) void k_refine(
         if (inv_ratio[idc * kMaxRatios + j] <= x) ++ldum;
       float dd;
       if ((ldum & 1) != 0) {
-        const float t = fmaxf(s[i] - mub, 0.f);
+        const float t = fmaxf(sg[i] - mub, 0.f);
         dd = t * t / v2;
       } else {
-        const float t = fminf(s[i] - muw, 0.f);
+        const float t = fminf(sg[i] - muw, 0.f);
         dd = t * t / v2;
       }
       distance += dd;
@@ -539,12 +628,17 @@ int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, 
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_refine(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
-                  const float* inv_ratio, int bank_rows, int bank_cols, float* scratch,
+int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,
+                  const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_refine<<<n_jobs, kRefineThreads, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
-                                             bank_cols, scratch, out);
+  k_refine_init<<<(n_jobs + 255) / 256, 256, 0, s>>>(jobs, n_jobs, work.state);
+  const dim3 grid(grid_n * grid_n, n_jobs);
+  for (int it = 0; it < max_iter; ++it)
+    k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, work.state, work.grid,
+                                                work.gram);
+  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, inv_ratio, bank_rows, bank_cols,
+                                                 work.state, work.grid, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
