@@ -56,8 +56,7 @@ int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, 
 struct RefineStateDev {
   float cx, cy, neighbour, min_res;
   int ok, best_g;
-  unsigned done;       // blocks of the current iteration that finished
-  int pad;
+  int pad[2];
   uint8_t sticky[kMaxCuts];
 };
 // One grid point of the current iteration.

@@ -278,14 +278,14 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 }
 
 // ---------------------------------------------------------------------------
-// Centre refinement, one launch per grid-search iteration.  Block (g, m)
-// evaluates grid point g of marker m: its homography (thread 0), the signals
-// of every selected cut (LDS), their out-of-bounds flags and the Gram matrix
-// C_kl = sum_i s_ki s_li in double.  The last of the G2 blocks of a marker to
-// finish (device-scope counter) applies the sticky out-of-bounds flags in
-// grid-point order and takes the residual of grid point g over its valid
-// cuts as sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl,
-// then moves the marker to the first minimum, as the reference's loop does.
+// Centre refinement, two launches per grid-search iteration.  Block (g, m)
+// of k_refine_iter evaluates grid point g of marker m: its homography
+// (thread 0), the signals of every selected cut (LDS), their out-of-bounds
+// flags and the Gram matrix C_kl = sum_i s_ki s_li in double.  Then one
+// block per marker (k_refine_pick) applies the sticky out-of-bounds flags in
+// grid-point order, takes the residual of grid point g over its valid cuts
+// as sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl and
+// moves the marker to the first minimum, as the reference's loop does.
 constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
@@ -383,28 +383,20 @@ __global__ __launch_bounds__(256) void k_refine_init(const RefineJobDev* __restr
   S.min_res = FLT_MAX;
   S.ok = 1;
   S.best_g = 0;
-  S.done = 0u;
   for (int k = 0; k < kMaxCuts; ++k) S.sticky[k] = 0;
 }
 
 __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n,
-    RefineStateDev* state, RefineGridDev* grid, double* gram) {
+    const RefineStateDev* __restrict__ state, RefineGridDev* __restrict__ grid,
+    double* __restrict__ gram) {
   __shared__ float sH[9], sHi[9];
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
-  __shared__ int is_last;
-  __shared__ uint8_t oobl[G2MAX][kMaxCuts];
-  __shared__ uint8_t valid[G2MAX][kMaxCuts];
-  __shared__ uint8_t sticky_new[kMaxCuts];
-  __shared__ int mvalid[G2MAX];
-  __shared__ double part[G2MAX][kCostParts];
-  __shared__ float res[G2MAX];
 
   const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x;
-  const int G2 = grid_n * grid_n;
   const RefineJobDev& J = jobs[m];
-  RefineStateDev& S = state[m];
+  const RefineStateDev& S = state[m];
   if (it >= J.n_iter || !S.ok) return;
   const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
@@ -444,13 +436,26 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     C[q] = acc;
   }
   if (tid < nc) GD.oob[tid] = oobk[tid] != 0;
-  // the last block of this marker to get here selects the next centre
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) is_last = atomicAdd(&S.done, 1u) == (unsigned)(G2 - 1);
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
+
+// The selection step of an iteration, one block per marker (after every
+// grid point's block has finished: the launch boundary publishes their
+// flags and Gram matrices).
+__global__ __launch_bounds__(kRefineBlock) void k_refine_pick(
+    const RefineJobDev* __restrict__ jobs, int it, int grid_n, RefineStateDev* __restrict__ state,
+    const RefineGridDev* __restrict__ grid, const double* __restrict__ gram) {
+  __shared__ uint8_t oobl[G2MAX][kMaxCuts];
+  __shared__ uint8_t valid[G2MAX][kMaxCuts];
+  __shared__ uint8_t sticky_new[kMaxCuts];
+  __shared__ int mvalid[G2MAX];
+  __shared__ double part[G2MAX][kCostParts];
+  __shared__ float res[G2MAX];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const int G2 = grid_n * grid_n;
+  const RefineJobDev& J = jobs[m];
+  RefineStateDev& S = state[m];
+  if (it >= J.n_iter || !S.ok) return;
+  const int nc = J.n_cuts;
   const RefineGridDev* GM = grid + (size_t)m * G2MAX;
   for (int q = tid; q < G2 * nc; q += kRefineBlock) {
     const int gg = q / nc, k = q - gg * nc;
@@ -517,7 +522,6 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
       S.neighbour /= (float)((grid_n - 1) / 2);
       S.min_res = res[b];
     }
-    S.done = 0u;
   }
 }
 
@@ -634,9 +638,11 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
   if (n_jobs <= 0) return 0;
   k_refine_init<<<(n_jobs + 255) / 256, 256, 0, s>>>(jobs, n_jobs, work.state);
   const dim3 grid(grid_n * grid_n, n_jobs);
-  for (int it = 0; it < max_iter; ++it)
+  for (int it = 0; it < max_iter; ++it) {
     k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, work.state, work.grid,
                                                 work.gram);
+    k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work.state, work.grid, work.gram);
+  }
   k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, inv_ratio, bank_rows, bank_cols,
                                                  work.state, work.grid, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
