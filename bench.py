@@ -123,7 +123,9 @@ def main():
     # per-kernel device times: a second pass of K steps, reading the HIP
     # event times the library records on its stream after each step
     kern = {}
-    sums = {"front_end_us": 0.0, "stage_us": 0.0, "host_us": 0.0}
+    sums = {k: 0.0 for k in ("front_end_us", "stage_us", "host_us", "load_us", "loop_one_us",
+                             "loop_two_us", "labels_us", "loop_three_us", "rescale_us",
+                             "ident_us", "finish_us")}
     work = {"select_samples": 0, "refine_samples": 0}
     for _ in range(K):
         det.detect_loaded()
@@ -208,8 +210,7 @@ def main():
         "accuracy": {"identified": len(errs), "ids_correct": ids_ok, "gt_markers": len(gts),
                      "center_err_px_mean": (sum(errs) / len(errs)) if errs else None,
                      "center_err_px_max": max(errs) if errs else None},
-        "breakdown_us": {"front_end": sums["front_end_us"] / K, "stage": sums["stage_us"] / K,
-                         "host": sums["host_us"] / K},
+        "breakdown_us": {k[:-3]: round(v / K, 1) for k, v in sums.items()},
         "kernels": kernels,
     }
     det.close()

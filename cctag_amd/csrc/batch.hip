@@ -39,6 +39,7 @@ struct cctagx_batch {
   LevelStager stager;
   std::vector<LevelEdges> views;
   std::vector<std::vector<Marker>> cand, found;
+  std::vector<FrameWorkspace> workspace;  // per frame, reused across detections
   std::vector<bool> loaded;
   int filter_variant = -1, edge_blocks = 0;  // requested (-1 / 0: autotune)
   bool tuned = false;
@@ -180,6 +181,7 @@ int cctagx_batch_create(const cctagx_params* params, int n_frames, const int32_t
   b->loaded.assign(n_frames, false);
   b->cand.resize(n_frames);
   b->found.resize(n_frames);
+  b->workspace.resize(n_frames);
   *out = b.release();
   return CCTAGX_OK;
 }
@@ -252,7 +254,7 @@ int cctagx_batch_detect(cctagx_batch* b, cctagx_marker* out, int max_out, int32_
     parallel_for(b->pool.get(), b->n_frames, [&](int f) {
       std::vector<LevelEdges> lv(b->views.begin() + (size_t)f * nl,
                                  b->views.begin() + (size_t)(f + 1) * nl);
-      frame_candidates(lv, b->host, nullptr, b->cand[f]);
+      frame_candidates(lv, b->host, nullptr, b->cand[f], nullptr, &b->workspace[f]);
     });
     b->stats.candidates_us = (float)(ms_since(t) * 1000.0);
     t = Clock::now();

@@ -61,6 +61,8 @@ struct cctagx_handle {
   std::unique_ptr<TaskPool> pool;  // host candidate-stage workers
   double host_ms = 0;  // last detection: host wall time after the front end
   double stage_ms = 0; // last detection: wall time of the level staging copies
+  HostTimes host_times;  // last detection: host stages
+  FrameWorkspace workspace;  // host edge sets reused across detections
   bool have_frame = false;
 };
 
@@ -184,7 +186,9 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   try {
     const PlaneDesc& d0 = h->fe.planes().planes[0];
     h->ident.set_image(h->fe.buffers().img + d0.pix, d0.w, d0.h);
-    detect_markers(h->views, h->host, h->bank, &h->ident, h->markers, h->pool.get());
+    h->host_times = HostTimes();
+    detect_markers(h->views, h->host, h->bank, &h->ident, h->markers, h->pool.get(),
+                   &h->host_times, &h->workspace);
   } catch (const std::exception& e) {
     set_error(e.what());
     return CCTAGX_ERR_HIP;
@@ -240,6 +244,15 @@ int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
   st->n_refine = h->ident.last_refine_jobs();
   st->select_samples = h->ident.last_select_samples();
   st->refine_samples = h->ident.last_refine_samples();
+  const HostTimes& t = h->host_times;
+  st->load_us = (float)(t.load * 1e3);
+  st->loop_one_us = (float)(t.loop_one * 1e3);
+  st->loop_two_us = (float)(t.loop_two * 1e3);
+  st->labels_us = (float)(t.labels * 1e3);
+  st->loop_three_us = (float)(t.loop_three * 1e3);
+  st->rescale_us = (float)(t.rescale * 1e3);
+  st->ident_us = (float)(t.ident * 1e3);
+  st->finish_us = (float)(t.finish * 1e3);
   return CCTAGX_OK;
 }
 

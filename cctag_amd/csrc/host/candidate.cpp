@@ -8,10 +8,13 @@
 #include "candidate.hpp"
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <deque>
 #include <list>
 #include <memory>
+#include <stdexcept>
 #include <unordered_set>
 
 #ifdef CCTAGX_TRACE
@@ -75,7 +78,7 @@ class AuxFlags {
 };
 }  // namespace
 
-void EdgeSet::load(const LevelEdges& L) {
+void EdgeSet::prepare(const LevelEdges& L) {
   w_ = L.w;
   h_ = L.h;
   shape_w_ = L.shape_w;
@@ -93,7 +96,12 @@ void EdgeSet::load(const LevelEdges& L) {
   vindex_ = L.vindex;
   vlist_ = L.vlist;
   pts.resize(L.n);
-  for (int i = 0; i < L.n; ++i) {
+  pin_.assign(L.n, 0);
+  seeds.assign(L.seeds, L.seeds + L.n_seeds);
+}
+
+void EdgeSet::fill(const LevelEdges& L, int begin, int end) {
+  for (int i = begin; i < end; ++i) {
     EdgePoint& e = pts[i];
     e.x = L.x[i];
     e.y = L.y[i];
@@ -105,8 +113,6 @@ void EdgeSet::load(const LevelEdges& L) {
     e.processed = 0;
     e.n_segment_out = -1;
   }
-  pin_.assign(L.n, 0);
-  seeds.assign(L.seeds, L.seeds + L.n_seeds);
 }
 
 Marker Marker::make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int level,
@@ -137,9 +143,42 @@ bool Marker::is_equal(const Marker& other) const {
 }
 
 // ---------------------------------------------------------------------------
+// Points already visited by one linking (a flat open-addressing set of
+// packed coordinates; a linking visits at most a few hundred points).
+class VisitedSet {
+ public:
+  VisitedSet() { slots_.fill(kEmpty); }
+  bool contains(uint32_t k) const {
+    for (uint32_t i = hash(k);; i = (i + 1) & (kSize - 1)) {
+      if (slots_[i] == k) return true;
+      if (slots_[i] == kEmpty) return false;
+    }
+  }
+  void insert(uint32_t k) {
+    if (count_ * 2 >= kSize) throw std::length_error("VisitedSet full");
+    for (uint32_t i = hash(k);; i = (i + 1) & (kSize - 1)) {
+      if (slots_[i] == k) return;
+      if (slots_[i] == kEmpty) {
+        slots_[i] = k;
+        ++count_;
+        return;
+      }
+    }
+  }
+
+ private:
+  static constexpr uint32_t kSize = 2048, kEmpty = 0xFFFFFFFFu;
+  static uint32_t hash(uint32_t k) { return (k * 2654435761u) >> (32 - 11); }
+  std::array<uint32_t, kSize> slots_;
+  size_t count_ = 0;
+};
+
 // Vote.cpp:255-394: convex edge linking from a seed in both directions.
-static void edge_linking_dir(EdgeSet& es, std::unordered_set<uint32_t>& processed, int p,
-                             int dir, std::deque<int>& seg, size_t window, float average_vote_min) {
+// Points the reference flags processed_in are appended to `marks` (the
+// flags themselves are set by the caller, in seed order).
+static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, int dir,
+                             std::deque<int>& seg, size_t window, float average_vote_min,
+                             std::vector<int>& marks) {
   std::deque<float> phi;
   size_t i = 0;
   bool found = true;
@@ -164,7 +203,7 @@ static void edge_linking_dir(EdgeSet& es, std::unordered_set<uint32_t>& processe
         const int sx = es.pts[p].x + xoff[k];
         const int sy = es.pts[p].y + yoff[k];
         if (sx >= 0 && sx < es.shape_w() && sy >= 0 && sy < es.shape_h() && es.at(sx, sy) >= 0 &&
-            processed.find(packxy(sx, sy)) == processed.end()) {
+            !processed.contains(packxy(sx, sy))) {
           if (phi.size() == window) {
             if (dir * std::sin(phi.back() - phi.front()) < 0.0f) stop = kConvexityLost;
           } else {
@@ -193,24 +232,24 @@ static void edge_linking_dir(EdgeSet& es, std::unordered_set<uint32_t>& processe
       size_t n = 0;
       for (int q : seg) {
         if (n == seg.size() - window) break;
-        es.set_processed_in(q, true);
+        marks.push_back(q);
         ++n;
       }
     }
   } else if (stop == kEdgeNotFound) {
-    for (int q : seg) es.set_processed_in(q, true);
+    for (int q : seg) marks.push_back(q);
   }
 }
 
-static void edge_linking(EdgeSet& es, std::deque<int>& seg, int pmax, size_t window,
-                         float average_vote_min) {
-  std::unordered_set<uint32_t> processed;
+static void edge_linking(const EdgeSet& es, std::deque<int>& seg, int pmax, size_t window,
+                         float average_vote_min, std::vector<int>& marks) {
+  VisitedSet processed;
   if (pmax >= 0) {
     seg.push_back(pmax);
-    es.set_processed_in(pmax, true);
+    marks.push_back(pmax);
     processed.insert(packxy(es.pts[pmax].x, es.pts[pmax].y));
-    edge_linking_dir(es, processed, pmax, 1, seg, window, average_vote_min);
-    edge_linking_dir(es, processed, pmax, -1, seg, window, average_vote_min);
+    edge_linking_dir(es, processed, pmax, 1, seg, window, average_vote_min, marks);
+    edge_linking_dir(es, processed, pmax, -1, seg, window, average_vote_min, marks);
   }
 }
 
@@ -655,15 +694,29 @@ static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::ve
   return false;
 }
 
-// Detection.cpp:69-113.
-static void construct_flow_component(int seed, EdgeSet& es,
-                                     std::vector<std::unique_ptr<Candidate>>& loopOne,
-                                     const HostParams& p) {
+// Detection.cpp:69-113, in two steps.  The linking from a seed depends only
+// on the edge points, so it is computed for every seed up front (in
+// parallel); the reference's skip of a seed already flagged by an earlier
+// seed's linking is then applied in seed order.
+namespace {
+struct SeedLink {
+  std::deque<int> seg;
+  std::vector<int> marks;  // points the linking flags processed_in
+};
+}  // namespace
+
+static void link_seed(int seed, const EdgeSet& es, const HostParams& p, SeedLink& out) {
+  edge_linking(es, out.seg, seed, p.window_size_on_inner_elliptic_segment, p.average_vote_min,
+               out.marks);
+}
+
+static void construct_flow_component(int seed, EdgeSet& es, SeedLink& link,
+                                     std::vector<std::unique_ptr<Candidate>>& loopOne) {
   if (es.test_processed_in(seed)) return;
+  for (int q : link.marks) es.set_processed_in(q, true);
   auto cand = std::make_unique<Candidate>();
   cand->seed = seed;
-  edge_linking(es, cand->convex_segment, seed, p.window_size_on_inner_elliptic_segment,
-               p.average_vote_min);
+  cand->convex_segment = std::move(link.seg);
   int nReceivedVote = 0, nVotedPoints = 0;
   for (int q : cand->convex_segment) {
     const int vs = es.voters_size(q);
@@ -821,24 +874,61 @@ static void loop_two_iteration(std::vector<Marker>& markers, const EdgeSet& es,
 }
 
 void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
-                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers) {
+                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
+                       HostTimes* times) {
+  using Clock = std::chrono::steady_clock;
+  auto t0 = Clock::now();
+  auto lap = [&](double HostTimes::*field) {
+    const auto t1 = Clock::now();
+    if (times) times->*field += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t0 = t1;
+  };
   const int nl = (int)levels.size();
-  // Loop one per level, in seed order (Detection.cpp:603-627).
+  // Loop one (Detection.cpp:603-627): seeds by received votes
+  // (Multiresolution.cpp:239-242, std::sort: its order among equal votes is
+  // the library's; the input order is the order in which the points reached
+  // min votes, as on the CPU), then flow components in seed order.  Seeds go
+  // in waves: the next kWave seeds of every level not yet flagged are linked
+  // in parallel, then accepted in order (a seed flagged by an earlier seed of
+  // its wave is dropped, as the serial loop would skip it).
+  constexpr size_t kWave = 32;
   std::vector<std::vector<std::unique_ptr<Candidate>>> loopOne(nl);
-  parallel_for(pool, nl, [&](int l) {
+  std::vector<size_t> n_seeds(nl, 0), next(nl, 0);
+  for (int l = 0; l < nl; ++l) {
     EdgeSet& es = levels[l];
-    if (es.seeds.empty()) return;
-    // Multiresolution.cpp:239-242: seeds by received votes, std::sort (its
-    // order among equal votes is the library's; the input order is the order
-    // in which the points reached min votes, as on the CPU).
+    if (es.seeds.empty()) continue;
     if (es.seeds.size() > 1)
       std::sort(es.seeds.begin(), es.seeds.end(),
                 [&](int a, int b) { return es.pts[a].is_max > es.pts[b].is_max; });
     const size_t nMaxSeeds = (size_t)std::max(level_h[l] / 2, p.maximum_nb_seeds);
-    const size_t nSeeds = std::min(es.seeds.size(), nMaxSeeds);
-    for (size_t i = 0; i < nSeeds; ++i) construct_flow_component(es.seeds[i], es, loopOne[l], p);
-    TRACE("level %d: %zu seeds, %zu loop-one candidates\n", l, es.seeds.size(), loopOne[l].size());
-  });
+    n_seeds[l] = std::min(es.seeds.size(), nMaxSeeds);
+  }
+  std::vector<std::pair<int, int>> wave;
+  std::vector<SeedLink> links;
+  for (;;) {
+    wave.clear();
+    for (int l = 0; l < nl; ++l) {
+      size_t taken = 0;
+      while (next[l] < n_seeds[l] && taken < kWave) {
+        const size_t i = next[l]++;
+        if (levels[l].test_processed_in(levels[l].seeds[i])) continue;
+        wave.emplace_back(l, (int)i);
+        ++taken;
+      }
+    }
+    if (wave.empty()) break;
+    links.clear();
+    links.resize(wave.size());
+    parallel_for(pool, (int)wave.size(), [&](int j) {
+      const int l = wave[j].first, i = wave[j].second;
+      link_seed(levels[l].seeds[i], levels[l], p, links[j]);
+    });
+    for (size_t j = 0; j < wave.size(); ++j) {
+      const int l = wave[j].first, i = wave[j].second;
+      construct_flow_component(levels[l].seeds[i], levels[l], links[j], loopOne[l]);
+    }
+  }
+  lap(&HostTimes::loop_one);
   // Loop two: one task per (level, candidate) (Detection.cpp:629-658).
   std::vector<std::vector<LoopTwoTask>> two(nl);
   std::vector<std::pair<int, int>> jobs;
@@ -854,8 +944,10 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     const int l = jobs[j].first, i = jobs[j].second;
     complete_flow_component(two[l][i], levels[l], (size_t)i, p);
   });
+  lap(&HostTimes::loop_two);
   std::vector<std::vector<Candidate>> loopTwo(nl);
   for (int l = 0; l < nl; ++l) assign_labels(levels[l], two[l], loopTwo[l]);
+  lap(&HostTimes::labels);
   // Loop three: one task per (level, candidate) (Detection.cpp:675-688).
   jobs.clear();
   for (int l = nl - 1; l >= 0; --l)
@@ -868,6 +960,7 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
   });
   for (std::vector<Marker>& f : found)
     for (Marker& m : f) markers.push_back(std::move(m));
+  lap(&HostTimes::loop_three);
 }
 
 // Multiresolution.cpp:43-170.
@@ -940,7 +1033,7 @@ void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const 
   });
 }
 
-void update_markers(std::vector<Marker>& markers, const Marker& add) {
+void update_markers(std::vector<Marker>& markers, Marker&& add) {
   bool flag = false;
   for (Marker& cur : markers) {
     if (cur.status > 0 && add.status > 0 && cur.is_equal(add)) {
@@ -948,7 +1041,7 @@ void update_markers(std::vector<Marker>& markers, const Marker& add) {
       flag = true;
     }
   }
-  if (!flag) markers.push_back(add);
+  if (!flag) markers.push_back(std::move(add));
 }
 
 }  // namespace cctagx

@@ -46,7 +46,14 @@ struct EdgePoint {
 // EdgePointCollection (Types.hpp:19-158).
 class EdgeSet {
  public:
-  void load(const LevelEdges& L);
+  void load(const LevelEdges& L) {
+    prepare(L);
+    fill(L, 0, L.n);
+  }
+  // load() in two steps: sizes, lookup tables and flags, then the points
+  // [begin, end) (independent ranges may be filled concurrently).
+  void prepare(const LevelEdges& L);
+  void fill(const LevelEdges& L, int begin, int end);
   // Point index at (x, y) or -1; the caller guarantees 0 <= x < shape_w and
   // 0 <= y < shape_h (the collection's shape is the level-0 size).
   int at(int x, int y) const {
@@ -135,15 +142,22 @@ struct Marker {
 // seeded 271828 and its own auxiliary point flags; the segment labels of
 // loop two are assigned afterwards in candidate order.  The result does not
 // depend on the number of threads.
+// Wall time of the host stages of one detection, milliseconds.
+struct HostTimes {
+  double load = 0, loop_one = 0, loop_two = 0, labels = 0, loop_three = 0, rescale = 0,
+         ident = 0, finish = 0;
+};
+
 void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
-                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers);
+                       const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
+                       HostTimes* times = nullptr);
 
 // Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges,
 // one task per marker (own rand_5_k stream seeded 271828).
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
                      TaskPool* pool);
 
-// Multiresolution.cpp:172-193.
-void update_markers(std::vector<Marker>& markers, const Marker& m);
+// Multiresolution.cpp:172-193 (`m` is consumed).
+void update_markers(std::vector<Marker>& markers, Marker&& m);
 
 }  // namespace cctagx

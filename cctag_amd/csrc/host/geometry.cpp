@@ -297,7 +297,8 @@ void sorted_outer_points(const Ellipse& e, const std::vector<Pt2>& points, std::
 }
 
 float median_ref(std::vector<float>& v) {
-  std::sort(v.begin(), v.end());
+  // the element a full sort would put at n/2 (leaves v partially ordered)
+  std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
   return v[v.size() / 2];
 }
 

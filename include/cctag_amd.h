@@ -169,6 +169,12 @@ typedef struct cctagx_detect_stats {
   int32_t n_refine;
   int64_t select_samples;
   int64_t refine_samples;
+  /* host wall time of the stages inside host_us: edge sets, seed linking
+   * (loop one), flow components (loop two), segment labels, candidate
+   * checks (loop three), level-0 re-fit, identification (including its two
+   * kernels) and de-duplication */
+  float load_us, loop_one_us, loop_two_us, labels_us, loop_three_us, rescale_us, ident_us,
+      finish_us;
 } cctagx_detect_stats;
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);
 
