@@ -63,6 +63,8 @@ struct RefineStateDev {
 struct RefineGridDev {
   float H[9], Hi[9];
   float gx, gy;
+  double cost_local;   // residual sum assuming only this point's own flags
+  int m_local;         // valid cuts under that assumption
   uint8_t oob[kMaxCuts];
 };
 constexpr int kMaxPairs = kMaxCuts * (kMaxCuts + 1) / 2;

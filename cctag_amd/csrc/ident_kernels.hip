@@ -289,7 +289,6 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
-constexpr int kCostParts = 10;    // partial sums per grid point in the selection
 
 // getNearbyPoints point g + computeHomographyFromEllipseAndImagedCenter
 // (Identification.cpp:739-798, 1071-1109) and the inverse.
@@ -372,6 +371,25 @@ __device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, flo
 
 __device__ __forceinline__ int pair_offset(int k, int nc) { return k * nc - k * (k - 1) / 2; }
 
+// Row k's share of sum_{k<l valid} sum_i (s_ki - s_li)^2 from the Gram
+// matrix C (packed upper triangle): (m-1) C_kk - 2 sum_{l>k valid} C_kl.
+__device__ __forceinline__ double gram_row(const double* C, const uint8_t* valid, int k, int nc,
+                                           int m) {
+  if (!valid[k]) return 0.0;
+  const double* row = C + pair_offset(k, nc);
+  double acc = (double)(m - 1) * row[0];
+  for (int l = k + 1; l < nc; ++l)
+    if (valid[l]) acc -= 2.0 * row[l - k];
+  return acc;
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed tree order (lane 0 holds
+// the result): identical wherever it is evaluated.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_refine_init(const RefineJobDev* __restrict__ jobs,
                                                      int n_jobs, RefineStateDev* state) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -393,6 +411,8 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   __shared__ float sH[9], sHi[9];
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
+  __shared__ double sC[kMaxPairs];
+  __shared__ uint8_t lvalid[kMaxCuts];
 
   const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x;
   const RefineJobDev& J = jobs[m];
@@ -434,8 +454,25 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     double acc = 0.0;
     for (int i = 0; i < n_samples; ++i) acc += (double)sig[k][i] * (double)sig[l][i];
     C[q] = acc;
+    sC[q] = acc;
   }
-  if (tid < nc) GD.oob[tid] = oobk[tid] != 0;
+  if (tid < nc) {
+    GD.oob[tid] = oobk[tid] != 0;
+    lvalid[tid] = !(S.sticky[tid] | (oobk[tid] != 0));
+  }
+  __syncthreads();
+  // residual of this grid point if no earlier grid point of the iteration
+  // flags another cut (the selection checks that and recomputes otherwise)
+  if (tid < 64) {
+    int mv = 0;
+    for (int k = 0; k < nc; ++k) mv += lvalid[k];
+    const double part = tid < nc ? gram_row(sC, lvalid, tid, nc, mv) : 0.0;
+    const double sum = wave_sum(part);
+    if (tid == 0) {
+      GD.cost_local = sum;
+      GD.m_local = mv;
+    }
+  }
 }
 
 // The selection step of an iteration, one block per marker (after every
@@ -448,8 +485,10 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_pick(
   __shared__ uint8_t valid[G2MAX][kMaxCuts];
   __shared__ uint8_t sticky_new[kMaxCuts];
   __shared__ int mvalid[G2MAX];
-  __shared__ double part[G2MAX][kCostParts];
   __shared__ float res[G2MAX];
+  __shared__ double cost_local[G2MAX];
+  __shared__ int m_local[G2MAX];
+  __shared__ uint8_t sticky_prev[kMaxCuts];
   const int m = blockIdx.x, tid = threadIdx.x;
   const int G2 = grid_n * grid_n;
   const RefineJobDev& J = jobs[m];
@@ -461,9 +500,14 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_pick(
     const int gg = q / nc, k = q - gg * nc;
     oobl[gg][k] = GM[gg].oob[k];
   }
+  if (tid < G2) {
+    cost_local[tid] = GM[tid].cost_local;
+    m_local[tid] = GM[tid].m_local;
+  }
+  if (tid < nc) sticky_prev[tid] = S.sticky[tid];
   __syncthreads();
   if (tid < nc) {
-    uint8_t run = S.sticky[tid];
+    uint8_t run = sticky_prev[tid];
     for (int gg = 0; gg < G2; ++gg) {
       run |= oobl[gg][tid];
       valid[gg][tid] = !run;
@@ -477,30 +521,25 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_pick(
     mvalid[tid] = cnt;
   }
   __syncthreads();
-  if (tid < G2 * kCostParts) {
-    const int gg = tid / kCostParts, pp = tid - gg * kCostParts;
-    const double* Cg = gram + ((size_t)m * G2MAX + gg) * kMaxPairs;
-    const double wdiag = (double)(mvalid[gg] - 1);
-    double acc = 0.0;
-    for (int k = pp; k < nc; k += kCostParts) {
-      if (!valid[gg][k]) continue;
-      const double* row = Cg + pair_offset(k, nc);
-      acc += wdiag * row[0];
-      for (int l = k + 1; l < nc; ++l)
-        if (valid[gg][l]) acc -= 2.0 * row[l - k];
-    }
-    part[gg][pp] = acc;
-  }
-  __syncthreads();
-  if (tid < G2) {
-    const int mv = mvalid[tid];
-    if (mv < 2) {
-      res[tid] = FLT_MAX;
+  // grid points whose valid cuts are the ones their block assumed take the
+  // block's residual; the others are recomputed from their Gram matrix with
+  // the same row sums and reduction order
+  const int wid = tid >> 6, lane = tid & 63;
+  for (int gg = wid; gg < G2; gg += kRefineBlock / 64) {
+    bool same = mvalid[gg] == m_local[gg];
+    for (int k = 0; k < nc; ++k)
+      same = same && (valid[gg][k] == (uint8_t)!(sticky_prev[k] | oobl[gg][k]));
+    double acc;
+    if (same) {
+      acc = cost_local[gg];
     } else {
-      double acc = 0.0;
-      for (int pp = 0; pp < kCostParts; ++pp) acc += part[tid][pp];
-      const int pairs = mv * (mv - 1) / 2;
-      res[tid] = (float)acc / (float)pairs;
+      const double* Cg = gram + ((size_t)m * G2MAX + gg) * kMaxPairs;
+      const double part = lane < nc ? gram_row(Cg, valid[gg], lane, nc, mvalid[gg]) : 0.0;
+      acc = __shfl(wave_sum(part), 0, 64);
+    }
+    if (lane == 0) {
+      const int mv = mvalid[gg];
+      res[gg] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
     }
   }
   __syncthreads();
@@ -535,6 +574,8 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ float sH[9], sHi[9];
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
+  __shared__ float xs[kMaxSamples];
+  __shared__ uint8_t parity[kMaxBankRows][kMaxSamples];
   const int m = blockIdx.x, tid = threadIdx.x;
   const RefineJobDev& J = jobs[m];
   const RefineStateDev& S = state[m];
@@ -595,21 +636,34 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     stat_mub[tid] = sumInf / (float)nInf;
   }
   __syncthreads();
+  // ring parity of every bank profile at every sample position x_i
+  if (tid == 0) {
+    const float stepX = (J.end_sig - J.begin_sig) / ((float)n_samples - 1.f);
+    float x = J.begin_sig;
+    for (int i = 0; i < n_samples; ++i) {
+      xs[i] = x;
+      x += stepX;
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < bank_rows * n_samples; q += kRefineBlock) {
+    const int idc = q / n_samples, i = q - idc * n_samples;
+    int ldum = 0;
+    for (int j = 0; j < bank_cols; ++j)
+      if (inv_ratio[idc * kMaxRatios + j] <= xs[i]) ++ldum;
+    parity[idc][i] = (uint8_t)(ldum & 1);
+  }
+  __syncthreads();
   // distance of each valid cut to each bank profile (:124-160)
   for (int q = tid; q < nc * bank_rows; q += kRefineBlock) {
     const int k = q / bank_rows, idc = q - k * bank_rows;
     if (S.sticky[k]) continue;
     const float* sg = sig[k];
-    const float stepX = (J.end_sig - J.begin_sig) / ((float)n_samples - 1.f);
     const float mub = stat_mub[k], muw = stat_muw[k], v2 = 2.0f * stat_var[k];
-    float x = J.begin_sig;
     float distance = 0;
     for (int i = 0; i < n_samples; ++i) {
-      int ldum = 0;
-      for (int j = 0; j < bank_cols; ++j)
-        if (inv_ratio[idc * kMaxRatios + j] <= x) ++ldum;
       float dd;
-      if ((ldum & 1) != 0) {
+      if (parity[idc][i]) {
         const float t = fmaxf(sg[i] - mub, 0.f);
         dd = t * t / v2;
       } else {
@@ -617,7 +671,6 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
         dd = t * t / v2;
       }
       distance += dd;
-      x += stepX;
     }
     O.id_dist[k * kMaxBankRows + idc] = distance;
   }
