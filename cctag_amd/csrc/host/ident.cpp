@@ -67,6 +67,43 @@ static float compute_median(const std::vector<float>& vec) {
   return (float)((out[s - 1] + out[s - 2]) / 2.0);
 }
 
+bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2>& stops,
+                     const HostParams& p, const IdentConstants& kc, float mean_ab, RefineJob& r) {
+  const float sqrt2 = std::sqrt(2.f);
+  e.canonic_form(r.canonic, r.t_can, r.t_inv_can);
+  r.cond = Mat3::zero();
+  r.cond.m[0][0] = sqrt2 / mean_ab;
+  r.cond.m[0][2] = -sqrt2 * e.cx() / mean_ab;
+  r.cond.m[1][1] = sqrt2 / mean_ab;
+  r.cond.m[1][2] = -sqrt2 * e.cy() / mean_ab;
+  r.cond.m[2][2] = 1.f;
+  r.cond_inv = inverse3(r.cond);
+  float gs = 0.f;
+  try {
+    const Ellipse te(mul(mul(transpose(r.cond_inv), e.matrix()), r.cond_inv));
+    gs = std::max(te.a(), te.b());
+  } catch (...) {
+    return false;
+  }
+  r.grid_scale = gs;
+  r.max_semi_axis = std::max(e.a(), e.b());
+  r.neighbour0 = p.imaged_center_neighbour_size;
+  float ns = p.imaged_center_neighbour_size;
+  int it = 0;
+  const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
+  // (bounded: a non-finite semi-axis would never end the reference's loop)
+  while (ns * r.max_semi_axis > 0.02 && it < 64) {
+    ++it;
+    ns /= div;
+  }
+  r.n_iter = it;
+  r.center0 = center0;
+  r.stops = stops;
+  r.begin_sig = kc.start_sig;
+  r.end_sig = 1.f;
+  return true;
+}
+
 namespace {
 // Working state of one frame between the backend calls.
 struct FrameWork {
@@ -127,7 +164,6 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     }
   // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
   std::vector<RefineJob> rjobs;
-  const float sqrt2 = std::sqrt(2.f);
   const IdentConstants kc = ident_constants(p, bank);
   for (FrameWork& F : fw) {
     std::vector<Marker>& markers = *F.markers;
@@ -139,44 +175,16 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
       const Ellipse& e = m.rescaled_outer_ellipse;
       RefineJob r;
       r.image = F.image;
-      e.canonic_form(r.canonic, r.t_can, r.t_inv_can);
       // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
       // first ellipse it sees.
       if (!have_mean_ab) {
         mean_ab = (e.a() + e.b()) / 2.f;
         have_mean_ab = true;
       }
-      r.cond = Mat3::zero();
-      r.cond.m[0][0] = sqrt2 / mean_ab;
-      r.cond.m[0][2] = -sqrt2 * e.cx() / mean_ab;
-      r.cond.m[1][1] = sqrt2 / mean_ab;
-      r.cond.m[1][2] = -sqrt2 * e.cy() / mean_ab;
-      r.cond.m[2][2] = 1.f;
-      r.cond_inv = inverse3(r.cond);
-      float gs = 0.f;
-      try {
-        const Ellipse te(mul(mul(transpose(r.cond_inv), e.matrix()), r.cond_inv));
-        gs = std::max(te.a(), te.b());
-      } catch (...) {
+      if (!make_refine_job(e, m.center_img, F.stops[i], p, kc, mean_ab, r)) {
         F.detected[i] = status::opti_has_diverged;
         continue;
       }
-      r.grid_scale = gs;
-      r.max_semi_axis = std::max(e.a(), e.b());
-      r.neighbour0 = p.imaged_center_neighbour_size;
-      float ns = p.imaged_center_neighbour_size;
-      int it = 0;
-      const float div = (float)((p.imaged_center_n_grid_sample - 1) / 2);
-      // (bounded: a non-finite semi-axis would never end the reference's loop)
-      while (ns * r.max_semi_axis > 0.02 && it < 64) {
-        ++it;
-        ns /= div;
-      }
-      r.n_iter = it;
-      r.center0 = m.center_img;
-      r.stops = F.stops[i];
-      r.begin_sig = kc.start_sig;
-      r.end_sig = 1.f;
       rjobs.push_back(std::move(r));
       F.rmarker.push_back(i);
     }

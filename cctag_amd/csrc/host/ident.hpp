@@ -97,6 +97,13 @@ struct IdentConstants {
 };
 IdentConstants ident_constants(const HostParams& p, const std::vector<std::vector<float>>& bank);
 
+// The grid search's per-marker constants (Identification.cpp:1318-1370):
+// canonic form of the outer ellipse, conditioner with the cached (a+b)/2
+// `mean_ab`, grid scale and iteration count.  False when the conditioned
+// ellipse is degenerate.
+bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2>& stops,
+                     const HostParams& p, const IdentConstants& kc, float mean_ab, RefineJob& r);
+
 // Runs identification on every marker (Detection.cpp:873-965) and sets
 // status, id, centre, homography, quality.  The (a+b)/2 that
 // conditionerFromEllipse caches on its first call is taken from the first
