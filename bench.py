@@ -125,7 +125,8 @@ def main():
     kern = {}
     sums = {k: 0.0 for k in ("front_end_us", "stage_us", "host_us", "load_us", "loop_one_us",
                              "loop_two_us", "labels_us", "loop_three_us", "rescale_us",
-                             "ident_us", "finish_us")}
+                             "ident_us", "finish_us", "ident_prep_us", "ident_select_us",
+                             "ident_refine_us", "ident_post_us")}
     work = {"select_samples": 0, "refine_samples": 0}
     for _ in range(K):
         det.detect_loaded()

@@ -265,7 +265,7 @@ int cctagx_batch_detect(cctagx_batch* b, cctagx_marker* out, int max_out, int32_
         frames[f].markers = &b->cand[f];
         frames[f].image = f;
       }
-      identify_frames(frames, b->host, b->bank, b->ident);
+      identify_frames(frames, b->host, b->bank, b->ident, b->pool.get());
     }
     parallel_for(b->pool.get(), b->n_frames,
                  [&](int f) { finish_markers(b->cand[f], b->found[f]); });

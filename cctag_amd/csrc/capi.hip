@@ -253,6 +253,10 @@ int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
   st->rescale_us = (float)(t.rescale * 1e3);
   st->ident_us = (float)(t.ident * 1e3);
   st->finish_us = (float)(t.finish * 1e3);
+  st->ident_prep_us = (float)(t.ident_prep * 1e3);
+  st->ident_select_us = (float)(t.ident_select * 1e3);
+  st->ident_refine_us = (float)(t.ident_refine * 1e3);
+  st->ident_post_us = (float)(t.ident_post * 1e3);
   return CCTAGX_OK;
 }
 

@@ -1,0 +1,31 @@
+// Device-side job layout and launchers of the differentiable refinement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "host/ident.hpp"
+
+namespace cctagx {
+
+struct GradJobDev {
+  const uint8_t* img;  // the marker's image (level 0), row-major, width iw
+  int iw, ih;
+  float Q11, Q22, Q33;             // canonic form of the outer ellipse
+  float t_can[9], t_inv_can[9];    // image <-> canonic frame
+  float begin_sig, end_sig;
+  float alpha0;                    // first step of the centre descent, px
+  int n_cuts;
+  float c0x, c0y;                  // initial imaged centre
+  float2 stops[kMaxCuts];
+};
+
+int launch_grad_eval(const GradJobDev* jobs, int n, const float* H, int n_samples, bool fp64,
+                     double* cost, double* grad, hipStream_t s);
+int launch_grad_homography(const GradJobDev* jobs, int n, const float* centers, float* H,
+                           hipStream_t s);
+int launch_grad_descend(const GradJobDev* jobs, int n, int steps, int n_samples, bool fp64,
+                        float* centers, double* cost, hipStream_t s);
+
+}  // namespace cctagx

@@ -2,6 +2,7 @@
 #include "ident.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -117,163 +118,194 @@ struct FrameWork {
 }  // namespace
 
 void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
-                     const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
+                     const std::vector<std::vector<float>>& bank, IdentBackend& backend,
+                     TaskPool* pool, HostTimes* times) {
+  using Clock = std::chrono::steady_clock;
+  auto t0 = Clock::now();
+  auto lap = [&](double HostTimes::*field) {
+    const auto t1 = Clock::now();
+    if (times) times->*field += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t0 = t1;
+  };
   std::vector<FrameWork> fw;
-  for (IdentFrame& f : frames) fw.push_back(FrameWork{f.markers, f.image, {}, {}, {}, {}});
-  // --- identify_step_1 (Identification.cpp:1172-1301) ----------------------
-  std::vector<CutJob> jobs;
-  for (FrameWork& F : fw) {
-    std::vector<Marker>& markers = *F.markers;
-    const size_t n = markers.size();
-    F.detected.assign(n, -1);
-    F.stops.assign(n, {});
-    for (size_t i = 0; i < n; ++i) {
-      const Marker& m = markers[i];
-      const Ellipse& e = m.rescaled_outer_ellipse;
-      CutJob job;
-      job.image = F.image;
-      sorted_outer_points(e, m.rescaled_outer_ellipse_points, job.outer_points,
-                          (size_t)p.n_samples_outer_ellipse);
-      if (job.outer_points.size() < 5) {
-        F.detected[i] = status::too_few_outer_points;
-        continue;
-      }
-      job.cx = e.cx();
-      job.cy = e.cy();
-      job.ca = std::cos((double)e.angle());
-      job.sa = std::sin((double)e.angle());
-      job.ea = e.a();
-      job.eb = e.b();
-      job.scale = m.scale;
-      jobs.push_back(std::move(job));
-      F.job_marker.push_back(i);
-    }
+  std::vector<std::pair<int, int>> all;  // (frame, marker)
+  for (size_t f = 0; f < frames.size(); ++f) {
+    IdentFrame& fr = frames[f];
+    fw.push_back(FrameWork{fr.markers, fr.image, {}, {}, {}, {}});
+    const size_t n = fr.markers->size();
+    fw.back().detected.assign(n, -1);
+    fw.back().stops.assign(n, {});
+    for (size_t i = 0; i < n; ++i) all.emplace_back((int)f, (int)i);
   }
+  // --- identify_step_1 (Identification.cpp:1172-1301) ----------------------
+  std::vector<CutJob> slot(all.size());
+  std::vector<uint8_t> have(all.size(), 0);
+  parallel_for(pool, (int)all.size(), [&](int q) {
+    FrameWork& F = fw[all[q].first];
+    const Marker& m = (*F.markers)[all[q].second];
+    const Ellipse& e = m.rescaled_outer_ellipse;
+    CutJob& job = slot[q];
+    job.image = F.image;
+    sorted_outer_points(e, m.rescaled_outer_ellipse_points, job.outer_points,
+                        (size_t)p.n_samples_outer_ellipse);
+    if (job.outer_points.size() < 5) return;
+    job.cx = e.cx();
+    job.cy = e.cy();
+    job.ca = std::cos((double)e.angle());
+    job.sa = std::sin((double)e.angle());
+    job.ea = e.a();
+    job.eb = e.b();
+    job.scale = m.scale;
+    have[q] = 1;
+  });
+  std::vector<CutJob> jobs;
+  for (size_t q = 0; q < all.size(); ++q) {
+    FrameWork& F = fw[all[q].first];
+    if (!have[q]) {
+      F.detected[all[q].second] = status::too_few_outer_points;
+      continue;
+    }
+    jobs.push_back(std::move(slot[q]));
+    F.job_marker.push_back(all[q].second);
+  }
+  lap(&HostTimes::ident_prep);
   std::vector<CutSelection> sel;
   backend.select_cuts(jobs, p, sel);
+  lap(&HostTimes::ident_select);
   size_t j = 0;
   for (FrameWork& F : fw)
     for (size_t i : F.job_marker) {
-      const CutSelection& c = sel[j++];
+      CutSelection& c = sel[j++];
       if (c.n_collected == 0) F.detected[i] = status::no_collected_cuts;
       else if (c.stops.empty()) F.detected[i] = status::no_selected_cuts;
       else {
         F.detected[i] = status::id_reliable;
-        F.stops[i] = c.stops;
+        F.stops[i] = std::move(c.stops);
       }
     }
   // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
-  std::vector<RefineJob> rjobs;
+  // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
+  // first ellipse it sees: per frame, the first marker refined.
   const IdentConstants kc = ident_constants(p, bank);
-  for (FrameWork& F : fw) {
-    std::vector<Marker>& markers = *F.markers;
-    bool have_mean_ab = false;
-    float mean_ab = 0.f;
-    for (size_t i = 0; i < markers.size(); ++i) {
-      if (F.detected[i] != status::id_reliable) continue;
-      const Marker& m = markers[i];
-      const Ellipse& e = m.rescaled_outer_ellipse;
-      RefineJob r;
-      r.image = F.image;
-      // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
-      // first ellipse it sees.
-      if (!have_mean_ab) {
-        mean_ab = (e.a() + e.b()) / 2.f;
-        have_mean_ab = true;
+  std::vector<float> mean_ab(fw.size(), 0.f);
+  for (size_t f = 0; f < fw.size(); ++f)
+    for (size_t i = 0; i < fw[f].markers->size(); ++i)
+      if (fw[f].detected[i] == status::id_reliable) {
+        const Ellipse& e = (*fw[f].markers)[i].rescaled_outer_ellipse;
+        mean_ab[f] = (e.a() + e.b()) / 2.f;
+        break;
       }
-      if (!make_refine_job(e, m.center_img, F.stops[i], p, kc, mean_ab, r)) {
-        F.detected[i] = status::opti_has_diverged;
-        continue;
-      }
-      rjobs.push_back(std::move(r));
-      F.rmarker.push_back(i);
-    }
+  std::vector<RefineJob> rslot(all.size());
+  std::fill(have.begin(), have.end(), 0);
+  parallel_for(pool, (int)all.size(), [&](int q) {
+    const int f = all[q].first, i = all[q].second;
+    FrameWork& F = fw[f];
+    if (F.detected[i] != status::id_reliable) return;
+    const Marker& m = (*F.markers)[i];
+    rslot[q].image = F.image;
+    have[q] = make_refine_job(m.rescaled_outer_ellipse, m.center_img, F.stops[i], p, kc,
+                              mean_ab[f], rslot[q]) ? 1 : 2;
+  });
+  std::vector<RefineJob> rjobs;
+  for (size_t q = 0; q < all.size(); ++q) {
+    FrameWork& F = fw[all[q].first];
+    if (have[q] == 2) F.detected[all[q].second] = status::opti_has_diverged;
+    if (have[q] != 1) continue;
+    rjobs.push_back(std::move(rslot[q]));
+    F.rmarker.push_back(all[q].second);
   }
+  lap(&HostTimes::ident_prep);
   std::vector<RefineResult> res;
   backend.refine(rjobs, p, res);
+  lap(&HostTimes::ident_refine);
   const int nb = (int)bank.size();
-  j = 0;
-  for (FrameWork& F : fw) {
+  std::vector<std::pair<int, size_t>> rj_at;  // refine job -> (frame, marker)
+  for (size_t f = 0; f < fw.size(); ++f)
+    for (size_t i : fw[f].rmarker) rj_at.emplace_back((int)f, i);
+  parallel_for(pool, (int)rj_at.size(), [&](int jj) {
+    FrameWork& F = fw[rj_at[jj].first];
+    const size_t i = rj_at[jj].second;
     std::vector<Marker>& markers = *F.markers;
     std::vector<int>& detected = F.detected;
-    for (size_t i : F.rmarker) {
-      const RefineJob& rj = rjobs[j];
-      const RefineResult& r = res[j++];
-      Marker& m = markers[i];
-      if (!r.ok) {
-        m.quality = 1.f / FLT_MAX;
-        m.center_img = Pt2(0.f, 0.f);
-        detected[i] = status::opti_has_diverged;
-        continue;
-      }
-      m.center_img = r.center;
-      m.homography = r.H;
-      const size_t ncut = rj.stops.size();
-      const size_t ns = r.signals.size() / ncut;
-      std::vector<size_t> correct;
-      for (size_t k = 0; k < ncut; ++k)
-        if (!r.oob[k]) correct.push_back(k);
-      std::vector<float> bar(ns), along(correct.size());
-      for (size_t s = 0; s < ns; ++s) {
-        for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
-        bar[s] = compute_median(along);
-      }
-      const float vMin = *std::min_element(bar.begin(), bar.end());
-      const float vMax = *std::max_element(bar.begin(), bar.end());
-      const float magnitude = vMax - vMin;
-      const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
-      m.quality = 1.f / residual;
-      if (residual > 2.7f) {
-        detected[i] = status::opti_has_diverged;
-        continue;
-      }
-      // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
-      std::vector<std::list<float>> vScore(nb);
-      for (size_t k = 0; k < ncut; ++k) {
-        if (r.oob[k]) continue;
-        std::map<float, int> sortedId;
-        for (int idc = 0; idc < nb; ++idc) {
-          const float v = std::exp(-r.id_dist[k * nb + idc]);
-          sortedId[v] = idc;
-        }
-        const auto best = sortedId.rbegin();
-        vScore[best->second].push_back(best->first);
-      }
-      size_t maxSize = 0;
-      int iMax = 0;
-      for (int c = 0; c < nb; ++c)
-        if (vScore[c].size() > maxSize) {
-          iMax = c;
-          maxSize = vScore[c].size();
-        }
-      float score = 0;
-      for (float v : vScore[iMax]) score += v;
-      score /= vScore[iMax].size();
-      m.id = iMax;
-      m.radius_ratios = bank[iMax];
-      try {
-        const Mat3 inv = inverse3(m.homography);
-        for (float rr : m.radius_ratios) {
-          const Ellipse circle(0.f, 0.f, 1.f / rr, 1.f / rr, 0.f);
-          const Ellipse ring(mul(mul(transpose(inv), circle.matrix()), inv));
-          (void)ring;
-        }
-      } catch (...) {
-        detected[i] = status::degenerate;
-        continue;
-      }
-      detected[i] = (score > p.min_ident_proba) ? status::id_reliable : status::id_not_reliable;
+    const RefineJob& rj = rjobs[jj];
+    const RefineResult& r = res[jj];
+    Marker& m = markers[i];
+    if (!r.ok) {
+      m.quality = 1.f / FLT_MAX;
+      m.center_img = Pt2(0.f, 0.f);
+      detected[i] = status::opti_has_diverged;
+      return;
     }
-    for (size_t i = 0; i < markers.size(); ++i) markers[i].status = detected[i];
-  }
+    m.center_img = r.center;
+    m.homography = r.H;
+    const size_t ncut = rj.stops.size();
+    const size_t ns = r.signals.size() / ncut;
+    std::vector<size_t> correct;
+    for (size_t k = 0; k < ncut; ++k)
+      if (!r.oob[k]) correct.push_back(k);
+    std::vector<float> bar(ns), along(correct.size());
+    for (size_t s = 0; s < ns; ++s) {
+      for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
+      bar[s] = compute_median(along);
+    }
+    const float vMin = *std::min_element(bar.begin(), bar.end());
+    const float vMax = *std::max_element(bar.begin(), bar.end());
+    const float magnitude = vMax - vMin;
+    const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
+    m.quality = 1.f / residual;
+    if (residual > 2.7f) {
+      detected[i] = status::opti_has_diverged;
+      return;
+    }
+    // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
+    std::vector<std::list<float>> vScore(nb);
+    for (size_t k = 0; k < ncut; ++k) {
+      if (r.oob[k]) return;
+      std::map<float, int> sortedId;
+      for (int idc = 0; idc < nb; ++idc) {
+        const float v = std::exp(-r.id_dist[k * nb + idc]);
+        sortedId[v] = idc;
+      }
+      const auto best = sortedId.rbegin();
+      vScore[best->second].push_back(best->first);
+    }
+    size_t maxSize = 0;
+    int iMax = 0;
+    for (int c = 0; c < nb; ++c)
+      if (vScore[c].size() > maxSize) {
+        iMax = c;
+        maxSize = vScore[c].size();
+      }
+    float score = 0;
+    for (float v : vScore[iMax]) score += v;
+    score /= vScore[iMax].size();
+    m.id = iMax;
+    m.radius_ratios = bank[iMax];
+    try {
+      const Mat3 inv = inverse3(m.homography);
+      for (float rr : m.radius_ratios) {
+        const Ellipse circle(0.f, 0.f, 1.f / rr, 1.f / rr, 0.f);
+        const Ellipse ring(mul(mul(transpose(inv), circle.matrix()), inv));
+        (void)ring;
+      }
+    } catch (...) {
+      detected[i] = status::degenerate;
+      return;
+    }
+    detected[i] = (score > p.min_ident_proba) ? status::id_reliable : status::id_not_reliable;
+  });
+  for (FrameWork& F : fw)
+    for (size_t i = 0; i < F.markers->size(); ++i) (*F.markers)[i].status = F.detected[i];
+  lap(&HostTimes::ident_post);
 }
 
 void identify_markers(std::vector<Marker>& markers, const HostParams& p,
-                      const std::vector<std::vector<float>>& bank, IdentBackend& backend) {
+                      const std::vector<std::vector<float>>& bank, IdentBackend& backend,
+                      TaskPool* pool, HostTimes* times) {
   std::vector<IdentFrame> f(1);
   f[0].markers = &markers;
   f[0].image = 0;
-  identify_frames(f, p, bank, backend);
+  identify_frames(f, p, bank, backend, pool, times);
 }
 
 }  // namespace cctagx

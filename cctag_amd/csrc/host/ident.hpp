@@ -109,7 +109,8 @@ bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2
 // conditionerFromEllipse caches on its first call is taken from the first
 // marker refined in this call.
 void identify_markers(std::vector<Marker>& markers, const HostParams& p,
-                      const std::vector<std::vector<float>>& bank, IdentBackend& backend);
+                      const std::vector<std::vector<float>>& bank, IdentBackend& backend,
+                      TaskPool* pool = nullptr, HostTimes* times = nullptr);
 
 // The same for several frames at once: one select_cuts and one refine call
 // of the backend for all their markers; each frame's markers come out as
@@ -119,6 +120,7 @@ struct IdentFrame {
   int image = 0;                     // backend image index of the frame
 };
 void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
-                     const std::vector<std::vector<float>>& bank, IdentBackend& backend);
+                     const std::vector<std::vector<float>>& bank, IdentBackend& backend,
+                     TaskPool* pool = nullptr, HostTimes* times = nullptr);
 
 }  // namespace cctagx

@@ -58,7 +58,7 @@ void detect_markers(const std::vector<LevelEdges>& levels, const HostParams& p,
   std::vector<Marker> markers;
   frame_candidates(levels, p, pool, markers, times, ws);
   auto t = Clock::now();
-  if (p.do_identification && ident) identify_markers(markers, p, bank, *ident);
+  if (p.do_identification && ident) identify_markers(markers, p, bank, *ident, pool, times);
   if (times) times->ident += ms_since(t);
   t = Clock::now();
   finish_markers(markers, out);

@@ -175,6 +175,9 @@ typedef struct cctagx_detect_stats {
    * kernels) and de-duplication */
   float load_us, loop_one_us, loop_two_us, labels_us, loop_three_us, rescale_us, ident_us,
       finish_us;
+  /* inside ident_us: host preparation and result reading, and the two
+   * identification steps (transfers and kernels) */
+  float ident_prep_us, ident_select_us, ident_refine_us, ident_post_us;
 } cctagx_detect_stats;
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);
 
@@ -214,6 +217,47 @@ typedef struct cctagx_batch_stats {
   int32_t n_markers;
 } cctagx_batch_stats;
 int cctagx_batch_get_stats(cctagx_batch* b, cctagx_batch_stats* out);
+
+/* ---- differentiable centre refinement ------------------------------------
+ * The identification's centre search as a differentiable function: for a
+ * marker with outer ellipse E and cut stops p_k, the score of a homography H
+ * (marker plane -> image) is the mean over valid cut pairs of the summed
+ * squared differences of their half-bilinear samples taken along
+ * pi(H (t q_k, 1)), q_k = pi(H^-1 p_k) -- what refineConicFamilyGlob
+ * (Identification.cpp:813-1158) evaluates at every grid point.  A forward /
+ * adjoint kernel pair returns the score and its gradient; a batched descent
+ * moves every marker's imaged centre c along -dC/dc with H = H(c) the closed
+ * form of computeHomographyFromEllipseAndImagedCenter (:739-798); the
+ * reference's grid search runs on the same markers as the fallback and the
+ * judge.  No counterpart in the reference. */
+typedef struct cctagx_refiner cctagx_refiner;
+typedef struct cctagx_refine_marker {
+  float ellipse[5];     /* outer ellipse: cx, cy, a, b, angle (image coordinates) */
+  float center0[2];     /* initial imaged centre */
+  int32_t n_cuts;       /* 2 .. 32 */
+  float stops[32][2];   /* cut stops on the outer edge */
+} cctagx_refine_marker;
+int cctagx_refiner_create(const cctagx_params* params, int max_markers, cctagx_refiner** out);
+int cctagx_refiner_destroy(cctagx_refiner* r);
+/* The image every marker is sampled in (row-major 8-bit, stride in bytes). */
+int cctagx_refiner_set_image(cctagx_refiner* r, const uint8_t* gray, int width, int height,
+                             size_t stride);
+int cctagx_refiner_set_markers(cctagx_refiner* r, const cctagx_refine_marker* markers, int n);
+/* Forward + adjoint at given homographies (n x 9, row-major): score (n) and
+ * gradient (n x 10: d/dh00 .. d/dh21 with h22 held, then d/d of a
+ * marker-plane translation (x, y) of the centre, H' = H [1 0 x; 0 1 y; 0 0 1]).
+ * fp64 != 0 samples in double (checking), else in float.  A marker with
+ * fewer than two valid cuts scores DBL_MAX with a zero gradient. */
+int cctagx_refiner_cost_grad(cctagx_refiner* r, const float* H, int fp64, double* cost,
+                             double* grad);
+/* H(c) of the closed form at given centres (n x 2) -> n x 9. */
+int cctagx_refiner_homography(cctagx_refiner* r, const float* centers, float* H);
+/* `steps` batched descent steps from center0: centres (n x 2) and scores (n). */
+int cctagx_refiner_descend(cctagx_refiner* r, int steps, int fp64, float* centers, double* cost);
+/* The reference grid search on the same markers: centres (n x 2) and ok flags (n). */
+int cctagx_refiner_grid(cctagx_refiner* r, float* centers, int32_t* ok);
+/* Device time of the last descend and grid calls, microseconds. */
+int cctagx_refiner_times(cctagx_refiner* r, float* descend_us, float* grid_us);
 
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
