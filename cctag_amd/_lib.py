@@ -123,6 +123,14 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RefineMarker(ctypes.Structure):
+    """cctagx_refine_marker."""
+    _fields_ = [
+        ("ellipse", c_f32 * 5), ("center0", c_f32 * 2), ("n_cuts", c_i32),
+        ("stops", (c_f32 * 2) * 32),
+    ]
+
+
 def _arr(p, n, dtype):
     if n == 0 or not p:
         return np.zeros(0, dtype=dtype)
@@ -198,6 +206,15 @@ def load() -> ctypes.CDLL:
     lib.cctagx_batch_set_partition.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.cctagx_batch_detect.argtypes = [vp, P(Marker), ctypes.c_int, P(c_i32), P(ctypes.c_int)]
     lib.cctagx_batch_get_stats.argtypes = [vp, P(BatchStats)]
+    lib.cctagx_refiner_create.argtypes = [P(Params), ctypes.c_int, P(vp)]
+    lib.cctagx_refiner_destroy.argtypes = [vp]
+    lib.cctagx_refiner_set_image.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+    lib.cctagx_refiner_set_markers.argtypes = [vp, P(RefineMarker), ctypes.c_int]
+    lib.cctagx_refiner_cost_grad.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    lib.cctagx_refiner_homography.argtypes = [vp, vp, vp]
+    lib.cctagx_refiner_descend.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.cctagx_refiner_grid.argtypes = [vp, vp, vp]
+    lib.cctagx_refiner_times.argtypes = [vp, P(c_f32), P(c_f32)]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int

@@ -184,3 +184,93 @@ class BatchDetector:
         _lib.check(self._lib.cctagx_batch_get_stats(self._h, ctypes.byref(st)),
                    "cctagx_batch_get_stats")
         return st.as_dict()
+
+
+class Refiner:
+    """Differentiable centre refinement (cctagx_refiner_*): the grid search's
+    score as a function of the homography, its gradient from a forward /
+    adjoint kernel pair, a batched descent on the imaged centres and the
+    reference grid search on the same markers."""
+
+    def __init__(self, max_markers: int, params: _lib.Params | None = None, n_crowns: int = 3):
+        self._lib = _lib.load()
+        self.params = params if params is not None else _lib.default_params(n_crowns)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.cctagx_refiner_create(ctypes.byref(self.params), int(max_markers),
+                                                   ctypes.byref(h)), "cctagx_refiner_create")
+        self._h = h
+        self.n = 0
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.cctagx_refiner_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_image(self, img: np.ndarray) -> None:
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape
+        _lib.check(self._lib.cctagx_refiner_set_image(self._h, img.ctypes.data, w, h, img.strides[0]),
+                   "cctagx_refiner_set_image")
+
+    def set_markers(self, markers) -> None:
+        """markers: dicts with ellipse (cx, cy, a, b, angle), center0 (x, y), stops (k x 2)."""
+        arr = (_lib.RefineMarker * max(len(markers), 1))()
+        for i, m in enumerate(markers):
+            arr[i].ellipse[:] = [float(v) for v in m["ellipse"]]
+            arr[i].center0[:] = [float(v) for v in m["center0"]]
+            st = np.asarray(m["stops"], np.float32)
+            arr[i].n_cuts = len(st)
+            for k, (x, y) in enumerate(st):
+                arr[i].stops[k][0] = float(x)
+                arr[i].stops[k][1] = float(y)
+        _lib.check(self._lib.cctagx_refiner_set_markers(self._h, arr, len(markers)),
+                   "cctagx_refiner_set_markers")
+        self.n = len(markers)
+
+    def cost_grad(self, H: np.ndarray, fp64: bool = False):
+        H = np.ascontiguousarray(np.asarray(H, np.float32).reshape(self.n, 9))
+        cost = np.zeros(self.n, np.float64)
+        grad = np.zeros((self.n, 10), np.float64)
+        _lib.check(self._lib.cctagx_refiner_cost_grad(self._h, H.ctypes.data, int(fp64),
+                                                      cost.ctypes.data, grad.ctypes.data),
+                   "cctagx_refiner_cost_grad")
+        return cost, grad
+
+    def homography(self, centers: np.ndarray) -> np.ndarray:
+        c = np.ascontiguousarray(np.asarray(centers, np.float32).reshape(self.n, 2))
+        H = np.zeros((self.n, 9), np.float32)
+        _lib.check(self._lib.cctagx_refiner_homography(self._h, c.ctypes.data, H.ctypes.data),
+                   "cctagx_refiner_homography")
+        return H.reshape(self.n, 3, 3)
+
+    def descend(self, steps: int = 20, fp64: bool = False):
+        c = np.zeros((self.n, 2), np.float32)
+        cost = np.zeros(self.n, np.float64)
+        _lib.check(self._lib.cctagx_refiner_descend(self._h, steps, int(fp64), c.ctypes.data,
+                                                    cost.ctypes.data), "cctagx_refiner_descend")
+        return c, cost
+
+    def grid(self):
+        c = np.zeros((self.n, 2), np.float32)
+        ok = np.zeros(self.n, np.int32)
+        _lib.check(self._lib.cctagx_refiner_grid(self._h, c.ctypes.data, ok.ctypes.data),
+                   "cctagx_refiner_grid")
+        return c, ok.astype(bool)
+
+    def times(self) -> dict:
+        d, g = ctypes.c_float(), ctypes.c_float()
+        _lib.check(self._lib.cctagx_refiner_times(self._h, ctypes.byref(d), ctypes.byref(g)),
+                   "cctagx_refiner_times")
+        return {"descend_us": d.value, "grid_us": g.value}

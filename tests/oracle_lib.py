@@ -42,6 +42,9 @@ def load():
     o.oracle_detect_threads.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(_lib.Marker), ctypes.c_int,
                                         ctypes.c_int]
+    o.oracle_grad_cost.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, vp, ctypes.POINTER(ctypes.c_int)]
+    o.oracle_grad_cost.restype = ctypes.c_double
     o.oracle_time_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
@@ -143,3 +146,18 @@ def time_detect(img: np.ndarray, iters: int = 1, params=None) -> float:
     h, w = img.shape
     prm = params if params is not None else _lib.default_params(3)
     return o.oracle_time_detect(ctypes.byref(prm), img.ctypes.data, w, h, iters)
+
+
+def grad_cost(img: np.ndarray, H, stops, n_samples: int = 100, begin: float = 0.25,
+              end: float = 1.0, grad: bool = True):
+    """Float64 refinement score at homography H (3x3) and, with grad, dC/dH."""
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    Hd = np.ascontiguousarray(np.asarray(H, np.float64).reshape(9))
+    st = np.ascontiguousarray(np.asarray(stops, np.float32).reshape(-1, 2))
+    g = np.zeros(9, np.float64)
+    nv = ctypes.c_int()
+    c = o.oracle_grad_cost(img.ctypes.data, w, h, Hd.ctypes.data, st.ctypes.data, len(st), n_samples,
+                           begin, end, g.ctypes.data if grad else None, ctypes.byref(nv))
+    return c, (g.reshape(3, 3) if grad else None), nv.value
