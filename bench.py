@@ -58,7 +58,8 @@ def frontend_bytes(name, lv):
             "pyramid": N,                                  # level 0 read, coarser levels written
             "filter": N + 4 * N,                           # u8 in, dx/dy int16 out
             "nms": 4 * N + N + 4 * N + N,                  # dx,dy in; cls, lab, root flag out
-            "cc_merge": N + 4 * N,                         # cls, labels
+            "cc_local": N + 4 * N,                         # cls in, labels out
+            "cc_border": 0,                                # tile edges only
             "cc_flatten": N + 4 * N + 4 * N + N,           # cls, labels in/out, root flag
             "thin1": N + 4 * N + N + 2 * N,                # cls, labels, roots; map + pass-1 out
             "thin2": 2 * N,                                # pass-1 in, thinned out
@@ -76,6 +77,83 @@ def frontend_bytes(name, lv):
     return tot
 
 
+def batched_comparison(frames, Detector, BatchDetector, reps=3):
+    """Uneven-size batch: the plain path frame by frame (one handle per
+    size) against one batched detection; both include the frames' uploads."""
+    if not frames:
+        return None
+    import collections
+    imgs = [f[0] for f in frames]
+    sizes = [(im.shape[1], im.shape[0]) for im in imgs]
+    dets = {sz: Detector(*sz) for sz in set(sizes)}
+    ref = [dets[sz].detect(im) for sz, im in zip(sizes, imgs)]   # warm-up and reference
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for sz, im in zip(sizes, imgs):
+            dets[sz].detect(im)
+    plain_s = (time.perf_counter() - t0) / reps
+    for d in dets.values():
+        d.close()
+    b = BatchDetector(sizes)
+    for f, im in enumerate(imgs):
+        b.load(f, im)
+    got = b.detect()                                             # autotune + graph capture
+    st0 = b.stats()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+    batch_s = (time.perf_counter() - t0) / reps
+    st = b.stats()
+    b.close()
+    n = len(imgs)
+    return {"workload": "%d frames of uneven sizes (640x480 .. 3840x2160), 1-32 markers and "
+                        "0-100 distractors each" % n,
+            "sizes": {"%dx%d" % k: v for k, v in sorted(collections.Counter(sizes).items())},
+            "markers": sum(len(r) for r in ref),
+            "plain_frames_per_s": n / plain_s, "batched_frames_per_s": n / batch_s,
+            "speedup": plain_s / batch_s, "identical_to_plain": got == ref,
+            "partition": {"filter_tile_h": [16, 32, 64][st["filter_variant"]],
+                          "edge_blocks": st["edge_blocks"], "tune_ms": st0["tune_us"] / 1e3},
+            "last_batch_us": {k: round(st[k], 1) for k in ("front_end_us", "stage_us",
+                                                            "candidates_us", "ident_us")}}
+
+
+def refine_comparison(mosaic, Refiner, steps=20, reps=3):
+    """Centre refinement of many candidate markers: batched descent on the
+    differentiable score against the reference grid search; centre error
+    against the synthetic ground truth."""
+    if mosaic is None:
+        return None
+    import numpy as np
+    img, ms = mosaic
+    r = Refiner(len(ms))
+    r.set_image(img)
+    r.set_markers(ms)
+    r.descend(steps)
+    r.grid()
+    d_us, g_us = [], []
+    for _ in range(reps):
+        c_desc, _ = r.descend(steps)
+        d_us.append(r.times()["descend_us"])
+        c_grid, ok = r.grid()
+        g_us.append(r.times()["grid_us"])
+    r.close()
+    gt = np.array([m["center_gt"] for m in ms])
+    c0 = np.array([m["center0"] for m in ms])
+
+    def err(c, sel=slice(None)):
+        e = np.hypot(*(np.asarray(c, np.float64)[sel] - gt[sel]).T)
+        return {"mean": float(e.mean()), "median": float(np.median(e)),
+                "p95": float(np.percentile(e, 95))}
+    return {"workload": "%d markers, %d cuts x 100 samples, fp32 sampling" % (len(ms), len(ms[0]["stops"])),
+            "steps": steps, "descend_ms": min(d_us) / 1e3, "grid_ms": min(g_us) / 1e3,
+            "speedup": min(g_us) / min(d_us), "grid_ok": int(ok.sum()),
+            "center_err_px": {"initial": err(c0), "descend": err(c_desc),
+                              "grid": err(c_grid, ok)}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +162,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound of the CPU baseline sample (whole frames, at least one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-frames", type=int, default=64,
+                    help="frames of the uneven-size batch (0 skips the batched comparison)")
+    ap.add_argument("--refine-markers", type=int, default=4096,
+                    help="markers of the refinement comparison (0 skips it)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -91,9 +173,20 @@ def main():
               "(--gpus %d, WORLD_SIZE %d requested)" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    # synthetic inputs of the batched and refinement comparisons, made in
+    # worker processes before this process touches the GPU
     from cctag_amd import synth
-    from cctag_amd.detector import Detector
+    side = {}
+    if args.batch_frames > 0 or args.refine_markers > 0:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(max(1, min(16, os.cpu_count() or 1))) as pool:
+            mosaic = (pool.apply_async(synth.refine_mosaic, (args.refine_markers,))
+                      if args.refine_markers > 0 else None)
+            side["batch"] = pool.map(synth.batch_frame, range(args.batch_frames))
+            side["mosaic"] = mosaic.get() if mosaic else None
+
+    import torch
+    from cctag_amd.detector import BatchDetector, Detector, Refiner
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
@@ -188,6 +281,9 @@ def main():
                "sample": "%d+1 full detections (front end, candidates, identification) of the "
                          "same 1920x1080 frame, one thread" % iters}
 
+    batched = batched_comparison(side.get("batch") or [], Detector, BatchDetector)
+    refine = refine_comparison(side.get("mosaic"), Refiner)
+
     line = {
         "metric": "1080p frames/s (16 markers), plain pipeline",
         "value": fps,
@@ -213,6 +309,8 @@ def main():
                      "center_err_px_max": max(errs) if errs else None},
         "breakdown_us": {k[:-3]: round(v / K, 1) for k, v in sums.items()},
         "kernels": kernels,
+        "batched": batched,
+        "grad_refine": refine,
     }
     det.close()
     print(json.dumps(line))

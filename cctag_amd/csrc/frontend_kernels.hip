@@ -261,7 +261,6 @@ __global__ __launch_bounds__(256) void k_nms(const PlaneDesc* __restrict__ P, in
       if (pass) c = (uint8_t)(m > high ? 3 : 1);
     }
     cls[g] = c;
-    lab[g] = c ? g : -1;
     root_strong[g] = 0;
   }
 }
@@ -300,7 +299,82 @@ __device__ void uf_merge(int32_t* L, int a, int b) {
   } while (!done);
 }
 
-__global__ __launch_bounds__(256) void k_cc_merge(const PlaneDesc* __restrict__ P, int np,
+// Two levels: k_cc_local joins the candidates of one 64x16 tile in LDS and
+// labels each with the global index of its tile-local root (the smallest
+// index of its tile-local component, since local and global raster orders
+// agree inside a tile); k_cc_border then joins across tile edges in global
+// memory.  The roots that result are the components' smallest indices, as
+// with a single global pass.
+__device__ __forceinline__ int lds_ld(const int* L, int i) {
+  return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lds_find(const int* L, int x) {
+  int p = lds_ld(L, x);
+  while (p != x) {
+    x = p;
+    p = lds_ld(L, x);
+  }
+  return x;
+}
+__device__ void lds_merge(int* L, int a, int b) {
+  bool done;
+  do {
+    a = lds_find(L, a);
+    b = lds_find(L, b);
+    if (a < b) {
+      const int old = atomicMin(L + b, a);
+      done = (old == b);
+      b = old;
+    } else if (b < a) {
+      const int old = atomicMin(L + a, b);
+      done = (old == a);
+      a = old;
+    } else {
+      done = true;
+    }
+  } while (!done);
+}
+
+__global__ __launch_bounds__(256) void k_cc_local(const PlaneDesc* __restrict__ P, int np,
+                                                   const uint8_t* __restrict__ cls,
+                                                   int32_t* __restrict__ lab) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH, N = TW * TH;
+  __shared__ int L[N];
+  __shared__ uint8_t C[N];
+  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
+  const PlaneDesc d = P[p];
+  const int t = blockIdx.x - d.tile_s;
+  const int tx_n = (d.w + TW - 1) / TW;
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  for (int k = threadIdx.x; k < N; k += 256) {
+    const int x = x0 + (k % TW), y = y0 + k / TW;
+    const uint8_t c = (x < d.w && y < d.h) ? (cls[d.pix + y * d.w + x] & 1) : 0;
+    C[k] = c;
+    L[k] = c ? k : -1;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < N; k += 256) {
+    if (!C[k]) continue;
+    const int lx = k % TW, ly = k / TW;
+    if (lx > 0 && C[k - 1]) lds_merge(L, k, k - 1);
+    if (ly > 0) {
+      if (lx > 0 && C[k - TW - 1]) lds_merge(L, k, k - TW - 1);
+      if (C[k - TW]) lds_merge(L, k, k - TW);
+      if (lx + 1 < TW && C[k - TW + 1]) lds_merge(L, k, k - TW + 1);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < N; k += 256) {
+    if (!C[k]) continue;
+    const int r = lds_find(L, k);
+    lab[d.pix + (y0 + k / TW) * d.w + x0 + k % TW] = d.pix + (y0 + r / TW) * d.w + x0 + r % TW;
+  }
+}
+
+// The neighbour pairs that cross a tile edge: left / up-left on the tile's
+// first column, up-left / up / up-right on its first row, up-right on its
+// last column.  One lane per edge pixel.
+__global__ __launch_bounds__(128) void k_cc_border(const PlaneDesc* __restrict__ P, int np,
                                                    const uint8_t* __restrict__ cls,
                                                    int32_t* __restrict__ lab) {
   constexpr int TW = kStencilTileW, TH = kStencilTileH;
@@ -308,21 +382,31 @@ __global__ __launch_bounds__(256) void k_cc_merge(const PlaneDesc* __restrict__ 
   const PlaneDesc d = P[p];
   const int t = blockIdx.x - d.tile_s;
   const int tx_n = (d.w + TW - 1) / TW;
-  const int x = (t % tx_n) * TW + (threadIdx.x & 63);
-  const int y0 = (t / tx_n) * TH;
-  if (x >= d.w) return;
-  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
-    const int y = y0 + ty;
-    if (y >= d.h) break;
-    const int g = d.pix + y * d.w + x;
-    if (!(cls[g] & 1)) continue;
-    if (x > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
-    if (y > 0) {
-      const int u = g - d.w;
-      if (x > 0 && (cls[u - 1] & 1)) uf_merge(lab, g, u - 1);
-      if (cls[u] & 1) uf_merge(lab, g, u);
-      if (x + 1 < d.w && (cls[u + 1] & 1)) uf_merge(lab, g, u + 1);
-    }
+  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  const int i = threadIdx.x;
+  int lx, ly;
+  if (i < TW) {
+    lx = i;
+    ly = 0;
+  } else if (i < TW + TH - 1) {
+    lx = 0;
+    ly = i - TW + 1;
+  } else if (i < TW + 2 * (TH - 1)) {
+    lx = TW - 1;
+    ly = i - TW - TH + 2;
+  } else {
+    return;
+  }
+  const int x = x0 + lx, y = y0 + ly;
+  if (x >= d.w || y >= d.h) return;
+  const int g = d.pix + y * d.w + x;
+  if (!(cls[g] & 1)) return;
+  if (lx == 0 && x > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
+  if (y > 0) {
+    const int u = g - d.w;
+    if ((lx == 0 || ly == 0) && x > 0 && (cls[u - 1] & 1)) uf_merge(lab, g, u - 1);
+    if (ly == 0 && (cls[u] & 1)) uf_merge(lab, g, u);
+    if ((ly == 0 || lx == TW - 1) && x + 1 < d.w && (cls[u + 1] & 1)) uf_merge(lab, g, u + 1);
   }
 }
 
@@ -915,8 +999,10 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   k_nms<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.dx, b.dy, L.prm.low, L.prm.high, b.cls, b.lab,
                                     b.root_strong);
   mark("nms");
-  k_cc_merge<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab);
-  mark("cc_merge");
+  k_cc_local<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab);
+  mark("cc_local");
+  k_cc_border<<<L.n_tiles_s, 128, 0, s>>>(P, np, b.cls, b.lab);
+  mark("cc_border");
   k_cc_flatten<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong);
   mark("cc_flatten");
   k_thin1<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong, b.edges, b.tmp);

@@ -273,11 +273,83 @@ def config_frame(config: int, k: int = 0):
     return make_frame(CONFIGS[config], 1000 * config + k)
 
 
+def batch_frame(k: int, seed: int = 4000):
+    """Frame k of the uneven-size batch (size drawn from BATCH_SIZES)."""
+    rng = np.random.Generator(np.random.PCG64(seed + k))
+    return make_frame(batch_config(rng), seed + 10000 + k)
+
+
 def batch_frames(n: int = 64, seed: int = 4000):
     """The uneven-size batch: n frames, sizes drawn from BATCH_SIZES."""
-    out = []
-    for k in range(n):
-        rng = np.random.Generator(np.random.PCG64(seed + k))
-        cfg = batch_config(rng)
-        out.append(make_frame(cfg, seed + 10000 + k))
-    return out
+    return [batch_frame(k, seed) for k in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# Inputs of the centre refinement built from ground truth: the outer ellipse
+# (the image of the unit circle), cut stops on it, the ellipse centre as the
+# start and the true imaged centre, in detector coordinates (pixel centres at
+# integer coordinates).
+
+DETECTOR_SHIFT = np.array([[1.0, 0.0, -0.5], [0.0, 1.0, -0.5], [0.0, 0.0, 1.0]])
+
+
+def ellipse_of(H: np.ndarray):
+    """(cx, cy, a, b, angle) of the image of the unit circle under H, a >= b."""
+    Hi = np.linalg.inv(H)
+    C = Hi.T @ np.diag([1.0, 1.0, -1.0]) @ Hi
+    A = C[:2, :2]
+    ctr = np.linalg.solve(A, -C[:2, 2])
+    f = C[2, 2] + C[:2, 2] @ ctr
+    lam, vec = np.linalg.eigh(A)
+    if f > 0:
+        lam, f = -lam, -f
+    ax = np.sqrt(-f / lam)
+    i = int(np.argmax(ax))
+    angle = math.atan2(vec[1, i], vec[0, i])
+    return float(ctr[0]), float(ctr[1]), float(ax[i]), float(ax[1 - i]), angle
+
+
+def refine_marker(H_synth: np.ndarray, n_cuts: int = 22, phase: float = 0.1) -> dict:
+    H = DETECTOR_SHIFT @ np.asarray(H_synth, np.float64)
+    th = phase + 2 * np.pi * np.arange(n_cuts) / n_cuts
+    pts = H @ np.stack([np.cos(th), np.sin(th), np.ones_like(th)])
+    stops = (pts[:2] / pts[2]).T
+    e = ellipse_of(H)
+    c = H @ np.array([0.0, 0.0, 1.0])
+    return {"ellipse": e, "center0": (e[0], e[1]), "stops": stops,
+            "center_gt": (c[0] / c[2], c[1] / c[2]), "H": H / H[2, 2]}
+
+
+def refine_mosaic(n: int = 4096, tile: int = 160, unique: int = 256, seed: int = 5000):
+    """n candidate markers for the refinement workload: `unique` markers
+    (outer radius 20-36 px, tilt <= 60 deg seen by a camera whose focal
+    length is 4-8 marker radii, so the ellipse centre is off the imaged
+    centre; random in-plane rotation and ID) each rendered into a tile x
+    tile patch with blur 0.5-1.5 px, tiled over a square mosaic, then noise
+    6 DN over the whole image.  Returns (uint8 image, list of refine_marker
+    dicts)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ratios = bank_ratios(3)
+    side = int(math.ceil(math.sqrt(n)))
+    tiles, hs = [], []
+    for _ in range(unique):
+        t = np.full((tile, tile), 128.0)
+        r = rng.uniform(20.0, 36.0)
+        c = tile / 2 + rng.uniform(-3, 3, size=2)
+        H = marker_homography(c[0], c[1], r, math.radians(rng.uniform(0, 60)),
+                              rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi),
+                              r * rng.uniform(4.0, 8.0))
+        _render_marker(t, H, ratios[int(rng.integers(0, len(ratios)))])
+        tiles.append(ndimage.gaussian_filter(t, rng.uniform(0.5, 1.5), mode="nearest"))
+        hs.append(H)
+    img = np.empty((side * tile, side * tile))
+    markers = []
+    for k in range(side * side):
+        u = k % unique
+        ty, tx = divmod(k, side)
+        img[ty * tile:(ty + 1) * tile, tx * tile:(tx + 1) * tile] = tiles[u]
+        if k < n:
+            T = np.array([[1.0, 0.0, tx * tile], [0.0, 1.0, ty * tile], [0.0, 0.0, 1.0]])
+            markers.append(refine_marker(T @ hs[u]))
+    img += rng.normal(0.0, 6.0, size=img.shape)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8), markers
