@@ -233,6 +233,14 @@ def main():
         for k in work:
             work[k] += ds[k]
 
+    # the drop-in call with the frame in host memory (upload included)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        det.detect(img)
+    torch.cuda.synchronize()
+    fps_upload = K / (time.perf_counter() - t0)
+
     # accuracy against the synthetic ground truth
     errs, ids_ok = [], 0
     for m in markers:
@@ -304,6 +312,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "vs_cpu": (fps / cpu["value"]) if cpu else None,
+        "with_upload_frames_per_s": fps_upload,
         "accuracy": {"identified": len(errs), "ids_correct": ids_ok, "gt_markers": len(gts),
                      "center_err_px_mean": (sum(errs) / len(errs)) if errs else None,
                      "center_err_px_max": max(errs) if errs else None},

@@ -161,8 +161,13 @@ int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStr
     }
   if (p0 < 0) return -1;
   const PlaneDesc& d = ps_.planes[p0];
-  CCTAGX_HIP(hipMemcpy2DAsync(launch_.buf.img + d.pix, d.w, gray, stride, d.w, d.h,
+  // rows packed: one contiguous copy (a 2-D copy from pageable memory goes row by row)
+  if (stride == (size_t)d.w)
+    CCTAGX_HIP(hipMemcpyAsync(launch_.buf.img + d.pix, gray, (size_t)d.w * d.h,
                               hipMemcpyHostToDevice, s));
+  else
+    CCTAGX_HIP(hipMemcpy2DAsync(launch_.buf.img + d.pix, d.w, gray, stride, d.w, d.h,
+                                hipMemcpyHostToDevice, s));
   return 0;
 }
 

@@ -40,9 +40,10 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   const size_t o_so = dev_.reserve(sizeof(SelectOutDev) * nm);
   const size_t o_rj = dev_.reserve(sizeof(RefineJobDev) * nm);
   const size_t o_ro = dev_.reserve(sizeof(RefineOutDev) * nm);
-  const size_t o_st = dev_.reserve(sizeof(RefineStateDev) * nm);
-  const size_t o_gr = dev_.reserve(sizeof(RefineGridDev) * (size_t)nm * kMaxGrid * kMaxGrid);
-  const size_t o_gm = dev_.reserve(sizeof(double) * (size_t)nm * kMaxGrid * kMaxGrid * kMaxPairs);
+  const size_t o_st = dev_.reserve(sizeof(RefineStateDev) * 2 * nm);
+  const size_t o_gr = dev_.reserve(sizeof(RefineGridDev) * 2 * (size_t)nm * kMaxGrid * kMaxGrid);
+  const size_t o_gm =
+      dev_.reserve(sizeof(double) * 2 * (size_t)nm * kMaxGrid * kMaxGrid * kMaxPairs);
   const size_t o_inv = dev_.reserve(sizeof(float) * kMaxBank * kMaxRatios);
   if (dev_.allocate() != 0) throw std::runtime_error(std::string("hipMalloc: ") + last_error());
   d_sel_jobs_ = dev_.at<SelectJobDev>(o_sj);

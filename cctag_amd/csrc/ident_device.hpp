@@ -68,10 +68,12 @@ struct RefineGridDev {
   uint8_t oob[kMaxCuts];
 };
 constexpr int kMaxPairs = kMaxCuts * (kMaxCuts + 1) / 2;
+// Two sets (by iteration parity): an iteration reads the previous set and
+// writes its own.
 struct RefineWork {
-  RefineStateDev* state;  // n_jobs
-  RefineGridDev* grid;    // n_jobs x kMaxGrid^2
-  double* gram;           // n_jobs x kMaxGrid^2 x kMaxPairs
+  RefineStateDev* state;  // 2 x n_jobs
+  RefineGridDev* grid;    // 2 x n_jobs x kMaxGrid^2
+  double* gram;           // 2 x n_jobs x kMaxGrid^2 x kMaxPairs
 };
 
 int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,

@@ -278,14 +278,16 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 }
 
 // ---------------------------------------------------------------------------
-// Centre refinement, two launches per grid-search iteration.  Block (g, m)
+// Centre refinement, one launch per grid-search iteration.  Block (g, m)
 // of k_refine_iter evaluates grid point g of marker m: its homography
 // (thread 0), the signals of every selected cut (LDS), their out-of-bounds
-// flags and the Gram matrix C_kl = sum_i s_ki s_li in double.  Then one
-// block per marker (k_refine_pick) applies the sticky out-of-bounds flags in
-// grid-point order, takes the residual of grid point g over its valid cuts
-// as sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl and
-// moves the marker to the first minimum, as the reference's loop does.
+// flags and the Gram matrix C_kl = sum_i s_ki s_li in double.  The
+// selection that follows an iteration -- the sticky out-of-bounds flags in
+// grid-point order, the residual of grid point g over its valid cuts as
+// sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl, the
+// move to the first minimum as the reference's loop does -- is redone by
+// every block of the next launch (and by the final kernel), so an
+// iteration costs one launch.
 constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
@@ -390,37 +392,155 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_refine_init(const RefineJobDev* __restrict__ jobs,
-                                                     int n_jobs, RefineStateDev* state) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_jobs) return;
-  RefineStateDev& S = state[m];
-  S.cx = jobs[m].c0x;
-  S.cy = jobs[m].c0y;
-  S.neighbour = jobs[m].neighbour0;
-  S.min_res = FLT_MAX;
-  S.ok = 1;
-  S.best_g = 0;
-  for (int k = 0; k < kMaxCuts; ++k) S.sticky[k] = 0;
+// Selection step of an iteration (what the reference's loop does after
+// evaluating its grid): the sticky out-of-bounds flags in grid-point order,
+// each point's residual (its block's own when the valid cuts are the ones it
+// assumed, otherwise recomputed from its Gram matrix with the same row sums
+// and reduction order), the first minimum, the next centre.  Called by a
+// whole block; every block that calls it with the same inputs gets the
+// same state.
+struct PickShared {
+  uint8_t oobl[G2MAX][kMaxCuts];
+  uint8_t valid[G2MAX][kMaxCuts];
+  uint8_t sticky_prev[kMaxCuts];
+  int mvalid[G2MAX];
+  float res[G2MAX];
+  double cost_local[G2MAX];
+  int m_local[G2MAX];
+};
+
+__device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateDev& prev,
+                           const RefineGridDev* GM, const double* gram_m, PickShared& P,
+                           RefineStateDev& S) {
+  const int tid = threadIdx.x, nc = J.n_cuts, G2 = grid_n * grid_n;
+  for (int q = tid; q < G2 * nc; q += kRefineBlock) {
+    const int gg = q / nc, k = q - gg * nc;
+    P.oobl[gg][k] = GM[gg].oob[k];
+  }
+  if (tid < G2) {
+    P.cost_local[tid] = GM[tid].cost_local;
+    P.m_local[tid] = GM[tid].m_local;
+  }
+  if (tid < kMaxCuts) P.sticky_prev[tid] = prev.sticky[tid];
+  __syncthreads();
+  if (tid < nc) {
+    uint8_t run = P.sticky_prev[tid];
+    for (int gg = 0; gg < G2; ++gg) {
+      run |= P.oobl[gg][tid];
+      P.valid[gg][tid] = !run;
+    }
+    S.sticky[tid] = run;
+  } else if (tid < kMaxCuts) {
+    S.sticky[tid] = P.sticky_prev[tid];
+  }
+  __syncthreads();
+  if (tid < G2) {
+    int cnt = 0;
+    for (int k = 0; k < nc; ++k) cnt += P.valid[tid][k];
+    P.mvalid[tid] = cnt;
+  }
+  __syncthreads();
+  const int wid = tid >> 6, lane = tid & 63;
+  for (int gg = wid; gg < G2; gg += kRefineBlock / 64) {
+    bool same = P.mvalid[gg] == P.m_local[gg];
+    for (int k = 0; k < nc; ++k)
+      same = same && (P.valid[gg][k] == (uint8_t)!(P.sticky_prev[k] | P.oobl[gg][k]));
+    double acc;
+    if (same) {
+      acc = P.cost_local[gg];
+    } else {
+      const double* Cg = gram_m + (size_t)gg * kMaxPairs;
+      const double part = lane < nc ? gram_row(Cg, P.valid[gg], lane, nc, P.mvalid[gg]) : 0.0;
+      acc = __shfl(wave_sum(part), 0, 64);
+    }
+    if (lane == 0) {
+      const int mv = P.mvalid[gg];
+      P.res[gg] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int b = -1;
+    float mr = FLT_MAX;
+    for (int gg = 0; gg < G2; ++gg)
+      if (P.mvalid[gg] >= 2 && P.res[gg] < mr) {
+        mr = P.res[gg];
+        b = gg;
+      }
+    S.neighbour = prev.neighbour;
+    if (b < 0) {
+      S.ok = 0;
+      S.best_g = prev.best_g;
+      S.cx = prev.cx;
+      S.cy = prev.cy;
+      S.min_res = prev.min_res;
+    } else {
+      S.ok = 1;
+      S.best_g = b;
+      S.cx = GM[b].gx;
+      S.cy = GM[b].gy;
+      S.neighbour /= (float)((grid_n - 1) / 2);
+      S.min_res = P.res[b];
+    }
+  }
+  __syncthreads();
 }
 
+// The state a marker starts iteration `it` with: the job's start for the
+// first one, otherwise the selection over the previous iteration's grid
+// (buffers of parity (it - 1) & 1).
+__device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int grid_n,
+                         const RefineWork& W, PickShared& P, RefineStateDev& S) {
+  const int tid = threadIdx.x;
+  if (it == 0) {
+    if (tid == 0) {
+      S.cx = J.c0x;
+      S.cy = J.c0y;
+      S.neighbour = J.neighbour0;
+      S.min_res = FLT_MAX;
+      S.ok = 1;
+      S.best_g = 0;
+    }
+    if (tid < kMaxCuts) S.sticky[tid] = 0;
+    __syncthreads();
+    return;
+  }
+  const int pb = (it - 1) & 1;
+  const RefineStateDev& prev = W.state[(size_t)pb * n_jobs + m];
+  if (!prev.ok) {
+    if (tid == 0) S = prev;
+    __syncthreads();
+    return;
+  }
+  pick_state(J, grid_n, prev, W.grid + ((size_t)pb * n_jobs + m) * G2MAX,
+             W.gram + ((size_t)pb * n_jobs + m) * G2MAX * kMaxPairs, P, S);
+}
+
+// Iteration `it` of the grid search: block (g, m) first derives marker m's
+// state from the previous iteration (every block of the marker derives the
+// same one; block 0 stores it), then evaluates grid point g: homography,
+// cut signals in LDS, out-of-bounds flags, Gram matrix and the residual
+// under its own flags.  Buffers alternate with the iteration's parity.
 __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
-    const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n,
-    const RefineStateDev* __restrict__ state, RefineGridDev* __restrict__ grid,
-    double* __restrict__ gram) {
+    const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, RefineWork W) {
   __shared__ float sH[9], sHi[9];
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
   __shared__ double sC[kMaxPairs];
   __shared__ uint8_t lvalid[kMaxCuts];
+  __shared__ PickShared P;
+  __shared__ RefineStateDev S;
 
-  const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x;
+  const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x, n_jobs = gridDim.y;
   const RefineJobDev& J = jobs[m];
-  const RefineStateDev& S = state[m];
-  if (it >= J.n_iter || !S.ok) return;
+  if (it >= J.n_iter) return;
+  state_at(J, it, m, n_jobs, grid_n, W, P, S);
+  const int cb = it & 1;
+  if (g == 0 && tid == 0) W.state[(size_t)cb * n_jobs + m] = S;
+  if (!S.ok) return;
   const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
-  RefineGridDev& GD = grid[(size_t)m * G2MAX + g];
+  RefineGridDev& GD = W.grid[((size_t)cb * n_jobs + m) * G2MAX + g];
   if (tid == 0) {
     float Hf[9], Hi[9], qx, qy;
     grid_homography(J, S.cx, S.cy, S.neighbour, g, grid_n, Hf, Hi, qx, qy);
@@ -442,7 +562,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
       atomicOr(&oobk[k], 1);
   }
   __syncthreads();
-  double* C = gram + ((size_t)m * G2MAX + g) * kMaxPairs;
+  double* C = W.gram + (((size_t)cb * n_jobs + m) * G2MAX + g) * kMaxPairs;
   const int npairs = nc * (nc + 1) / 2;
   for (int q = tid; q < npairs; q += kRefineBlock) {
     int k = 0, r = q;
@@ -475,118 +595,36 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   }
 }
 
-// The selection step of an iteration, one block per marker (after every
-// grid point's block has finished: the launch boundary publishes their
-// flags and Gram matrices).
-__global__ __launch_bounds__(kRefineBlock) void k_refine_pick(
-    const RefineJobDev* __restrict__ jobs, int it, int grid_n, RefineStateDev* __restrict__ state,
-    const RefineGridDev* __restrict__ grid, const double* __restrict__ gram) {
-  __shared__ uint8_t oobl[G2MAX][kMaxCuts];
-  __shared__ uint8_t valid[G2MAX][kMaxCuts];
-  __shared__ uint8_t sticky_new[kMaxCuts];
-  __shared__ int mvalid[G2MAX];
-  __shared__ float res[G2MAX];
-  __shared__ double cost_local[G2MAX];
-  __shared__ int m_local[G2MAX];
-  __shared__ uint8_t sticky_prev[kMaxCuts];
-  const int m = blockIdx.x, tid = threadIdx.x;
-  const int G2 = grid_n * grid_n;
-  const RefineJobDev& J = jobs[m];
-  RefineStateDev& S = state[m];
-  if (it >= J.n_iter || !S.ok) return;
-  const int nc = J.n_cuts;
-  const RefineGridDev* GM = grid + (size_t)m * G2MAX;
-  for (int q = tid; q < G2 * nc; q += kRefineBlock) {
-    const int gg = q / nc, k = q - gg * nc;
-    oobl[gg][k] = GM[gg].oob[k];
-  }
-  if (tid < G2) {
-    cost_local[tid] = GM[tid].cost_local;
-    m_local[tid] = GM[tid].m_local;
-  }
-  if (tid < nc) sticky_prev[tid] = S.sticky[tid];
-  __syncthreads();
-  if (tid < nc) {
-    uint8_t run = sticky_prev[tid];
-    for (int gg = 0; gg < G2; ++gg) {
-      run |= oobl[gg][tid];
-      valid[gg][tid] = !run;
-    }
-    sticky_new[tid] = run;
-  }
-  __syncthreads();
-  if (tid < G2) {
-    int cnt = 0;
-    for (int k = 0; k < nc; ++k) cnt += valid[tid][k];
-    mvalid[tid] = cnt;
-  }
-  __syncthreads();
-  // grid points whose valid cuts are the ones their block assumed take the
-  // block's residual; the others are recomputed from their Gram matrix with
-  // the same row sums and reduction order
-  const int wid = tid >> 6, lane = tid & 63;
-  for (int gg = wid; gg < G2; gg += kRefineBlock / 64) {
-    bool same = mvalid[gg] == m_local[gg];
-    for (int k = 0; k < nc; ++k)
-      same = same && (valid[gg][k] == (uint8_t)!(sticky_prev[k] | oobl[gg][k]));
-    double acc;
-    if (same) {
-      acc = cost_local[gg];
-    } else {
-      const double* Cg = gram + ((size_t)m * G2MAX + gg) * kMaxPairs;
-      const double part = lane < nc ? gram_row(Cg, valid[gg], lane, nc, mvalid[gg]) : 0.0;
-      acc = __shfl(wave_sum(part), 0, 64);
-    }
-    if (lane == 0) {
-      const int mv = mvalid[gg];
-      res[gg] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
-    }
-  }
-  __syncthreads();
-  if (tid < nc) S.sticky[tid] = sticky_new[tid];
-  if (tid == 0) {
-    int b = -1;
-    float mr = FLT_MAX;
-    for (int gg = 0; gg < G2; ++gg)
-      if (mvalid[gg] >= 2 && res[gg] < mr) {
-        mr = res[gg];
-        b = gg;
-      }
-    if (b < 0) {
-      S.ok = 0;
-    } else {
-      S.best_g = b;
-      S.cx = GM[b].gx;
-      S.cy = GM[b].gy;
-      S.neighbour /= (float)((grid_n - 1) / 2);
-      S.min_res = res[b];
-    }
-  }
-}
-
 // Signals at the last iteration's best grid point, then the per-cut
 // statistics and distances of orazioDistanceRobust (Identification.cpp:
 // 88-160).
 __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
-    const RefineJobDev* __restrict__ jobs, int n_samples, const float* __restrict__ inv_ratio,
-    int bank_rows, int bank_cols, const RefineStateDev* __restrict__ state,
-    const RefineGridDev* __restrict__ grid, RefineOutDev* __restrict__ out) {
+    const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
+    const float* __restrict__ inv_ratio, int bank_rows, int bank_cols, RefineWork W,
+    RefineOutDev* __restrict__ out) {
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ float sH[9], sHi[9];
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
   __shared__ float xs[kMaxSamples];
   __shared__ uint8_t parity[kMaxBankRows][kMaxSamples];
-  const int m = blockIdx.x, tid = threadIdx.x;
+  __shared__ PickShared P;
+  __shared__ RefineStateDev S;
+  const int m = blockIdx.x, tid = threadIdx.x, n_jobs = gridDim.x;
   const RefineJobDev& J = jobs[m];
-  const RefineStateDev& S = state[m];
   RefineOutDev& O = out[m];
-  if (!S.ok || J.n_iter == 0) {
+  if (J.n_iter == 0) {
+    if (tid == 0) O.ok = 0;
+    return;
+  }
+  // the selection after the last iteration
+  state_at(J, J.n_iter, m, n_jobs, grid_n, W, P, S);
+  if (!S.ok) {
     if (tid == 0) O.ok = 0;
     return;
   }
   const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
-  const RefineGridDev& GB = grid[(size_t)m * G2MAX + S.best_g];
+  const RefineGridDev& GB = W.grid[((size_t)((J.n_iter - 1) & 1) * n_jobs + m) * G2MAX + S.best_g];
   if (tid < 9) {
     sH[tid] = GB.H[tid];
     sHi[tid] = GB.Hi[tid];
@@ -689,15 +727,11 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_refine_init<<<(n_jobs + 255) / 256, 256, 0, s>>>(jobs, n_jobs, work.state);
   const dim3 grid(grid_n * grid_n, n_jobs);
-  for (int it = 0; it < max_iter; ++it) {
-    k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, work.state, work.grid,
-                                                work.gram);
-    k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work.state, work.grid, work.gram);
-  }
-  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, inv_ratio, bank_rows, bank_cols,
-                                                 work.state, work.grid, out);
+  for (int it = 0; it < max_iter; ++it)
+    k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, work);
+  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
+                                                 bank_cols, work, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
