@@ -291,6 +291,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
+constexpr int kFuseMaxBlocks = 1024;  // grid blocks up to which the selection is fused
 
 // getNearbyPoints point g + computeHomographyFromEllipseAndImagedCenter
 // (Identification.cpp:739-798, 1071-1109) and the inverse.
@@ -516,13 +517,18 @@ __device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int g
              W.gram + ((size_t)pb * n_jobs + m) * G2MAX * kMaxPairs, P, S);
 }
 
-// Iteration `it` of the grid search: block (g, m) first derives marker m's
-// state from the previous iteration (every block of the marker derives the
-// same one; block 0 stores it), then evaluates grid point g: homography,
-// cut signals in LDS, out-of-bounds flags, Gram matrix and the residual
-// under its own flags.  Buffers alternate with the iteration's parity.
+// Iteration `it` of the grid search: block (g, m) takes marker m's state --
+// with `fused`, derived from the previous iteration by the block itself
+// (every block of the marker derives the same one; block 0 stores it),
+// otherwise as k_refine_pick left it -- then evaluates grid point g:
+// homography, cut signals in LDS, out-of-bounds flags, Gram matrix and the
+// residual under its own flags.  Buffers alternate with the iteration's
+// parity.  Fusing saves a launch per iteration when few markers leave the
+// chip idle; with many markers the repeated selection costs more than the
+// launch.
 __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
-    const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, RefineWork W) {
+    const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, int fused,
+    RefineWork W) {
   __shared__ float sH[9], sHi[9];
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
@@ -534,9 +540,14 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x, n_jobs = gridDim.y;
   const RefineJobDev& J = jobs[m];
   if (it >= J.n_iter) return;
-  state_at(J, it, m, n_jobs, grid_n, W, P, S);
   const int cb = it & 1;
-  if (g == 0 && tid == 0) W.state[(size_t)cb * n_jobs + m] = S;
+  if (fused) {
+    state_at(J, it, m, n_jobs, grid_n, W, P, S);
+    if (g == 0 && tid == 0) W.state[(size_t)cb * n_jobs + m] = S;
+  } else {
+    if (tid == 0) S = W.state[(size_t)cb * n_jobs + m];
+    __syncthreads();
+  }
   if (!S.ok) return;
   const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
@@ -593,6 +604,28 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
       GD.m_local = mv;
     }
   }
+}
+
+// The state a marker starts iteration it + 1 with (after k_refine_iter of
+// iteration it), one block per marker; k_refine_init gives iteration 0's.
+__global__ __launch_bounds__(kRefineBlock) void k_refine_pick(const RefineJobDev* __restrict__ jobs,
+                                                              int it, int grid_n, RefineWork W) {
+  __shared__ PickShared P;
+  __shared__ RefineStateDev S;
+  const int m = blockIdx.x, n_jobs = gridDim.x;
+  const RefineJobDev& J = jobs[m];
+  if (it + 1 > J.n_iter) return;
+  state_at(J, it + 1, m, n_jobs, grid_n, W, P, S);
+  if (threadIdx.x == 0) W.state[(size_t)((it + 1) & 1) * n_jobs + m] = S;
+}
+
+__global__ __launch_bounds__(kRefineBlock) void k_refine_init(const RefineJobDev* __restrict__ jobs,
+                                                              int grid_n, RefineWork W) {
+  __shared__ PickShared P;
+  __shared__ RefineStateDev S;
+  const int m = blockIdx.x;
+  state_at(jobs[m], 0, m, gridDim.x, grid_n, W, P, S);
+  if (threadIdx.x == 0) W.state[m] = S;
 }
 
 // Signals at the last iteration's best grid point, then the per-cut
@@ -728,8 +761,13 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
                   RefineOutDev* out, hipStream_t s) {
   if (n_jobs <= 0) return 0;
   const dim3 grid(grid_n * grid_n, n_jobs);
-  for (int it = 0; it < max_iter; ++it)
-    k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, work);
+  const int fused = grid.x * grid.y <= kFuseMaxBlocks;
+  if (!fused) k_refine_init<<<n_jobs, kRefineBlock, 0, s>>>(jobs, grid_n, work);
+  for (int it = 0; it < max_iter; ++it) {
+    k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, it, n_samples, grid_n, fused, work);
+    if (!fused && it + 1 < max_iter)
+      k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work);
+  }
   k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
                                                  bank_cols, work, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;

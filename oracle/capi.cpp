@@ -134,8 +134,9 @@ oracle::Params front_params(const cctagx_params& c) {
 extern "C" {
 
 // cctagDetection on the CPU; returns the marker count or -1.  `threads` > 1
-// runs the shared host candidate stage on a pool of that many threads (the
-// product's configuration), 1 serially.
+// runs the shared host candidate stage and the per-candidate identification
+// on a pool of that many threads (the reference's tbb::parallel_for loops),
+// 1 serially; the front end is serial either way.
 int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w, int h,
                           cctagx_marker* out, int max_out, int threads) {
   try {
@@ -146,6 +147,7 @@ int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w
     std::vector<cctagx::Marker> markers;
     std::unique_ptr<cctagx::TaskPool> pool;
     if (threads > 1) pool = std::make_unique<cctagx::TaskPool>(threads);
+    backend.set_pool(pool.get());
     cctagx::detect_markers(level_edges(levels), cctagx::host_params(*params), bank, &backend,
                            markers, pool.get());
     for (size_t i = 0; i < markers.size() && (int)i < max_out; ++i) cctagx::to_c_marker(markers[i], out[i]);
@@ -160,11 +162,14 @@ int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
   return oracle_detect_threads(params, img, w, h, out, max_out, 1);
 }
 
-// Wall time of the full CPU detection (front end + candidates + identification).
-double oracle_time_detect(const cctagx_params* params, const uint8_t* img, int w, int h, int iters) {
+// Wall time of the full CPU detection (front end + candidates + identification)
+// on `threads` threads.
+double oracle_time_detect(const cctagx_params* params, const uint8_t* img, int w, int h, int iters,
+                          int threads) {
   std::vector<cctagx_marker> out(256);
   auto t0 = std::chrono::steady_clock::now();
-  for (int i = 0; i < iters; ++i) oracle_detect(params, img, w, h, out.data(), (int)out.size());
+  for (int i = 0; i < iters; ++i)
+    oracle_detect_threads(params, img, w, h, out.data(), (int)out.size(), threads);
   auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count() / iters;
 }

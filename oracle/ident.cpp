@@ -271,7 +271,7 @@ void CpuIdentBackend::select_cuts(const std::vector<cctagx::CutJob>& jobs,
                                   std::vector<cctagx::CutSelection>& out) {
   const cctagx::IdentConstants k = cctagx::ident_constants(p, bank_);
   out.assign(jobs.size(), cctagx::CutSelection());
-  for (size_t jb = 0; jb < jobs.size(); ++jb) {
+  cctagx::parallel_for(pool_, (int)jobs.size(), [&](int jb) {
     const cctagx::CutJob& job = jobs[jb];
     std::vector<Cut> cuts;
     for (const Pt2& op : job.outer_points) {  // collectCuts, Identification.cpp:471-496
@@ -285,7 +285,7 @@ void CpuIdentBackend::select_cuts(const std::vector<cctagx::CutJob>& jobs,
       if (!c.oob) cuts.push_back(c);
     }
     out[jb].n_collected = (int)cuts.size();
-    if (cuts.empty()) continue;
+    if (cuts.empty()) return;
     // selectCutCheapUniform, Identification.cpp:546-601
     const size_t selectSize = std::min((size_t)p.num_cuts_in_ident_step, cuts.size());
     std::vector<float> var;
@@ -308,7 +308,7 @@ void CpuIdentBackend::select_cuts(const std::vector<cctagx::CutJob>& jobs,
       if (at < ind.size() && out[jb].stops.size() < selectSize) out[jb].stops.push_back(cuts[ind[at]].stop);
       else break;
     }
-  }
+  });
 }
 
 void CpuIdentBackend::refine(const std::vector<cctagx::RefineJob>& jobs, const cctagx::HostParams& p,
@@ -316,7 +316,7 @@ void CpuIdentBackend::refine(const std::vector<cctagx::RefineJob>& jobs, const c
   const cctagx::IdentConstants k = cctagx::ident_constants(p, bank_);
   out.assign(jobs.size(), cctagx::RefineResult());
   const size_t G = (size_t)p.imaged_center_n_grid_sample;
-  for (size_t jb = 0; jb < jobs.size(); ++jb) {
+  cctagx::parallel_for(pool_, (int)jobs.size(), [&](int jb) {
     const cctagx::RefineJob& r = jobs[jb];
     cctagx::RefineResult& res = out[jb];
     std::vector<Cut> cuts(r.stops.size());
@@ -372,7 +372,7 @@ void CpuIdentBackend::refine(const std::vector<cctagx::RefineJob>& jobs, const c
       neighbour /= (float)((G - 1) / 2);
     }
     res.ok = ok;
-    if (!ok) continue;
+    if (!ok) return;
     res.center = center;
     res.H = H;
     res.min_res = minRes;
@@ -386,7 +386,7 @@ void CpuIdentBackend::refine(const std::vector<cctagx::RefineJob>& jobs, const c
       std::copy(cuts[i].sig.begin(), cuts[i].sig.end(), res.signals.begin() + i * k.n_samples);
       if (!cuts[i].oob) orazio_distances(cuts[i], k, &res.id_dist[i * k.bank_rows]);
     }
-  }
+  });
 }
 
 }  // namespace oracle

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../cctag_amd/csrc/host/ident.hpp"
+#include "../cctag_amd/csrc/host/pool.hpp"
 
 namespace oracle {
 
@@ -17,8 +18,12 @@ class CpuIdentBackend : public cctagx::IdentBackend {
                    std::vector<cctagx::CutSelection>& out) override;
   void refine(const std::vector<cctagx::RefineJob>& jobs, const cctagx::HostParams& p,
               std::vector<cctagx::RefineResult>& out) override;
+  // Candidates run in parallel on `pool` (null: serially), as the reference
+  // identifies them under tbb::parallel_for.
+  void set_pool(cctagx::TaskPool* pool) { pool_ = pool; }
 
  private:
+  cctagx::TaskPool* pool_ = nullptr;
   const uint8_t* img_;
   int w_, h_;
   std::vector<std::vector<float>> bank_;

@@ -46,7 +46,7 @@ def load():
                                    ctypes.c_double, ctypes.c_double, vp, ctypes.POINTER(ctypes.c_int)]
     o.oracle_grad_cost.restype = ctypes.c_double
     o.oracle_time_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
-                                     ctypes.c_int, ctypes.c_int]
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
     _o = o
     return o
@@ -140,12 +140,13 @@ def detect(img: np.ndarray, params=None, threads: int = 1) -> list:
     return [buf[i].as_dict() for i in range(min(n, 512))]
 
 
-def time_detect(img: np.ndarray, iters: int = 1, params=None) -> float:
+def time_detect(img: np.ndarray, iters: int = 1, params=None, threads: int = 1) -> float:
+    """Seconds per full CPU detection, averaged over `iters`."""
     o = load()
     img = np.ascontiguousarray(img, dtype=np.uint8)
     h, w = img.shape
     prm = params if params is not None else _lib.default_params(3)
-    return o.oracle_time_detect(ctypes.byref(prm), img.ctypes.data, w, h, iters)
+    return o.oracle_time_detect(ctypes.byref(prm), img.ctypes.data, w, h, iters, threads)
 
 
 def grad_cost(img: np.ndarray, H, stops, n_samples: int = 100, begin: float = 0.25,
