@@ -367,7 +367,9 @@ static void outlier_removal(const EdgeSet& es, const std::vector<int>& children,
       distance_points_ellipse(dist, pts, q);
       if (weightedType != kNoWeight)
         for (size_t i = 0; i < dist.size(); ++i) dist[i] = dist[i] * weights[i];
-      const float S = median_ref(dist);
+      // most fits lose to the best so far: a counting pass rejects them
+      // with the result the median comparison would give
+      const float S = median_below(dist, Sm) != 0 ? median_ref(dist) : Sm;
       if (S < Sm) {
         counter = 0;
         qm = Ellipse(Q);
@@ -655,9 +657,19 @@ static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::ve
         ++cnt;
         continue;
       }
+      // distances are >= 0, so S1 + S2 >= max(S1, S2) after rounding too:
+      // either median at or above Sm rejects the fit, decided by counting
       distance_points_ellipse(dist, pts, q);
+      if (median_below(dist, Sm) == 0) {
+        ++cnt;
+        continue;
+      }
       const float S1 = median_ref(dist);
       distance_points_ellipse(anotherDist, anotherPts, q);
+      if (median_below(anotherDist, Sm) == 0) {
+        ++cnt;
+        continue;
+      }
       const float S2 = median_ref(anotherDist);
       const float S = S1 + S2;
       if (S < Sm) {

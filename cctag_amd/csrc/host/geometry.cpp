@@ -302,6 +302,18 @@ float median_ref(std::vector<float>& v) {
   return v[v.size() / 2];
 }
 
+int median_below(const std::vector<float>& v, float bound) {
+  // sorted[n/2] < bound exactly when more than n/2 elements are below it
+  size_t below = 0;
+  bool nan = false;
+  for (const float x : v) {
+    below += x < bound;
+    nan |= x != x;
+  }
+  if (nan) return -1;
+  return below > v.size() / 2 ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Eigen-restatements used by the fits.
 namespace {

@@ -92,6 +92,10 @@ void fit_circle(const std::vector<Pt2>& pts, Ellipse& e);
 
 // Statistic.hpp:140-145: v[n/2] after sort.
 float median_ref(std::vector<float>& v);
+// Whether median_ref(v) < bound, decided by one counting pass without
+// reordering v: 1 or 0, or -1 when v holds a NaN (no strict weak order;
+// the caller then takes median_ref itself).
+int median_below(const std::vector<float>& v, float bound);
 
 // PCG32 XSH-RR 64/32 (O'Neill 2014), seeded like pcg32(271828) of
 // Statistic.cpp:19, with pcg_extras::bounded_rand.

@@ -68,6 +68,21 @@ static float compute_median(const std::vector<float>& vec) {
   return (float)((out[s - 1] + out[s - 2]) / 2.0);
 }
 
+void median_profile_range(const std::vector<float>& signals, const std::vector<uint8_t>& oob,
+                          size_t n_samples, float& bar_min, float& bar_max) {
+  // Identification.cpp:920-934
+  std::vector<size_t> correct;
+  for (size_t k = 0; k < oob.size(); ++k)
+    if (!oob[k]) correct.push_back(k);
+  std::vector<float> bar(n_samples), along(correct.size());
+  for (size_t s = 0; s < n_samples; ++s) {
+    for (size_t k = 0; k < correct.size(); ++k) along[k] = signals[correct[k] * n_samples + s];
+    bar[s] = correct.empty() ? 0.f : compute_median(along);
+  }
+  bar_min = *std::min_element(bar.begin(), bar.end());
+  bar_max = *std::max_element(bar.begin(), bar.end());
+}
+
 bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2>& stops,
                      const HostParams& p, const IdentConstants& kc, float mean_ab, RefineJob& r) {
   const float sqrt2 = std::sqrt(2.f);
@@ -239,18 +254,7 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     m.center_img = r.center;
     m.homography = r.H;
     const size_t ncut = rj.stops.size();
-    const size_t ns = r.signals.size() / ncut;
-    std::vector<size_t> correct;
-    for (size_t k = 0; k < ncut; ++k)
-      if (!r.oob[k]) correct.push_back(k);
-    std::vector<float> bar(ns), along(correct.size());
-    for (size_t s = 0; s < ns; ++s) {
-      for (size_t k = 0; k < correct.size(); ++k) along[k] = r.signals[correct[k] * ns + s];
-      bar[s] = compute_median(along);
-    }
-    const float vMin = *std::min_element(bar.begin(), bar.end());
-    const float vMax = *std::max_element(bar.begin(), bar.end());
-    const float magnitude = vMax - vMin;
+    const float magnitude = r.bar_max - r.bar_min;
     const float residual = (float)(std::sqrt((double)r.min_res) / (double)magnitude);
     m.quality = 1.f / residual;
     if (residual > 2.7f) {
@@ -259,15 +263,27 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     }
     // orazioDistanceRobust (Identification.cpp:50-193), serial over cuts.
     std::vector<std::list<float>> vScore(nb);
+    std::vector<float> v(nb);
     for (size_t k = 0; k < ncut; ++k) {
       if (r.oob[k]) return;
-      std::map<float, int> sortedId;
+      // the largest exp(-d) keyed in a std::map<float, int>: the last row
+      // among equal keys; a scan gives the same when no key is NaN
+      bool nan = false;
       for (int idc = 0; idc < nb; ++idc) {
-        const float v = std::exp(-r.id_dist[k * nb + idc]);
-        sortedId[v] = idc;
+        v[idc] = std::exp(-r.id_dist[k * nb + idc]);
+        nan |= v[idc] != v[idc];
       }
-      const auto best = sortedId.rbegin();
-      vScore[best->second].push_back(best->first);
+      if (!nan) {
+        int best = 0;
+        for (int idc = 1; idc < nb; ++idc)
+          if (v[idc] >= v[best]) best = idc;
+        vScore[best].push_back(v[best]);
+      } else {
+        std::map<float, int> sortedId;
+        for (int idc = 0; idc < nb; ++idc) sortedId[v[idc]] = idc;
+        const auto best = sortedId.rbegin();
+        vScore[best->second].push_back(best->first);
+      }
     }
     size_t maxSize = 0;
     int iMax = 0;

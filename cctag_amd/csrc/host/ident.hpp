@@ -69,9 +69,16 @@ struct RefineResult {
   Mat3 H = Mat3::zero();
   float min_res = 0;                 // residual of the last grid search
   std::vector<uint8_t> oob;          // per cut, sticky out-of-bounds flag
-  std::vector<float> signals;        // n_cuts x n_samples at the optimum
+  // range of the median profile of the valid cuts' signals at the optimum
+  // (Identification.cpp:920-934)
+  float bar_min = 0, bar_max = 0;
   std::vector<float> id_dist;        // n_cuts x bank rows: orazio distance, valid cuts only
 };
+
+// bar_min / bar_max of a RefineResult from the n_cuts x n_samples signals at
+// the optimum and the cuts' out-of-bounds flags.
+void median_profile_range(const std::vector<float>& signals, const std::vector<uint8_t>& oob,
+                          size_t n_samples, float& bar_min, float& bar_max);
 
 class IdentBackend {
  public:

@@ -23,6 +23,8 @@ int GpuIdentBackend::init(const std::vector<std::vector<float>>& bank, const Hos
                           hipStream_t s) {
   stream_ = s;
   k_ = ident_constants(p, bank);
+  if (k_.bank_rows > kMaxBankRows || k_.bank_rows > kMaxBank || k_.bank_cols > kMaxRatios)
+    throw std::runtime_error("marker bank larger than the identification kernels accept");
   for (hipEvent_t& e : ev_)
     if (!e) CCTAGX_HIP(hipEventCreate(&e));
   ensure(64, 64 * 160);
@@ -40,6 +42,7 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   const size_t o_so = dev_.reserve(sizeof(SelectOutDev) * nm);
   const size_t o_rj = dev_.reserve(sizeof(RefineJobDev) * nm);
   const size_t o_ro = dev_.reserve(sizeof(RefineOutDev) * nm);
+  const size_t o_rd = dev_.reserve(sizeof(float) * (size_t)nm * kMaxCuts * kMaxBankRows);
   const size_t o_st = dev_.reserve(sizeof(RefineStateDev) * 2 * nm);
   const size_t o_gr = dev_.reserve(sizeof(RefineGridDev) * 2 * (size_t)nm * kMaxGrid * kMaxGrid);
   const size_t o_gm =
@@ -51,6 +54,7 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   d_sel_out_ = dev_.at<SelectOutDev>(o_so);
   d_ref_jobs_ = dev_.at<RefineJobDev>(o_rj);
   d_ref_out_ = dev_.at<RefineOutDev>(o_ro);
+  d_ref_dist_ = dev_.at<float>(o_rd);
   d_work_.state = dev_.at<RefineStateDev>(o_st);
   d_work_.grid = dev_.at<RefineGridDev>(o_gr);
   d_work_.gram = dev_.at<double>(o_gm);
@@ -60,7 +64,8 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
             "hipMemcpy bank");
   if (h_sel_jobs_.ensure(sizeof(SelectJobDev) * nm) || h_pts_.ensure(sizeof(float4) * np) ||
       h_sel_out_.ensure(sizeof(SelectOutDev) * nm) || h_ref_jobs_.ensure(sizeof(RefineJobDev) * nm) ||
-      h_ref_out_.ensure(sizeof(RefineOutDev) * nm))
+      h_ref_out_.ensure(sizeof(RefineOutDev) * nm) ||
+      h_ref_dist_.ensure(sizeof(float) * (size_t)nm * kMaxCuts * kMaxBankRows))
     throw std::runtime_error(std::string("hipHostMalloc: ") + last_error());
   cap_markers_ = nm;
   cap_points_ = np;
@@ -127,6 +132,8 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   for (const RefineJob& r : jobs)
     refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
   RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
+  const int nb = k_.bank_rows;
+  int n_dist = 0;
   for (int i = 0; i < n; ++i) {
     const RefineJob& r = jobs[i];
     RefineJobDev& d = hj[i];
@@ -149,6 +156,8 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
     d.n_cuts = (int)r.stops.size();
     d.c0x = r.center0.x;
     d.c0y = r.center0.y;
+    d.dist_offset = n_dist;
+    n_dist += d.n_cuts * nb;
     for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
   }
   hipStream_t s = stream_;
@@ -157,15 +166,19 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   int max_iter = 0;
   for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
   if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
-                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_, d_ref_out_, s) != 0)
+                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_, d_ref_out_, d_ref_dist_,
+                    s) != 0)
     throw std::runtime_error("k_refine launch failed");
   hipEventRecord(ev_[3], s);
   RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
+  float* hd = h_ref_dist_.as<float>();
   hip_check(hipMemcpyAsync(ho, d_ref_out_, sizeof(RefineOutDev) * n, hipMemcpyDeviceToHost, s), "D2H");
+  if (n_dist > 0)
+    hip_check(hipMemcpyAsync(hd, d_ref_dist_, sizeof(float) * n_dist, hipMemcpyDeviceToHost, s),
+              "D2H");
   hip_check(hipStreamSynchronize(s), "sync refine");
   hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
   refine_us_ *= 1000.f;
-  const int ns = k_.n_samples, nb = k_.bank_rows;
   for (int i = 0; i < n; ++i) {
     const RefineOutDev& o = ho[i];
     RefineResult& r = out[i];
@@ -175,14 +188,14 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) r.H.m[a][b] = o.H[a * 3 + b];
     r.min_res = o.min_res;
+    r.bar_min = o.bar_min;
+    r.bar_max = o.bar_max;
     const int nc = (int)jobs[i].stops.size();
     r.oob.assign(o.oob, o.oob + nc);
-    r.signals.resize((size_t)nc * ns);
     r.id_dist.assign((size_t)nc * nb, 0.f);
-    for (int k = 0; k < nc; ++k) {
-      for (int ii = 0; ii < ns; ++ii) r.signals[(size_t)k * ns + ii] = o.signals[k * kMaxSamples + ii];
-      for (int c = 0; c < nb; ++c) r.id_dist[(size_t)k * nb + c] = o.id_dist[k * kMaxBankRows + c];
-    }
+    const float* src = hd + hj[i].dist_offset;
+    for (int k = 0; k < nc; ++k)
+      if (!r.oob[k]) std::copy(src + (size_t)k * nb, src + (size_t)(k + 1) * nb, &r.id_dist[(size_t)k * nb]);
   }
 }
 

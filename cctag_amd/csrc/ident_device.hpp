@@ -37,6 +37,7 @@ struct RefineJobDev {
   float grid_scale, neighbour0, begin_sig, end_sig;
   int n_iter, n_cuts;
   float c0x, c0y;
+  int dist_offset;  // first float of its distances in the packed output
   float2 stops[kMaxCuts];
 };
 
@@ -45,9 +46,8 @@ struct RefineOutDev {
   float cx, cy;
   float H[9];
   float min_res;
+  float bar_min, bar_max;  // range of the per-sample median over the valid cuts
   uint8_t oob[kMaxCuts];
-  float signals[kMaxCuts * kMaxSamples];
-  float id_dist[kMaxCuts * kMaxBankRows];
 };
 
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
@@ -76,9 +76,11 @@ struct RefineWork {
   double* gram;           // 2 x n_jobs x kMaxGrid^2 x kMaxPairs
 };
 
+// out: one header per job; dist: per job n_cuts x bank_rows distances from
+// its dist_offset on (invalid cuts' rows are not written).
 int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
-                  RefineOutDev* out, hipStream_t s);
+                  RefineOutDev* out, float* dist, hipStream_t s);
 
 // A device image the identification jobs sample.
 struct DeviceImage {
@@ -120,12 +122,13 @@ class GpuIdentBackend : public IdentBackend {
   IdentConstants k_{};
   int cap_markers_ = 0, cap_points_ = 0;
   DeviceArena dev_;
-  PinnedBuffer h_sel_jobs_, h_pts_, h_sel_out_, h_ref_jobs_, h_ref_out_;
+  PinnedBuffer h_sel_jobs_, h_pts_, h_sel_out_, h_ref_jobs_, h_ref_out_, h_ref_dist_;
   SelectJobDev* d_sel_jobs_ = nullptr;
   float4* d_pts_ = nullptr;
   SelectOutDev* d_sel_out_ = nullptr;
   RefineJobDev* d_ref_jobs_ = nullptr;
   RefineOutDev* d_ref_out_ = nullptr;
+  float* d_ref_dist_ = nullptr;
   RefineWork d_work_{};
   float* d_inv_ratio_ = nullptr;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};

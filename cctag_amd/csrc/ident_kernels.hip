@@ -38,6 +38,16 @@ struct Img {
   }
 };
 
+// Img::bilinear with the four corner pixels already loaded.
+__device__ __forceinline__ float bilinear_of(float x, float y, uint8_t a, uint8_t b, uint8_t c,
+                                             uint8_t d) {
+  const int px = (int)x, py = (int)y;
+  const float fx = x - px, fy = y - py;
+  const float fx1 = 1.0f - fx, fy1 = 1.0f - fy;
+  const float w1 = fx1 * fy1, w2 = fx * fy1, w3 = fx1 * fy, w4 = fx * fy;
+  return ((float)a * w1 + (float)b * w2 + (float)c * w3 + (float)d * w4) / 2;
+}
+
 __device__ __forceinline__ void apply_h(float& xr, float& yr, const float* H, float x, float y) {
   const float u = H[0] * x + H[1] * y + H[2];
   const float v = H[3] * x + H[4] * y + H[5];
@@ -290,6 +300,8 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 // iteration costs one launch.
 constexpr int kRefineBlock = 256;
 constexpr int kRefineChunk = 10;  // samples per sampling work item
+constexpr int kLoadBatch = 5;     // samples whose image reads are issued together
+static_assert(kRefineChunk % kLoadBatch == 0, "whole load batches per chunk");
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
 constexpr int kFuseMaxBlocks = 1024;  // grid blocks up to which the selection is fused
 
@@ -357,17 +369,37 @@ __device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, flo
   }
   const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
   bool oob = false;
+  // per group of kLoadBatch samples: positions first, then every corner load
+  // (an outside sample reads pixel (0, 0), so the loads need no branch and
+  // are in flight together), then the values
 #pragma unroll
-  for (int t = 0; t < kRefineChunk; ++t) {
-    float xr, yr;
-    apply_h(xr, yr, H, x, y);
-    const bool live = i0 + t < n_samples;
-    const bool in = live && xr >= 0.f && xr < xmax && yr >= 0.f && yr < ymax;
-    oob |= live && !in;
-    const float v = in ? im.bilinear(xr, yr) : 0.f;
-    if (live) out[i0 + t] = v;
-    x += stepX;
-    y += stepY;
+  for (int t0 = 0; t0 < kRefineChunk; t0 += kLoadBatch) {
+    float xr[kLoadBatch], yr[kLoadBatch];
+    bool in[kLoadBatch];
+#pragma unroll
+    for (int t = 0; t < kLoadBatch; ++t) {
+      apply_h(xr[t], yr[t], H, x, y);
+      const bool live = i0 + t0 + t < n_samples;
+      in[t] = live && xr[t] >= 0.f && xr[t] < xmax && yr[t] >= 0.f && yr[t] < ymax;
+      oob |= live && !in[t];
+      x += stepX;
+      y += stepY;
+    }
+    uint8_t p00[kLoadBatch], p01[kLoadBatch], p10[kLoadBatch], p11[kLoadBatch];
+#pragma unroll
+    for (int t = 0; t < kLoadBatch; ++t) {
+      const int px = in[t] ? (int)xr[t] : 0, py = in[t] ? (int)yr[t] : 0;
+      const uint8_t* p0 = im.p + px + py * im.w;
+      p00[t] = p0[0];
+      p01[t] = p0[1];
+      p10[t] = p0[im.w];
+      p11[t] = p0[im.w + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < kLoadBatch; ++t) {
+      if (i0 + t0 + t >= n_samples) continue;
+      out[i0 + t0 + t] = in[t] ? bilinear_of(xr[t], yr[t], p00[t], p01[t], p10[t], p11[t]) : 0.f;
+    }
   }
   return oob;
 }
@@ -405,9 +437,11 @@ struct PickShared {
   uint8_t valid[G2MAX][kMaxCuts];
   uint8_t sticky_prev[kMaxCuts];
   int mvalid[G2MAX];
+  int same[G2MAX];
   float res[G2MAX];
   double cost_local[G2MAX];
   int m_local[G2MAX];
+  double part[G2MAX][64];  // row shares of the points that are recomputed
 };
 
 __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateDev& prev,
@@ -437,27 +471,37 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
   __syncthreads();
   if (tid < G2) {
     int cnt = 0;
-    for (int k = 0; k < nc; ++k) cnt += P.valid[tid][k];
+    bool same = true;
+    for (int k = 0; k < nc; ++k) {
+      cnt += P.valid[tid][k];
+      same = same && (P.valid[tid][k] == (uint8_t)!(P.sticky_prev[k] | P.oobl[tid][k]));
+    }
     P.mvalid[tid] = cnt;
+    P.same[tid] = same && cnt == P.m_local[tid];
   }
   __syncthreads();
-  const int wid = tid >> 6, lane = tid & 63;
-  for (int gg = wid; gg < G2; gg += kRefineBlock / 64) {
-    bool same = P.mvalid[gg] == P.m_local[gg];
-    for (int k = 0; k < nc; ++k)
-      same = same && (P.valid[gg][k] == (uint8_t)!(P.sticky_prev[k] | P.oobl[gg][k]));
-    double acc;
-    if (same) {
-      acc = P.cost_local[gg];
-    } else {
-      const double* Cg = gram_m + (size_t)gg * kMaxPairs;
-      const double part = lane < nc ? gram_row(Cg, P.valid[gg], lane, nc, P.mvalid[gg]) : 0.0;
-      acc = __shfl(wave_sum(part), 0, 64);
+  // points whose valid cuts differ from the ones their block assumed: row
+  // shares from the Gram matrices, all rows at once, then the wavefront
+  // reduction tree of wave_sum replayed in LDS (same additions, same order)
+  for (int q = tid; q < G2 * 64; q += kRefineBlock) {
+    const int gg = q >> 6, k = q & 63;
+    if (P.same[gg]) continue;
+    P.part[gg][k] = k < nc ? gram_row(gram_m + (size_t)gg * kMaxPairs, P.valid[gg], k, nc,
+                                      P.mvalid[gg])
+                           : 0.0;
+  }
+  __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int q = tid; q < G2 * o; q += kRefineBlock) {
+      const int gg = q / o, k = q - gg * o;
+      if (!P.same[gg]) P.part[gg][k] += P.part[gg][k + o];
     }
-    if (lane == 0) {
-      const int mv = P.mvalid[gg];
-      P.res[gg] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
-    }
+    __syncthreads();
+  }
+  if (tid < G2) {
+    const double acc = P.same[tid] ? P.cost_local[tid] : P.part[tid][0];
+    const int mv = P.mvalid[tid];
+    P.res[tid] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
   }
   __syncthreads();
   if (tid == 0) {
@@ -530,12 +574,17 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, int fused,
     RefineWork W) {
   __shared__ float sH[9], sHi[9];
-  __shared__ float sig[kMaxCuts][kMaxSamples + 1];
+  // the selection's scratch is dead once the sampling starts
+  __shared__ union {
+    PickShared P;
+    float sig[kMaxCuts][kMaxSamples + 1];
+  } U;
   __shared__ int oobk[kMaxCuts];
   __shared__ double sC[kMaxPairs];
   __shared__ uint8_t lvalid[kMaxCuts];
-  __shared__ PickShared P;
   __shared__ RefineStateDev S;
+  PickShared& P = U.P;
+  auto& sig = U.sig;
 
   const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x, n_jobs = gridDim.y;
   const RefineJobDev& J = jobs[m];
@@ -634,14 +683,26 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_init(const RefineJobDev
 __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
     const float* __restrict__ inv_ratio, int bank_rows, int bank_cols, RefineWork W,
-    RefineOutDev* __restrict__ out) {
+    RefineOutDev* __restrict__ out, float* __restrict__ dist) {
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
+  __shared__ float ordered[kMaxSamples][kMaxCuts];  // per sample, the valid cuts' values sorted
+  __shared__ float bar[kMaxSamples];
+  __shared__ int valid_cut[kMaxCuts];
+  __shared__ int n_valid;
   __shared__ float sH[9], sHi[9];
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
   __shared__ float xs[kMaxSamples];
+  __shared__ float ratio[kMaxBankRows * kMaxRatios];
   __shared__ uint8_t parity[kMaxBankRows][kMaxSamples];
-  __shared__ PickShared P;
+  // a sample's term of the distance, for each ring parity; the selection's
+  // scratch is dead by then
+  __shared__ union {
+    PickShared P;
+    float term[2][kMaxCuts][kMaxSamples + 1];
+  } U;
   __shared__ RefineStateDev S;
+  PickShared& P = U.P;
+  auto& term = U.term;
   const int m = blockIdx.x, tid = threadIdx.x, n_jobs = gridDim.x;
   const RefineJobDev& J = jobs[m];
   RefineOutDev& O = out[m];
@@ -649,6 +710,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     if (tid == 0) O.ok = 0;
     return;
   }
+  for (int q = tid; q < bank_rows * kMaxRatios; q += kRefineBlock) ratio[q] = inv_ratio[q];
   // the selection after the last iteration
   state_at(J, J.n_iter, m, n_jobs, grid_n, W, P, S);
   if (!S.ok) {
@@ -677,9 +739,11 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     O.min_res = S.min_res;
   }
   if (tid < nc) O.oob[tid] = S.sticky[tid];
-  for (int q = tid; q < nc * n_samples; q += kRefineBlock) {
-    const int k = q / n_samples, ii = q - k * n_samples;
-    O.signals[k * kMaxSamples + ii] = sig[k][ii];
+  if (tid == 0) {
+    int mv = 0;
+    for (int k = 0; k < nc; ++k)
+      if (!S.sticky[k]) valid_cut[mv++] = k;
+    n_valid = mv;
   }
   // orazioDistanceRobust statistics per valid cut (Identification.cpp:88-121)
   if (tid < nc && !S.sticky[tid]) {
@@ -705,10 +769,8 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     stat_var[tid] = mv.var;
     stat_muw[tid] = sumSup / (float)nSup;
     stat_mub[tid] = sumInf / (float)nInf;
-  }
-  __syncthreads();
-  // ring parity of every bank profile at every sample position x_i
-  if (tid == 0) {
+  } else if (tid == kRefineBlock - 64) {
+    // sample positions x_i along the profile (another wavefront)
     const float stepX = (J.end_sig - J.begin_sig) / ((float)n_samples - 1.f);
     float x = J.begin_sig;
     for (int i = 0; i < n_samples; ++i) {
@@ -717,33 +779,78 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     }
   }
   __syncthreads();
+  // the median profile over the valid cuts (Identification.cpp:920-934):
+  // each value lands at its rank (ties broken by cut order), then the middle
+  // order statistic, or the two middle ones summed in float and halved in
+  // double, as the host's partial_sort_copy median does
+  const int mv = n_valid;
+  for (int q = tid; q < n_samples * mv; q += kRefineBlock) {
+    const int i = q / mv, j = q - i * mv;
+    const float v = sig[valid_cut[j]][i];
+    int rank = 0;
+    for (int l = 0; l < mv; ++l) {
+      const float u = sig[valid_cut[l]][i];
+      rank += (u < v) || (u == v && l < j);
+    }
+    ordered[i][rank] = v;
+  }
+  __syncthreads();
+  for (int i = tid; i < n_samples; i += kRefineBlock) {
+    float b = 0.f;
+    if (mv % 2) {
+      b = ordered[i][mv / 2];
+    } else if (mv > 0) {
+      const float sum = ordered[i][mv / 2] + ordered[i][mv / 2 - 1];
+      b = (float)((double)sum / 2.0);
+    }
+    bar[i] = b;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (int i = tid; i < n_samples; i += 64) {
+      lo = fminf(lo, bar[i]);
+      hi = fmaxf(hi, bar[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = fminf(lo, __shfl_xor(lo, o, 64));
+      hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if (tid == 0) {
+      O.bar_min = lo;
+      O.bar_max = hi;
+    }
+  }
+  // ring parity of every bank profile at every sample position, and each
+  // sample's term under either parity (:124-160): the division is done once
+  // per sample instead of once per sample and bank row
   for (int q = tid; q < bank_rows * n_samples; q += kRefineBlock) {
     const int idc = q / n_samples, i = q - idc * n_samples;
     int ldum = 0;
     for (int j = 0; j < bank_cols; ++j)
-      if (inv_ratio[idc * kMaxRatios + j] <= xs[i]) ++ldum;
+      if (ratio[idc * kMaxRatios + j] <= xs[i]) ++ldum;
     parity[idc][i] = (uint8_t)(ldum & 1);
   }
+  for (int q = tid; q < nc * n_samples; q += kRefineBlock) {
+    const int k = q / n_samples, i = q - k * n_samples;
+    if (S.sticky[k]) continue;
+    const float v = sig[k][i], v2 = 2.0f * stat_var[k];
+    const float tb = fmaxf(v - stat_mub[k], 0.f);
+    const float tw = fminf(v - stat_muw[k], 0.f);
+    term[1][k][i] = tb * tb / v2;
+    term[0][k][i] = tw * tw / v2;
+  }
   __syncthreads();
-  // distance of each valid cut to each bank profile (:124-160)
+  // distance of each valid cut to each bank profile, summed in sample order
   for (int q = tid; q < nc * bank_rows; q += kRefineBlock) {
     const int k = q / bank_rows, idc = q - k * bank_rows;
     if (S.sticky[k]) continue;
-    const float* sg = sig[k];
-    const float mub = stat_mub[k], muw = stat_muw[k], v2 = 2.0f * stat_var[k];
+    const uint8_t* par = parity[idc];
+    const float* t0 = term[0][k];
+    const float* t1 = term[1][k];
     float distance = 0;
-    for (int i = 0; i < n_samples; ++i) {
-      float dd;
-      if (parity[idc][i]) {
-        const float t = fmaxf(sg[i] - mub, 0.f);
-        dd = t * t / v2;
-      } else {
-        const float t = fminf(sg[i] - muw, 0.f);
-        dd = t * t / v2;
-      }
-      distance += dd;
-    }
-    O.id_dist[k * kMaxBankRows + idc] = distance;
+    for (int i = 0; i < n_samples; ++i) distance += par[i] ? t1[i] : t0[i];
+    dist[J.dist_offset + k * bank_rows + idc] = distance;
   }
 }
 
@@ -758,7 +865,7 @@ int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, 
 
 int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
-                  RefineOutDev* out, hipStream_t s) {
+                  RefineOutDev* out, float* dist, hipStream_t s) {
   if (n_jobs <= 0) return 0;
   const dim3 grid(grid_n * grid_n, n_jobs);
   const int fused = grid.x * grid.y <= kFuseMaxBlocks;
@@ -769,7 +876,7 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
       k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work);
   }
   k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
-                                                 bank_cols, work, out);
+                                                 bank_cols, work, out, dist);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -379,13 +379,14 @@ void CpuIdentBackend::refine(const std::vector<cctagx::RefineJob>& jobs, const c
     const Mat3 Hinv = cctagx::inverse3(H);  // getSignals at the optimum
     for (Cut& c : cuts) extract_signal(c, img_, w_, h_, H, Hinv);
     res.oob.resize(cuts.size());
-    res.signals.resize(cuts.size() * k.n_samples);
+    std::vector<float> signals(cuts.size() * k.n_samples);
     res.id_dist.assign(cuts.size() * k.bank_rows, 0.f);
     for (size_t i = 0; i < cuts.size(); ++i) {
       res.oob[i] = cuts[i].oob;
-      std::copy(cuts[i].sig.begin(), cuts[i].sig.end(), res.signals.begin() + i * k.n_samples);
+      std::copy(cuts[i].sig.begin(), cuts[i].sig.end(), signals.begin() + i * k.n_samples);
       if (!cuts[i].oob) orazio_distances(cuts[i], k, &res.id_dist[i * k.bank_rows]);
     }
+    cctagx::median_profile_range(signals, res.oob, (size_t)k.n_samples, res.bar_min, res.bar_max);
   });
 }
 
