@@ -37,6 +37,10 @@ struct PlaneCounts {
   int32_t pad;
 };
 
+// Blocks per plane of the two-pass scans over edge points (voter offsets,
+// seed compaction); each block scans one contiguous range.
+constexpr int kScanBlocks = 64;
+
 // Tile geometry shared by host launch code and kernels.
 constexpr int kFilterTileW = 64, kFilterTileH = 32;
 // Filter tile heights the launch can choose from (64 wide, 256 threads: a
@@ -85,6 +89,7 @@ struct FrontEndBuffers {
   int32_t* ismax;
   int32_t* seedmark;   // voter index -> point that reached min votes with it
   int32_t* seeds;
+  int32_t* scan_part;  // nplanes x kScanBlocks partial sums of the two-pass scans
   PlaneCounts* counts; // nplanes
 };
 

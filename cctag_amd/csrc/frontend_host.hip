@@ -97,6 +97,7 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t o_ismax = arena_.reserve(4 * N);
   const size_t o_seedmark = arena_.reserve(4 * N);
   const size_t o_seeds = arena_.reserve(4 * N);
+  const size_t o_part = arena_.reserve(sizeof(int32_t) * NP * kScanBlocks);
   const size_t o_counts = arena_.reserve(sizeof(PlaneCounts) * NP);
   const size_t o_planes = arena_.reserve(sizeof(PlaneDesc) * NP);
   size_t o_fstart[kFilterVariants];
@@ -129,6 +130,7 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   b.ismax = arena_.at<int32_t>(o_ismax);
   b.seedmark = arena_.at<int32_t>(o_seedmark);
   b.seeds = arena_.at<int32_t>(o_seeds);
+  b.scan_part = arena_.at<int32_t>(o_part);
   b.counts = arena_.at<PlaneCounts>(o_counts);
   PlaneDesc* dp = arena_.at<PlaneDesc>(o_planes);
   CCTAGX_HIP(hipMemcpy(dp, ps.planes.data(), sizeof(PlaneDesc) * NP, hipMemcpyHostToDevice));

@@ -835,46 +835,100 @@ __global__ __launch_bounds__(256) void k_vote(const PlaneDesc* __restrict__ P,
 // filled with atomics, then each list sorted ascending (the CPU pushes voters
 // in index order), flow length accumulated in that order, and the point's
 // seed key (the voter that brought it to min votes) recorded.
-__global__ __launch_bounds__(1024) void k_voters_scan(const PlaneDesc* __restrict__ P,
-                                                       PlaneCounts* __restrict__ counts,
-                                                       const int32_t* __restrict__ vcount,
-                                                       int32_t* __restrict__ vindex,
-                                                       int32_t* __restrict__ cursor) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
-  const int p = blockIdx.x;
+// Two-pass scans over a plane's edge points: kScanBlocks blocks per plane,
+// block b owning one contiguous range of whole 1024-point steps.  Pass one
+// stores each block's sum, pass two adds the sums of the blocks before it
+// and scans its range.
+constexpr int kScanStep = 1024;  // threads of a scan block = points per step
+
+__device__ __forceinline__ void scan_range(int n, int& lo, int& hi) {
+  const int per = (n + kScanBlocks - 1) / kScanBlocks;
+  const int chunk = (per + kScanStep - 1) / kScanStep * kScanStep;
+  lo = min(n, (int)blockIdx.x * chunk);
+  hi = min(n, lo + chunk);
+}
+
+// Block-wide inclusive scan of v (1024 threads); *total gets the block sum.
+__device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const int t = __shfl_up(incl, s, 64);
+    if (lane >= s) incl += t;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int wb = 0, tot = 0;
+  for (int w = 0; w < kScanStep / 64; ++w) {
+    if (w < wid) wb += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return wb + incl;
+}
+
+// Sum of the blocks before this one (pass two's offset).
+__device__ __forceinline__ int prior_parts(const int32_t* part) {
+  __shared__ int off;
+  if (threadIdx.x < 64) {
+    int v = (int)threadIdx.x < (int)blockIdx.x ? part[threadIdx.x] : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x == 0) off = v;
+  }
+  __syncthreads();
+  return off;
+}
+
+// a10: voter lists (Types.cpp:66-84), flow length (Vote.cpp:232), seeds
+// (Vote.cpp:236-240).  CSR offsets by a scan of the vote counts; lists
+// filled with atomics, then each list sorted ascending (the CPU pushes voters
+// in index order), flow length accumulated in that order, and the point's
+// seed key (the voter that brought it to min votes) recorded.
+__global__ __launch_bounds__(kScanStep) void k_voters_sum(const PlaneDesc* __restrict__ P,
+                                                          const PlaneCounts* __restrict__ counts,
+                                                          const int32_t* __restrict__ vcount,
+                                                          int32_t* __restrict__ part) {
+  __shared__ int wsum[kScanStep / 64];
+  const int p = blockIdx.y;
+  const int32_t* a = vcount + P[p].pix;
+  int lo, hi;
+  scan_range(counts[p].n_edges, lo, hi);
+  int acc = 0;
+  for (int i = lo + threadIdx.x; i < hi; i += kScanStep) acc += a[i];
+  int tot;
+  block_incl_scan(acc, wsum, &tot);
+  if (threadIdx.x == 0) part[p * kScanBlocks + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanStep) void k_voters_scan(const PlaneDesc* __restrict__ P,
+                                                           PlaneCounts* __restrict__ counts,
+                                                           const int32_t* __restrict__ vcount,
+                                                           const int32_t* __restrict__ part,
+                                                           int32_t* __restrict__ vindex,
+                                                           int32_t* __restrict__ cursor) {
+  __shared__ int wsum[kScanStep / 64];
+  const int p = blockIdx.y;
   const PlaneDesc d = P[p];
   const int n = counts[p].n_edges;
   const int32_t* a = vcount + d.pix;
   int32_t* o = vindex + d.pix + p;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < n; base += 1024) {
+  int lo, hi;
+  scan_range(n, lo, hi);
+  int carry = prior_parts(part + p * kScanBlocks);
+  for (int base = lo; base < hi; base += kScanStep) {
     const int i = base + threadIdx.x;
-    const int v = i < n ? a[i] : 0;
-    int incl = v;
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-      const int t = __shfl_up(incl, s, 64);
-      if (lane >= s) incl += t;
+    const int v = i < hi ? a[i] : 0;
+    int tot;
+    const int incl = block_incl_scan(v, wsum, &tot);
+    if (i < hi) {
+      o[i] = carry + incl - v;
+      cursor[d.pix + i] = carry + incl - v;
     }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    int wb = 0, tot = 0;
-    for (int w = 0; w < 16; ++w) {
-      if (w < wid) wb += wsum[w];
-      tot += wsum[w];
-    }
-    if (i < n) {
-      o[i] = carry + wb + incl - v;
-      cursor[d.pix + i] = carry + wb + incl - v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
-    __syncthreads();
+    carry += tot;
   }
-  if (threadIdx.x == 0) {
+  if (blockIdx.x == kScanBlocks - 1 && threadIdx.x == 0) {
     o[n] = carry;
     counts[p].n_voters = carry;
   }
@@ -896,78 +950,131 @@ __global__ __launch_bounds__(256) void k_voters_fill(const PlaneDesc* __restrict
   }
 }
 
-__global__ __launch_bounds__(256) void k_voters_finish(const PlaneDesc* __restrict__ P,
-                                                        const PlaneCounts* __restrict__ counts,
-                                                        const int32_t* __restrict__ vindex,
-                                                        const float* __restrict__ tdist,
-                                                        int min_votes,
-                                                        int32_t* __restrict__ vlist,
-                                                        float* __restrict__ flow,
-                                                        int32_t* __restrict__ ismax,
-                                                        int32_t* __restrict__ seedmark) {
+// The lists of a block's 256 consecutive points are one contiguous run of
+// the voter array.  When the run fits in LDS: every voter finds its list's
+// owner and its rank there (voter indices are distinct), lands at its place,
+// the sorted run goes back and the voters' distances are gathered in one
+// parallel pass; each thread then folds its own flow length from LDS.
+// Longer runs sort in place.
+constexpr int kFinishBlock = 256;
+constexpr int kFinishStage = 2048;  // voters staged per block
+
+__global__ __launch_bounds__(kFinishBlock) void k_voters_finish(const PlaneDesc* __restrict__ P,
+                                                                const PlaneCounts* __restrict__ counts,
+                                                                const int32_t* __restrict__ vindex,
+                                                                const float* __restrict__ tdist,
+                                                                int min_votes,
+                                                                int32_t* __restrict__ vlist,
+                                                                float* __restrict__ flow,
+                                                                int32_t* __restrict__ ismax,
+                                                                int32_t* __restrict__ seedmark) {
+  __shared__ int32_t stage_in[kFinishStage];
+  __shared__ int32_t stage_out[kFinishStage];
+  __shared__ float dist_s[kFinishStage];
+  __shared__ uint8_t owner[kFinishStage];
+  __shared__ int seg_b[kFinishBlock], seg_e[kFinishBlock];
   const int p = blockIdx.y;
   const PlaneDesc d = P[p];
   const int n = counts[p].n_edges;
   const int32_t* vi = vindex + d.pix + p;
   int32_t* vl = vlist + d.pix;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const int b = vi[c], e = vi[c + 1];
-    // insertion sort of the (short) voter list
-    for (int i = b + 1; i < e; ++i) {
-      const int key = vl[i];
-      int j = i - 1;
-      while (j >= b && vl[j] > key) {
-        vl[j + 1] = vl[j];
-        --j;
+  const float* td = tdist + d.pix;
+  const int t = threadIdx.x;
+  for (int c0 = blockIdx.x * kFinishBlock; c0 < n; c0 += gridDim.x * kFinishBlock) {
+    const int c = c0 + t;
+    const int c_end = min(n, c0 + kFinishBlock);
+    const int run_b = vi[c0], run_e = vi[c_end], len = run_e - run_b;
+    const int b = c < n ? vi[c] : run_e, e = c < n ? vi[c + 1] : run_e;
+    const bool staged = len <= kFinishStage;
+    if (staged) {
+      for (int q = t; q < len; q += kFinishBlock) stage_in[q] = vl[run_b + q];
+      seg_b[t] = b - run_b;
+      seg_e[t] = e - run_b;
+      for (int i = b; i < e; ++i) owner[i - run_b] = (uint8_t)t;
+      __syncthreads();
+      for (int q = t; q < len; q += kFinishBlock) {
+        const int o = owner[q], key = stage_in[q];
+        const int sb = seg_b[o], se = seg_e[o];
+        int rank = 0;
+        for (int j = sb; j < se; ++j) rank += stage_in[j] < key;
+        stage_out[sb + rank] = key;
       }
-      vl[j + 1] = key;
+      __syncthreads();
+      for (int q = t; q < len; q += kFinishBlock) {
+        const int v = stage_out[q];
+        vl[run_b + q] = v;
+        dist_s[q] = td[v];
+      }
+      __syncthreads();
+    } else {
+      // insertion sort in place
+      for (int i = b + 1; i < e; ++i) {
+        const int key = vl[i];
+        int j = i - 1;
+        while (j >= b && vl[j] > key) {
+          vl[j + 1] = vl[j];
+          --j;
+        }
+        vl[j + 1] = key;
+      }
     }
-    float fl = 0.0f;
-    for (int k = 0; k < e - b; ++k)
-      fl = __fdiv_rn(__fadd_rn(__fmul_rn(fl, (float)k), tdist[d.pix + vl[b + k]]), (float)(k + 1));
-    flow[d.pix + c] = fl;
-    const int nv = e - b;
-    ismax[d.pix + c] = nv >= min_votes ? nv : -1;
-    if (nv >= min_votes) seedmark[d.pix + vl[b + min_votes - 1]] = c;
+    if (c < n) {
+      const int nv = e - b;
+      float fl = 0.0f;
+      if (staged) {
+        for (int k = 0; k < nv; ++k)
+          fl = __fdiv_rn(__fadd_rn(__fmul_rn(fl, (float)k), dist_s[b - run_b + k]), (float)(k + 1));
+      } else {
+        for (int k = 0; k < nv; ++k)
+          fl = __fdiv_rn(__fadd_rn(__fmul_rn(fl, (float)k), td[vl[b + k]]), (float)(k + 1));
+      }
+      flow[d.pix + c] = fl;
+      ismax[d.pix + c] = nv >= min_votes ? nv : -1;
+      if (nv >= min_votes)
+        seedmark[d.pix + (staged ? stage_out[b - run_b + min_votes - 1] : vl[b + min_votes - 1])] = c;
+    }
+    __syncthreads();
   }
 }
 
 // Seeds in the order they reached min votes = ascending key voter index.
-__global__ __launch_bounds__(1024) void k_seeds(const PlaneDesc* __restrict__ P,
-                                                 PlaneCounts* __restrict__ counts,
-                                                 const int32_t* __restrict__ seedmark,
-                                                 int32_t* __restrict__ seeds) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
-  const int p = blockIdx.x;
+__global__ __launch_bounds__(kScanStep) void k_seeds_sum(const PlaneDesc* __restrict__ P,
+                                                         const PlaneCounts* __restrict__ counts,
+                                                         const int32_t* __restrict__ seedmark,
+                                                         int32_t* __restrict__ part) {
+  __shared__ int wsum[kScanStep / 64];
+  const int p = blockIdx.y;
+  const int32_t* a = seedmark + P[p].pix;
+  int lo, hi;
+  scan_range(counts[p].n_edges, lo, hi);
+  int acc = 0;
+  for (int i = lo + threadIdx.x; i < hi; i += kScanStep) acc += a[i] >= 0;
+  int tot;
+  block_incl_scan(acc, wsum, &tot);
+  if (threadIdx.x == 0) part[p * kScanBlocks + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanStep) void k_seeds(const PlaneDesc* __restrict__ P,
+                                                     PlaneCounts* __restrict__ counts,
+                                                     const int32_t* __restrict__ seedmark,
+                                                     const int32_t* __restrict__ part,
+                                                     int32_t* __restrict__ seeds) {
+  __shared__ int wsum[kScanStep / 64];
+  const int p = blockIdx.y;
   const PlaneDesc d = P[p];
-  const int n = counts[p].n_edges;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < n; base += 1024) {
+  int lo, hi;
+  scan_range(counts[p].n_edges, lo, hi);
+  int carry = prior_parts(part + p * kScanBlocks);
+  for (int base = lo; base < hi; base += kScanStep) {
     const int i = base + threadIdx.x;
-    const int s = i < n ? seedmark[d.pix + i] : -1;
+    const int s = i < hi ? seedmark[d.pix + i] : -1;
     const int v = s >= 0;
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    int wb = 0, tot = 0;
-    for (int w = 0; w < 16; ++w) {
-      if (w < wid) wb += wsum[w];
-      tot += wsum[w];
-    }
-    if (v) seeds[d.pix + carry + wb + incl - 1] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
-    __syncthreads();
+    int tot;
+    const int incl = block_incl_scan(v, wsum, &tot);
+    if (v) seeds[d.pix + carry + incl - 1] = s;
+    carry += tot;
   }
-  if (threadIdx.x == 0) counts[p].n_seeds = carry;
+  if (blockIdx.x == kScanBlocks - 1 && threadIdx.x == 0) counts[p].n_seeds = carry;
 }
 
 }  // namespace
@@ -1024,14 +1131,17 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
                            L.prm.n_crowns, L.prm.ratio_voting, L.prm.angle_voting, b.chosen,
                            b.tdist, b.vcount);
   mark("vote");
-  k_voters_scan<<<np, 1024, 0, s>>>(P, b.counts, b.vcount, b.vindex, b.ismax);
+  const dim3 gs(kScanBlocks, np);
+  k_voters_sum<<<gs, kScanStep, 0, s>>>(P, b.counts, b.vcount, b.scan_part);
+  k_voters_scan<<<gs, kScanStep, 0, s>>>(P, b.counts, b.vcount, b.scan_part, b.vindex, b.ismax);
   mark("voters_scan");
   k_voters_fill<<<g, 256, 0, s>>>(P, b.counts, b.chosen, b.ismax, b.vlist);
   mark("voters_fill");
-  k_voters_finish<<<g, 256, 0, s>>>(P, b.counts, b.vindex, b.tdist, L.prm.min_votes, b.vlist,
-                                    b.flow, b.ismax, b.seedmark);
+  k_voters_finish<<<g, kFinishBlock, 0, s>>>(P, b.counts, b.vindex, b.tdist, L.prm.min_votes,
+                                              b.vlist, b.flow, b.ismax, b.seedmark);
   mark("voters_finish");
-  k_seeds<<<np, 1024, 0, s>>>(P, b.counts, b.seedmark, b.seeds);
+  k_seeds_sum<<<gs, kScanStep, 0, s>>>(P, b.counts, b.seedmark, b.scan_part);
+  k_seeds<<<gs, kScanStep, 0, s>>>(P, b.counts, b.seedmark, b.scan_part, b.seeds);
   mark("seeds");
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -103,8 +103,9 @@ struct MeanVar {
 // sample by sample; returns false when a sample falls outside the image
 // (the caller then discards what f saw).  Sample positions follow the
 // reference's running sums; the image reads of kWalkBatch samples are issued
-// together so their latencies overlap.
-constexpr int kWalkBatch = 8;
+// together so their latencies overlap (a sample past the end re-reads the
+// batch's first one, which is inside).
+constexpr int kWalkBatch = 20;
 
 template <class F>
 __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, float begin, float end,
@@ -140,12 +141,20 @@ __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, 
       y += stepY;
     }
     if (!inside) return false;
-    float v[kWalkBatch];
+    uint8_t p00[kWalkBatch], p01[kWalkBatch], p10[kWalkBatch], p11[kWalkBatch];
 #pragma unroll
-    for (int t = 0; t < kWalkBatch; ++t) v[t] = i0 + t < n ? im.bilinear(px[t], py[t]) : 0.f;
+    for (int t = 0; t < kWalkBatch; ++t) {
+      const bool live = i0 + t < n;
+      const int ix = (int)(live ? px[t] : px[0]), iy = (int)(live ? py[t] : py[0]);
+      const uint8_t* q = im.p + ix + iy * im.w;
+      p00[t] = q[0];
+      p01[t] = q[1];
+      p10[t] = q[im.w];
+      p11[t] = q[im.w + 1];
+    }
 #pragma unroll
     for (int t = 0; t < kWalkBatch; ++t)
-      if (i0 + t < n) f(i0 + t, v[t]);
+      if (i0 + t < n) f(i0 + t, bilinear_of(px[t], py[t], p00[t], p01[t], p10[t], p11[t]));
   }
   return true;
 }
@@ -172,6 +181,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
   __shared__ int idx_of_rank[kMaxOuterPoints];
   __shared__ int w_cnt[NW], w_cand[NW];
   __shared__ float w_max[NW];
+  __shared__ float ref_sig[kMaxRefineSamples][kSelectBlock];  // the thread's refinement cut
   const SelectJobDev J = jobs[blockIdx.x];
   const Img im{J.img, J.iw, J.ih};
   const int n = J.n_outer;
@@ -228,8 +238,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
     const float hx = half * gdx, hy = half * gdy;
     const float psx = s.x - hx, psy = s.y - hy;
     const float pex = s.x + half * gdx, pey = s.y + half * gdy;
-    float sig[kMaxRefineSamples];
-    const bool ok = cut_walk(im, psx, psy, pex, pey, 0.f, 1.f, n_ref, [&](int i, float v) { sig[i] = v; });
+    float (*sig)[kSelectBlock] = ref_sig;
+    const bool ok =
+        cut_walk(im, psx, psy, pex, pey, 0.f, 1.f, n_ref, [&](int i, float v) { sig[i][t] = v; });
     if (ok) {
       float key[3], loc[3];
       for (int k = 0; k < 3; ++k) {
@@ -239,7 +250,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
           float tmp = 0;
           for (int j = 0; j < 9; ++j) {
             const int r = i - 4 + j;
-            const float v = r < 0 ? sig[0] : (r >= n_ref ? sig[n_ref - 1] : sig[r]);
+            const float v = r < 0 ? sig[0][t] : (r >= n_ref ? sig[n_ref - 1][t] : sig[r][t]);
             tmp += v * kRefineKernels[k][j];
           }
           if (i == 0 || tmp > best) {
