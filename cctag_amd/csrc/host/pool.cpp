@@ -2,9 +2,36 @@
 #include "pool.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 
 namespace cctagx {
+namespace {
+
+inline void cpu_relax() {
+#if defined(__x86_64__) || defined(__i386__)
+  __builtin_ia32_pause();
+#endif
+}
+
+// Spins until done() or kSpinMicros have passed: pause for the first few
+// microseconds, then yield the core between checks.  Returns done().
+template <class F>
+bool spin_until(F&& done) {
+  using Clock = std::chrono::steady_clock;
+  const auto t0 = Clock::now();
+  for (int i = 0;; ++i) {
+    if (done()) return true;
+    if ((i & 63) == 63) {
+      const auto us = std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - t0).count();
+      if (us >= TaskPool::kSpinMicros) return done();
+      if (us >= 20) std::this_thread::yield();
+    }
+    cpu_relax();
+  }
+}
+
+}  // namespace
 
 TaskPool::TaskPool(int threads) {
   for (int i = 1; i < threads; ++i) workers_.emplace_back([this] { worker(); });
@@ -13,7 +40,7 @@ TaskPool::TaskPool(int threads) {
 TaskPool::~TaskPool() {
   {
     std::lock_guard<std::mutex> lk(m_);
-    stop_ = true;
+    stop_.store(true);
   }
   wake_.notify_all();
   for (std::thread& t : workers_) t.join();
@@ -30,12 +57,8 @@ int TaskPool::default_threads() {
 
 void TaskPool::drain(const std::function<void(int)>& fn, int n) {
   for (;;) {
-    int i;
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      if (next_ >= n) return;
-      i = next_++;
-    }
+    const int i = next_.fetch_add(1, std::memory_order_relaxed);
+    if (i >= n) return;
     try {
       fn(i);
     } catch (...) {
@@ -48,25 +71,20 @@ void TaskPool::drain(const std::function<void(int)>& fn, int n) {
 void TaskPool::worker() {
   uint64_t seen = 0;
   for (;;) {
-    const std::function<void(int)>* fn;
-    int n;
-    {
+    if (!spin_until([&] { return stop_.load() || gen_.load() != seen; })) {
       std::unique_lock<std::mutex> lk(m_);
-      wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
-      if (stop_) return;
-      seen = gen_;
-      // a job the caller already finished alone: nothing to join
-      if (!fn_ || next_ >= n_) continue;
-      fn = fn_;
-      n = n_;
-      ++busy_;
+      wake_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
     }
-    drain(*fn, n);
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      --busy_;
+    if (stop_.load()) return;
+    seen = gen_.load();
+    busy_.fetch_add(1);
+    const std::function<void(int)>* fn = fn_.load();
+    if (fn) drain(*fn, n_.load());
+    busy_.fetch_sub(1);
+    if (caller_waits_.load()) {
+      { std::lock_guard<std::mutex> lk(m_); }
+      done_.notify_all();
     }
-    done_.notify_all();
   }
 }
 
@@ -76,22 +94,31 @@ void TaskPool::run(int n, const std::function<void(int)>& fn) {
     for (int i = 0; i < n; ++i) fn(i);
     return;
   }
+  // a worker still leaving the previous job must not see this one's fields
+  // half written
+  spin_until([&] { return busy_.load() == 0; });
+  while (busy_.load() != 0) std::this_thread::yield();
+  n_.store(n);
+  next_.store(0);
+  error_ = nullptr;
+  fn_.store(&fn);
   {
     std::lock_guard<std::mutex> lk(m_);
-    fn_ = &fn;
-    n_ = n;
-    next_ = 0;
-    error_ = nullptr;
-    ++gen_;
+    gen_.fetch_add(1);
   }
   wake_.notify_all();
   drain(fn, n);
+  // every index is handed out; wait for the workers still inside the job
+  if (!spin_until([&] { return busy_.load() == 0; })) {
+    std::unique_lock<std::mutex> lk(m_);
+    caller_waits_.store(true);
+    done_.wait(lk, [&] { return busy_.load() == 0; });
+    caller_waits_.store(false);
+  }
+  fn_.store(nullptr);
   std::exception_ptr err;
   {
-    std::unique_lock<std::mutex> lk(m_);
-    // every index handed out and every worker that joined this job gone
-    done_.wait(lk, [&] { return next_ >= n_ && busy_ == 0; });
-    fn_ = nullptr;
+    std::lock_guard<std::mutex> lk(m_);
     err = error_;
     error_ = nullptr;
   }
