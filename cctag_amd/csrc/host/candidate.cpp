@@ -9,12 +9,15 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <exception>
 #include <list>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 #include <unordered_set>
 
 #ifdef CCTAGX_TRACE
@@ -97,7 +100,12 @@ void EdgeSet::prepare(const LevelEdges& L) {
   vlist_ = L.vlist;
   pts.resize(L.n);
   pin_.assign(L.n, 0);
+  angle_.assign(L.n, kNoAngle);
   seeds.assign(L.seeds, L.seeds + L.n_seeds);
+}
+
+float EdgeSet::compute_angle(const EdgePoint& e) {
+  return std::fmod(std::atan2(e.gy, e.gx) + 2.0f * kPi, 2.0f * kPi);
 }
 
 void EdgeSet::fill(const LevelEdges& L, int begin, int end) {
@@ -132,14 +140,15 @@ Marker Marker::make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int
   return m;
 }
 
-bool Marker::is_equal(const Marker& other) const {
+Ellipse Marker::half_circle() const {
   Ellipse a = rescaled_outer_ellipse;
   a.set_a(a.b() * 0.5f);
   a.set_b(a.b() * 0.5f);
-  Ellipse b = other.rescaled_outer_ellipse;
-  b.set_a(b.b() * 0.5f);
-  b.set_b(b.b() * 0.5f);
-  return is_overlapping_ellipses(a, b);
+  return a;
+}
+
+bool Marker::is_equal(const Marker& other) const {
+  return is_overlapping_ellipses(half_circle(), other.half_circle());
 }
 
 // ---------------------------------------------------------------------------
@@ -188,8 +197,7 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   static const int xoff[8] = {1, 1, 0, -1, -1, -1, 0, 1};
   const int yoff[8] = {0, dir * -1, dir * -1, dir * -1, 0, dir * 1, dir * 1, dir * 1};
   while ((i < maxLength) && found && (averageVote >= average_vote_min)) {
-    const EdgePoint& ep = es.pts[p];
-    const float angle = std::fmod(std::atan2(ep.gy, ep.gx) + 2.0f * kPi, 2.0f * kPi);
+    const float angle = es.gradient_angle(p);
     phi.push_back(angle);
     if (phi.size() > window) phi.pop_front();
     const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
@@ -714,7 +722,10 @@ namespace {
 struct SeedLink {
   std::deque<int> seg;
   std::vector<int> marks;  // points the linking flags processed_in
+  std::exception_ptr error;
 };
+// A seed's linking in loop one.
+enum SeedState : int { kFree = 0, kTaken = 1, kLinked = 2 };
 }  // namespace
 
 static void link_seed(int seed, const EdgeSet& es, const HostParams& p, SeedLink& out) {
@@ -899,13 +910,14 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
   // Loop one (Detection.cpp:603-627): seeds by received votes
   // (Multiresolution.cpp:239-242, std::sort: its order among equal votes is
   // the library's; the input order is the order in which the points reached
-  // min votes, as on the CPU), then flow components in seed order.  Seeds go
-  // in waves: the next kWave seeds of every level not yet flagged are linked
-  // in parallel, then accepted in order (a seed flagged by an earlier seed of
-  // its wave is dropped, as the serial loop would skip it).
-  constexpr size_t kWave = 32;
+  // min votes, as on the CPU), then flow components in seed order.  A seed
+  // flagged by an earlier seed's linking is skipped, so the flow components
+  // are committed in seed order by one task while the others link seeds
+  // ahead of it (a linking reads only the edge data); a linker skips a seed
+  // already flagged by the commits so far, and the committer links any seed
+  // nobody has taken.
   std::vector<std::vector<std::unique_ptr<Candidate>>> loopOne(nl);
-  std::vector<size_t> n_seeds(nl, 0), next(nl, 0);
+  std::vector<size_t> n_seeds(nl, 0);
   for (int l = 0; l < nl; ++l) {
     EdgeSet& es = levels[l];
     if (es.seeds.empty()) continue;
@@ -915,30 +927,79 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     const size_t nMaxSeeds = (size_t)std::max(level_h[l] / 2, p.maximum_nb_seeds);
     n_seeds[l] = std::min(es.seeds.size(), nMaxSeeds);
   }
-  std::vector<std::pair<int, int>> wave;
-  std::vector<SeedLink> links;
-  for (;;) {
-    wave.clear();
-    for (int l = 0; l < nl; ++l) {
-      size_t taken = 0;
-      while (next[l] < n_seeds[l] && taken < kWave) {
-        const size_t i = next[l]++;
-        if (levels[l].test_processed_in(levels[l].seeds[i])) continue;
-        wave.emplace_back(l, (int)i);
-        ++taken;
+  {
+    std::vector<std::pair<int, int>> order;  // (level, seed rank), level by level
+    for (int l = 0; l < nl; ++l)
+      for (size_t i = 0; i < n_seeds[l]; ++i) order.emplace_back(l, (int)i);
+    const int n = (int)order.size();
+    // a linking's result is allocated by the thread that links the seed
+    std::vector<std::unique_ptr<SeedLink>> links(n);
+    std::unique_ptr<std::atomic<int>[]> state(new std::atomic<int>[n]);  // kFree .. kLinked
+    for (int k = 0; k < n; ++k) state[k].store(kFree, std::memory_order_relaxed);
+    // the commits' processed_in flags as the linkers see them (plain bytes
+    // accessed atomically)
+    std::vector<std::vector<uint8_t>> flagged(nl);
+    for (int l = 0; l < nl; ++l) flagged[l].assign(levels[l].pts.size(), 0);
+    std::atomic<int> next{0}, committed{0};
+    std::atomic<bool> done{false};
+    const int tasks = pool ? pool->threads() : 1;
+    const int lookahead = 4 * tasks;
+    auto link_one = [&](int k) {
+      const int l = order[k].first;
+      links[k].reset(new SeedLink());
+      try {
+        link_seed(levels[l].seeds[order[k].second], levels[l], p, *links[k]);
+      } catch (...) {
+        links[k]->error = std::current_exception();
       }
-    }
-    if (wave.empty()) break;
-    links.clear();
-    links.resize(wave.size());
-    parallel_for(pool, (int)wave.size(), [&](int j) {
-      const int l = wave[j].first, i = wave[j].second;
-      link_seed(levels[l].seeds[i], levels[l], p, links[j]);
+      state[k].store(kLinked, std::memory_order_release);
+    };
+    auto linker = [&] {
+      for (;;) {
+        const int k = next.fetch_add(1, std::memory_order_relaxed);
+        if (k >= n) return;
+        while (k - committed.load(std::memory_order_acquire) > lookahead) {
+          if (done.load(std::memory_order_acquire)) return;
+          std::this_thread::yield();
+        }
+        int expect = kFree;
+        if (!state[k].compare_exchange_strong(expect, kTaken)) continue;
+        const int l = order[k].first;
+        if (__atomic_load_n(&flagged[l][levels[l].seeds[order[k].second]], __ATOMIC_RELAXED)) {
+          state[k].store(kFree, std::memory_order_release);  // the committer skips it
+          continue;
+        }
+        link_one(k);
+      }
+    };
+    auto committer = [&] {
+      struct Done {
+        std::atomic<bool>& d;
+        ~Done() { d.store(true, std::memory_order_release); }
+      } guard{done};
+      for (int k = 0; k < n; ++k) {
+        const int l = order[k].first;
+        EdgeSet& es = levels[l];
+        const int seed = es.seeds[order[k].second];
+        if (!es.test_processed_in(seed)) {
+          int expect = kFree;
+          if (state[k].compare_exchange_strong(expect, kTaken)) {
+            link_one(k);
+          } else {
+            while (state[k].load(std::memory_order_acquire) != kLinked) std::this_thread::yield();
+          }
+          SeedLink& link = *links[k];
+          if (link.error) std::rethrow_exception(link.error);
+          construct_flow_component(seed, es, link, loopOne[l]);
+          for (const int q : link.marks) __atomic_store_n(&flagged[l][q], (uint8_t)1, __ATOMIC_RELAXED);
+        }
+        committed.store(k + 1, std::memory_order_release);
+      }
+    };
+    parallel_for(pool, tasks, [&](int t) {
+      if (t == 0) committer();
+      else linker();
     });
-    for (size_t j = 0; j < wave.size(); ++j) {
-      const int l = wave[j].first, i = wave[j].second;
-      construct_flow_component(levels[l].seeds[i], levels[l], links[j], loopOne[l]);
-    }
   }
   lap(&HostTimes::loop_one);
   // Loop two: one task per (level, candidate) (Detection.cpp:629-658).
@@ -1045,15 +1106,22 @@ void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const 
   });
 }
 
-void update_markers(std::vector<Marker>& markers, Marker&& add) {
-  bool flag = false;
-  for (Marker& cur : markers) {
-    if (cur.status > 0 && add.status > 0 && cur.is_equal(add)) {
-      if (add.quality > cur.quality) cur = add;
-      flag = true;
+std::vector<int> update_markers(const std::vector<Marker>& markers,
+                                const std::vector<Ellipse>& half, const std::vector<int>& order) {
+  std::vector<int> kept;
+  for (const int add : order) {
+    const Marker& a = markers[add];
+    bool flag = false;
+    for (int& cur : kept) {
+      const Marker& c = markers[cur];
+      if (c.status > 0 && a.status > 0 && is_overlapping_ellipses(half[cur], half[add])) {
+        if (a.quality > c.quality) cur = add;
+        flag = true;
+      }
     }
+    if (!flag) kept.push_back(add);
   }
-  if (!flag) markers.push_back(std::move(add));
+  return kept;
 }
 
 }  // namespace cctagx

@@ -74,6 +74,21 @@ class EdgeSet {
   // Types.hpp:142-155: an invalid index throws std::out_of_range.
   void set_processed_in(int i, bool f) { pin_.at(check(i)) = f; }
   bool test_processed_in(int i) const { return pin_.at(check(i)); }
+  // fmod(atan2(gy, gx) + 2 pi, 2 pi) of point i (Vote.cpp:296), computed on
+  // first use and kept; concurrent callers store the same bits.
+  float gradient_angle(int i) const {
+    uint32_t bits = __atomic_load_n(&angle_[i], __ATOMIC_RELAXED);
+    if (bits == kNoAngle) {
+      const float a = compute_angle(pts[i]);
+      __builtin_memcpy(&bits, &a, 4);
+      __atomic_store_n(&angle_[i], bits, __ATOMIC_RELAXED);
+      return a;
+    }
+    float a;
+    __builtin_memcpy(&a, &bits, 4);
+    return a;
+  }
+  static float compute_angle(const EdgePoint& e);
 
   std::vector<EdgePoint> pts;
   std::vector<int32_t> seeds;
@@ -88,6 +103,8 @@ class EdgeSet {
   std::vector<int32_t> rank_;  // set bits before each word
   const int32_t *before_ = nullptr, *after_ = nullptr, *vindex_ = nullptr, *vlist_ = nullptr;
   std::vector<uint8_t> pin_;
+  static constexpr uint32_t kNoAngle = 0xFFFFFFFFu;  // a NaN no angle computation yields
+  mutable std::vector<uint32_t> angle_;
 };
 
 // The detection parameters the host stages read (Params.hpp:108-199).
@@ -130,8 +147,10 @@ struct Marker {
   // CCTag.hpp:68-93 constructor.
   static Marker make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int level,
                      float scale, float quality);
-  // CCTag.cpp:42-59.
+  // CCTag.cpp:42-59: the outer ellipses shrunk to circles of half their
+  // minor axis overlap.
   bool is_equal(const Marker& other) const;
+  Ellipse half_circle() const;
 };
 
 // cctagDetectionFromEdges (Detection.cpp:568-693) for every level at once,
@@ -160,7 +179,11 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
                      TaskPool* pool);
 
-// Multiresolution.cpp:172-193 (`m` is consumed).
-void update_markers(std::vector<Marker>& markers, Marker&& m);
+// Multiresolution.cpp:172-193 over a whole list: `order` names markers in
+// arrival order; the result names the kept ones (an entry replaced by a
+// better-quality overlapping arrival takes that arrival's index).  `half`
+// holds each marker's comparison circle (Marker::half_circle).
+std::vector<int> update_markers(const std::vector<Marker>& markers,
+                                const std::vector<Ellipse>& half, const std::vector<int>& order);
 
 }  // namespace cctagx

@@ -45,9 +45,26 @@ void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p
 
 void finish_markers(std::vector<Marker>& markers, std::vector<Marker>& out) {
   out.clear();
-  std::vector<Marker> prelim;
-  for (Marker& m : markers) update_markers(prelim, std::move(m));
-  for (Marker& m : prelim) update_markers(out, std::move(m));
+  // both de-duplication passes on indices: a marker's comparison circle is
+  // built once, and only the kept markers are moved out (`markers` is
+  // consumed)
+  std::vector<Ellipse> half(markers.size());
+  std::vector<int> all(markers.size());
+  for (size_t i = 0; i < markers.size(); ++i) {
+    half[i] = markers[i].half_circle();
+    all[i] = (int)i;
+  }
+  const std::vector<int> prelim = update_markers(markers, half, all);
+  const std::vector<int> kept = update_markers(markers, half, prelim);
+  // an index kept twice (one arrival replacing two entries) is copied,
+  // its last occurrence moved
+  std::vector<int> left(markers.size(), 0);
+  for (const int i : kept) ++left[i];
+  out.reserve(kept.size());
+  for (const int i : kept) {
+    if (--left[i] > 0) out.push_back(markers[i]);
+    else out.push_back(std::move(markers[i]));
+  }
   std::stable_sort(out.begin(), out.end(), [](const Marker& a, const Marker& b) { return a.id < b.id; });
 }
 

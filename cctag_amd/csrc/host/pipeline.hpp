@@ -30,6 +30,7 @@ void detect_markers(const std::vector<LevelEdges>& levels, const HostParams& p,
 void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p,
                       TaskPool* pool, std::vector<Marker>& markers, HostTimes* times = nullptr,
                       FrameWorkspace* ws = nullptr);
+// (`markers` is consumed.)
 void finish_markers(std::vector<Marker>& markers, std::vector<Marker>& out);
 
 }  // namespace cctagx
