@@ -154,6 +154,11 @@ def refine_comparison(mosaic, Refiner, steps=20, reps=3):
                               "grid": err(c_grid, ok)}}
 
 
+def progress(msg: str) -> None:
+    """One line per finished stage on stderr (stdout carries only the JSON)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +189,7 @@ def main():
                       if args.refine_markers > 0 else None)
             side["batch"] = pool.map(synth.batch_frame, range(args.batch_frames))
             side["mosaic"] = mosaic.get() if mosaic else None
+        progress("synthetic inputs ready")
 
     import torch
     from cctag_amd.detector import BatchDetector, Detector, Refiner
@@ -210,6 +216,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     fps = K / elapsed
+    progress("timed steps done: %.1f frames/s" % fps)
     markers = step_markers[-1]
     assert all(m == markers for m in step_markers), "detection differs between steps"
 
@@ -240,6 +247,7 @@ def main():
         det.detect(img)
     torch.cuda.synchronize()
     fps_upload = K / (time.perf_counter() - t0)
+    progress("per-kernel times and upload-inclusive rate done")
 
     # accuracy against the synthetic ground truth
     errs, ids_ok = [], 0
@@ -294,9 +302,12 @@ def main():
                          "same 1920x1080 frame on %d threads (front end serial, candidate "
                          "stage and per-candidate identification parallel)" % (iters, threads),
                "single_thread_value": 1.0 / t_serial}
+        progress("CPU baseline done: %.2f frames/s on %d threads" % (cpu["value"], threads))
 
     batched = batched_comparison(side.get("batch") or [], Detector, BatchDetector)
+    progress("batched comparison done")
     refine = refine_comparison(side.get("mosaic"), Refiner)
+    progress("refinement comparison done")
 
     line = {
         "metric": "1080p frames/s (16 markers), plain pipeline",

@@ -913,20 +913,28 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
   // min votes, as on the CPU), then flow components in seed order.  A seed
   // flagged by an earlier seed's linking is skipped, so the flow components
   // are committed in seed order by one task while the others link seeds
-  // ahead of it (a linking reads only the edge data); a linker skips a seed
-  // already flagged by the commits so far, and the committer links any seed
-  // nobody has taken.
+  // ahead of it (a linking reads only the edge data).  A linker passes over
+  // a seed that the commits so far or any finished linking would flag; the
+  // committer links any seed nobody has taken.
   std::vector<std::vector<std::unique_ptr<Candidate>>> loopOne(nl);
   std::vector<size_t> n_seeds(nl, 0);
   for (int l = 0; l < nl; ++l) {
     EdgeSet& es = levels[l];
-    if (es.seeds.empty()) continue;
-    if (es.seeds.size() > 1)
-      std::sort(es.seeds.begin(), es.seeds.end(),
-                [&](int a, int b) { return es.pts[a].is_max > es.pts[b].is_max; });
     const size_t nMaxSeeds = (size_t)std::max(level_h[l] / 2, p.maximum_nb_seeds);
     n_seeds[l] = std::min(es.seeds.size(), nMaxSeeds);
   }
+  // the sorts on (votes, seed) pairs: introsort's moves depend only on the
+  // comparisons' outcomes, so the order (ties included) is the one sorting
+  // the seeds through the vote lookup gives
+  parallel_for(pool, nl, [&](int l) {
+    EdgeSet& es = levels[l];
+    if (es.seeds.size() < 2) return;
+    std::vector<std::pair<int32_t, int32_t>> kv(es.seeds.size());
+    for (size_t i = 0; i < kv.size(); ++i) kv[i] = {es.pts[es.seeds[i]].is_max, es.seeds[i]};
+    std::sort(kv.begin(), kv.end(), [](const std::pair<int32_t, int32_t>& a,
+                                       const std::pair<int32_t, int32_t>& b) { return a.first > b.first; });
+    for (size_t i = 0; i < kv.size(); ++i) es.seeds[i] = kv[i].second;
+  });
   {
     std::vector<std::pair<int, int>> order;  // (level, seed rank), level by level
     for (int l = 0; l < nl; ++l)
@@ -936,14 +944,18 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     std::vector<std::unique_ptr<SeedLink>> links(n);
     std::unique_ptr<std::atomic<int>[]> state(new std::atomic<int>[n]);  // kFree .. kLinked
     for (int k = 0; k < n; ++k) state[k].store(kFree, std::memory_order_relaxed);
-    // the commits' processed_in flags as the linkers see them (plain bytes
-    // accessed atomically)
-    std::vector<std::vector<uint8_t>> flagged(nl);
-    for (int l = 0; l < nl; ++l) flagged[l].assign(levels[l].pts.size(), 0);
+    // the commits' processed_in flags as the linkers see them, and the
+    // points some finished linking would flag if its seed is accepted
+    // (plain bytes accessed atomically)
+    std::vector<std::vector<uint8_t>> flagged(nl), tentative(nl);
+    for (int l = 0; l < nl; ++l) {
+      flagged[l].assign(levels[l].pts.size(), 0);
+      tentative[l].assign(levels[l].pts.size(), 0);
+    }
     std::atomic<int> next{0}, committed{0};
     std::atomic<bool> done{false};
     const int tasks = pool ? pool->threads() : 1;
-    const int lookahead = 4 * tasks;
+    const int lookahead = 16 * tasks;
     auto link_one = [&](int k) {
       const int l = order[k].first;
       links[k].reset(new SeedLink());
@@ -952,10 +964,12 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
       } catch (...) {
         links[k]->error = std::current_exception();
       }
+      for (const int q : links[k]->marks) __atomic_store_n(&tentative[l][q], (uint8_t)1, __ATOMIC_RELAXED);
       state[k].store(kLinked, std::memory_order_release);
     };
     auto linker = [&] {
       for (;;) {
+        if (done.load(std::memory_order_acquire)) return;
         const int k = next.fetch_add(1, std::memory_order_relaxed);
         if (k >= n) return;
         while (k - committed.load(std::memory_order_acquire) > lookahead) {
@@ -964,9 +978,12 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
         }
         int expect = kFree;
         if (!state[k].compare_exchange_strong(expect, kTaken)) continue;
-        const int l = order[k].first;
-        if (__atomic_load_n(&flagged[l][levels[l].seeds[order[k].second]], __ATOMIC_RELAXED)) {
-          state[k].store(kFree, std::memory_order_release);  // the committer skips it
+        // a seed an earlier linking covers is most likely skipped: leave it
+        // to the committer, which links it itself if it is not
+        const int l = order[k].first, seed = levels[l].seeds[order[k].second];
+        if (__atomic_load_n(&flagged[l][seed], __ATOMIC_RELAXED) ||
+            __atomic_load_n(&tentative[l][seed], __ATOMIC_RELAXED)) {
+          state[k].store(kFree, std::memory_order_release);
           continue;
         }
         link_one(k);
@@ -982,12 +999,18 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
         EdgeSet& es = levels[l];
         const int seed = es.seeds[order[k].second];
         if (!es.test_processed_in(seed)) {
-          int expect = kFree;
-          if (state[k].compare_exchange_strong(expect, kTaken)) {
-            link_one(k);
-          } else {
-            while (state[k].load(std::memory_order_acquire) != kLinked) std::this_thread::yield();
+          // link it here unless a linker holds it; a linker that passes it
+          // over hands it back free
+          for (;;) {
+            int expect = kFree;
+            if (state[k].compare_exchange_strong(expect, kTaken)) {
+              link_one(k);
+              break;
+            }
+            if (expect == kLinked) break;
+            std::this_thread::yield();
           }
+          std::atomic_thread_fence(std::memory_order_acquire);
           SeedLink& link = *links[k];
           if (link.error) std::rethrow_exception(link.error);
           construct_flow_component(seed, es, link, loopOne[l]);

@@ -14,19 +14,18 @@ inline void cpu_relax() {
 #endif
 }
 
-// Spins until done() or kSpinMicros have passed: pause for the first few
-// microseconds, then yield the core between checks.  Returns done().
+// Spins (pause only: an idle sibling hardware thread should cost the busy
+// one as little as possible) until done() or `micros` have passed.
+// Returns done().
 template <class F>
-bool spin_until(F&& done) {
+bool spin_until(F&& done, int micros) {
   using Clock = std::chrono::steady_clock;
   const auto t0 = Clock::now();
   for (int i = 0;; ++i) {
     if (done()) return true;
-    if ((i & 63) == 63) {
-      const auto us = std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - t0).count();
-      if (us >= TaskPool::kSpinMicros) return done();
-      if (us >= 20) std::this_thread::yield();
-    }
+    if ((i & 255) == 255 &&
+        std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - t0).count() >= micros)
+      return done();
     cpu_relax();
   }
 }
@@ -71,7 +70,7 @@ void TaskPool::drain(const std::function<void(int)>& fn, int n) {
 void TaskPool::worker() {
   uint64_t seen = 0;
   for (;;) {
-    if (!spin_until([&] { return stop_.load() || gen_.load() != seen; })) {
+    if (!spin_until([&] { return stop_.load() || gen_.load() != seen; }, kIdleSpinMicros)) {
       std::unique_lock<std::mutex> lk(m_);
       wake_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
     }
@@ -96,7 +95,7 @@ void TaskPool::run(int n, const std::function<void(int)>& fn) {
   }
   // a worker still leaving the previous job must not see this one's fields
   // half written
-  spin_until([&] { return busy_.load() == 0; });
+  spin_until([&] { return busy_.load() == 0; }, kWaitSpinMicros);
   while (busy_.load() != 0) std::this_thread::yield();
   n_.store(n);
   next_.store(0);
@@ -109,7 +108,7 @@ void TaskPool::run(int n, const std::function<void(int)>& fn) {
   wake_.notify_all();
   drain(fn, n);
   // every index is handed out; wait for the workers still inside the job
-  if (!spin_until([&] { return busy_.load() == 0; })) {
+  if (!spin_until([&] { return busy_.load() == 0; }, kWaitSpinMicros)) {
     std::unique_lock<std::mutex> lk(m_);
     caller_waits_.store(true);
     done_.wait(lk, [&] { return busy_.load() == 0; });

@@ -5,7 +5,7 @@
 //
 // A detection issues dozens of short parallel loops separated by short
 // serial steps, so idle workers first spin on the job generation for a
-// bounded time (kSpinMicros) and only then sleep on the condition variable;
+// bounded time (kIdleSpinMicros) and only then sleep on the condition variable;
 // indices are handed out by an atomic counter.
 #pragma once
 
@@ -35,9 +35,10 @@ class TaskPool {
   // min(16, hardware threads), or CCTAGX_HOST_THREADS when set.
   static int default_threads();
 
-  // How long an idle worker (and a caller waiting for stragglers) spins
-  // before it sleeps.
-  static constexpr int kSpinMicros = 1000;
+  // How long an idle worker spins for the next job, and a caller for the
+  // workers still inside its job, before sleeping.
+  static constexpr int kIdleSpinMicros = 1000;
+  static constexpr int kWaitSpinMicros = 2000;
 
  private:
   void worker();

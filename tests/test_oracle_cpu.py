@@ -277,3 +277,16 @@ def test_candidate_stage_independent_of_thread_count(threads):
     it on a thread pool, the oracle serially: both must agree exactly."""
     img, _ = frames.config_frame(1)
     assert oracle_lib.detect(img, threads=threads) == oracle_lib.detect(img)
+
+
+@pytest.mark.timeout(300)
+def test_speculative_seed_linking_repeated():
+    """Loop one commits seeds in order while other threads link seeds ahead
+    and may hand a seed back to the committer: many runs with more threads
+    than cores, on frames with many seeds, all finish and all agree with the
+    serial run (a hand-back the committer missed used to leave it waiting)."""
+    for k in range(2):
+        img, _ = frames.config_frame(2, k)
+        ref = oracle_lib.detect(img)
+        for _ in range(6):
+            assert oracle_lib.detect(img, threads=24) == ref
