@@ -26,6 +26,9 @@ CLANGXX = "/opt/rocm/llvm/bin/clang++" if os.path.exists("/opt/rocm/llvm/bin/cla
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas",
           "-Wno-sign-compare"]
+# host stages: AVX2-era x86-64 (vector widths change, IEEE results do not:
+# no contraction, no reassociation)
+HOST_FLAGS = ["-march=x86-64-v3"]
 HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}",
                       "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -43,7 +46,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, *HIP_FLAGS, "-I", CSRC, "-c", src, "-o", obj]
     else:
-        cmd = [CLANGXX, *COMMON, "-I", CSRC, "-c", src, "-o", obj]
+        cmd = [CLANGXX, *COMMON, *HOST_FLAGS, "-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed for %s:\n%s%s" % (src, r.stdout, r.stderr))

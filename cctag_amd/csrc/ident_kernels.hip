@@ -314,7 +314,7 @@ constexpr int kRefineChunk = 10;  // samples per sampling work item
 constexpr int kLoadBatch = 5;     // samples whose image reads are issued together
 static_assert(kRefineChunk % kLoadBatch == 0, "whole load batches per chunk");
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
-constexpr int kFuseMaxBlocks = 1024;  // grid blocks up to which the selection is fused
+constexpr int kFuseMaxBlocks = 2048;  // grid blocks up to which the selection is fused
 
 // getNearbyPoints point g + computeHomographyFromEllipseAndImagedCenter
 // (Identification.cpp:739-798, 1071-1109) and the inverse.
