@@ -228,6 +228,7 @@ def main():
                              "ident_us", "finish_us", "ident_prep_us", "ident_select_us",
                              "ident_refine_us", "ident_post_us")}
     work = {"select_samples": 0, "refine_samples": 0}
+    det.set_stage_timing(True)
     for _ in range(K):
         det.detect_loaded()
         ds = det.detect_stats()
@@ -239,6 +240,7 @@ def main():
             sums[k] += ds[k]
         for k in work:
             work[k] += ds[k]
+    det.set_stage_timing(False)
 
     # the drop-in call with the frame in host memory (upload included)
     torch.cuda.synchronize()

@@ -188,6 +188,7 @@ def load() -> ctypes.CDLL:
     lib.cctagx_destroy.argtypes = [vp]
     lib.cctagx_front_end.argtypes = [vp, vp, ctypes.c_size_t]
     lib.cctagx_get_level.argtypes = [vp, ctypes.c_int, P(LevelView)]
+    lib.cctagx_set_stage_timing.argtypes = [vp, ctypes.c_int]
     lib.cctagx_stage_count.argtypes = [vp]
     lib.cctagx_stage_time.argtypes = [vp, ctypes.c_int, P(ctypes.c_char_p), P(c_f32)]
     lib.cctagx_time_front_end.argtypes = [vp, ctypes.c_int, P(c_f32)]

@@ -64,6 +64,7 @@ namespace {
 // the same state whatever was tried.
 int autotune(cctagx_batch* b) {
   StageTimer timer;
+  timer.per_kernel = true;
   auto run = [&](int fv, int eb, const char* const* names, int n_names, float& us) -> int {
     b->fe.set_partition(fv, eb);
     float best = 1e30f;

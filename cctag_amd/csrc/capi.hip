@@ -334,6 +334,12 @@ int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* v) {
   return CCTAGX_OK;
 }
 
+int cctagx_set_stage_timing(cctagx_handle* h, int per_kernel) {
+  if (!h) return CCTAGX_ERR_ARG;
+  h->timer.per_kernel = per_kernel != 0;
+  return CCTAGX_OK;
+}
+
 int cctagx_stage_count(cctagx_handle* h) {
   if (!h) return CCTAGX_ERR_ARG;
   return h->timer.used > 0 ? (int)h->timer.used - 1 : 0;

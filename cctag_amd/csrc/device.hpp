@@ -57,45 +57,53 @@ class DeviceArena {
   size_t size_ = 0;
 };
 
-// Pinned host buffer.
+// Pinned host buffer, also mapped for the device: kernels may read it (jobs
+// the host wrote before the launch) or write it (results the host reads
+// after synchronizing; allocate such buffers with hipHostMallocCoherent).
 class PinnedBuffer {
  public:
   PinnedBuffer() = default;
   PinnedBuffer(const PinnedBuffer&) = delete;
   PinnedBuffer& operator=(const PinnedBuffer&) = delete;
-  PinnedBuffer(PinnedBuffer&& o) noexcept : p_(o.p_), cap_(o.cap_) {
-    o.p_ = nullptr;
+  PinnedBuffer(PinnedBuffer&& o) noexcept : p_(o.p_), d_(o.d_), cap_(o.cap_) {
+    o.p_ = o.d_ = nullptr;
     o.cap_ = 0;
   }
   PinnedBuffer& operator=(PinnedBuffer&& o) noexcept {
     if (this != &o) {
       release();
       p_ = o.p_;
+      d_ = o.d_;
       cap_ = o.cap_;
-      o.p_ = nullptr;
+      o.p_ = o.d_ = nullptr;
       o.cap_ = 0;
     }
     return *this;
   }
   ~PinnedBuffer() { release(); }
-  int ensure(size_t bytes) {
+  int ensure(size_t bytes, unsigned flags = hipHostMallocDefault) {
     if (bytes <= cap_) return 0;
     release();
-    CCTAGX_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+    CCTAGX_HIP(hipHostMalloc(&p_, bytes, flags));
     cap_ = bytes;
+    CCTAGX_HIP(hipHostGetDevicePointer(&d_, p_, 0));
     return 0;
   }
   void release() {
     if (p_) hipHostFree(p_);
-    p_ = nullptr;
+    p_ = d_ = nullptr;
     cap_ = 0;
   }
   template <class T>
   T* as() const { return static_cast<T*>(p_); }
+  // The same memory as the device addresses it.
+  template <class T>
+  T* dev() const { return static_cast<T*>(d_); }
   size_t capacity() const { return cap_; }
 
  private:
   void* p_ = nullptr;
+  void* d_ = nullptr;
   size_t cap_ = 0;
 };
 

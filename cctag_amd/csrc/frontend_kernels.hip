@@ -1086,10 +1086,11 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   const FrontEndBuffers& b = L.buf;
   const PlaneDesc* P = L.dev_planes;
   const int np = L.n_planes;
-  auto mark = [&](const char* name) {
-    if (timer) timer->mark(name, s);
+  auto mark = [&](const char* name, bool edge = false) {
+    if (timer && (edge || timer->per_kernel)) timer->mark(name, s);
   };
-  mark("start");
+
+  mark("start", true);
   if (L.n_levels > 1 && L.n_tiles_p > 0) {
     k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img);
     mark("pyramid");
@@ -1142,7 +1143,7 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   mark("voters_finish");
   k_seeds_sum<<<gs, kScanStep, 0, s>>>(P, b.counts, b.seedmark, b.scan_part);
   k_seeds<<<gs, kScanStep, 0, s>>>(P, b.counts, b.seedmark, b.scan_part, b.seeds);
-  mark("seeds");
+  mark("seeds", true);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -14,6 +14,9 @@ struct StageTimer {
   std::vector<hipEvent_t> events;
   std::vector<const char*> names;
   size_t used = 0;
+  // every kernel marked, or only the sequence's first and last marks (each
+  // mark is an event record on the stream)
+  bool per_kernel = false;
   void reset() {
     used = 0;
     names.clear();

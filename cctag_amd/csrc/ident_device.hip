@@ -37,24 +37,15 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   dev_.release();
   dev_.clear_plan();
-  const size_t o_sj = dev_.reserve(sizeof(SelectJobDev) * nm);
-  const size_t o_pts = dev_.reserve(sizeof(float4) * np);
-  const size_t o_so = dev_.reserve(sizeof(SelectOutDev) * nm);
+
   const size_t o_rj = dev_.reserve(sizeof(RefineJobDev) * nm);
-  const size_t o_ro = dev_.reserve(sizeof(RefineOutDev) * nm);
-  const size_t o_rd = dev_.reserve(sizeof(float) * (size_t)nm * kMaxCuts * kMaxBankRows);
   const size_t o_st = dev_.reserve(sizeof(RefineStateDev) * 2 * nm);
   const size_t o_gr = dev_.reserve(sizeof(RefineGridDev) * 2 * (size_t)nm * kMaxGrid * kMaxGrid);
   const size_t o_gm =
       dev_.reserve(sizeof(double) * 2 * (size_t)nm * kMaxGrid * kMaxGrid * kMaxPairs);
   const size_t o_inv = dev_.reserve(sizeof(float) * kMaxBank * kMaxRatios);
   if (dev_.allocate() != 0) throw std::runtime_error(std::string("hipMalloc: ") + last_error());
-  d_sel_jobs_ = dev_.at<SelectJobDev>(o_sj);
-  d_pts_ = dev_.at<float4>(o_pts);
-  d_sel_out_ = dev_.at<SelectOutDev>(o_so);
   d_ref_jobs_ = dev_.at<RefineJobDev>(o_rj);
-  d_ref_out_ = dev_.at<RefineOutDev>(o_ro);
-  d_ref_dist_ = dev_.at<float>(o_rd);
   d_work_.state = dev_.at<RefineStateDev>(o_st);
   d_work_.grid = dev_.at<RefineGridDev>(o_gr);
   d_work_.gram = dev_.at<double>(o_gm);
@@ -62,10 +53,13 @@ void GpuIdentBackend::ensure(int n_markers, int n_points) {
   hip_check(hipMemcpy(d_inv_ratio_, &k_.inv_ratio[0][0], sizeof(float) * kMaxBank * kMaxRatios,
                       hipMemcpyHostToDevice),
             "hipMemcpy bank");
+  // the select kernel reads its jobs from and writes its results to mapped
+  // host memory, as the refinement's last kernel does its results
   if (h_sel_jobs_.ensure(sizeof(SelectJobDev) * nm) || h_pts_.ensure(sizeof(float4) * np) ||
-      h_sel_out_.ensure(sizeof(SelectOutDev) * nm) || h_ref_jobs_.ensure(sizeof(RefineJobDev) * nm) ||
-      h_ref_out_.ensure(sizeof(RefineOutDev) * nm) ||
-      h_ref_dist_.ensure(sizeof(float) * (size_t)nm * kMaxCuts * kMaxBankRows))
+      h_sel_out_.ensure(sizeof(SelectOutDev) * nm, hipHostMallocCoherent) ||
+      h_ref_jobs_.ensure(sizeof(RefineJobDev) * nm) ||
+      h_ref_out_.ensure(sizeof(RefineOutDev) * nm, hipHostMallocCoherent) ||
+      h_ref_dist_.ensure(sizeof(float) * (size_t)nm * kMaxCuts * kMaxBankRows, hipHostMallocCoherent))
     throw std::runtime_error(std::string("hipHostMalloc: ") + last_error());
   cap_markers_ = nm;
   cap_points_ = np;
@@ -99,15 +93,13 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
     for (const Pt2& q : j.outer_points) hp[off++] = make_float4(q.x, q.y, q.dx, q.dy);
   }
   hipStream_t s = stream_;
-  hip_check(hipMemcpyAsync(d_sel_jobs_, hj, sizeof(SelectJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
-  hip_check(hipMemcpyAsync(d_pts_, hp, sizeof(float4) * npts, hipMemcpyHostToDevice, s), "H2D");
   hipEventRecord(ev_[0], s);
-  if (launch_select_cuts(d_sel_jobs_, n, d_pts_, k_.start_sig, k_.n_samples,
-                         k_.n_refine_samples, p.num_cuts_in_ident_step, d_sel_out_, s) != 0)
+  if (launch_select_cuts(h_sel_jobs_.dev<SelectJobDev>(), n, h_pts_.dev<float4>(), k_.start_sig,
+                         k_.n_samples, k_.n_refine_samples, p.num_cuts_in_ident_step,
+                         h_sel_out_.dev<SelectOutDev>(), s) != 0)
     throw std::runtime_error("k_select_cuts launch failed");
   hipEventRecord(ev_[1], s);
-  SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
-  hip_check(hipMemcpyAsync(ho, d_sel_out_, sizeof(SelectOutDev) * n, hipMemcpyDeviceToHost, s), "D2H");
+  const SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
   hip_check(hipStreamSynchronize(s), "sync select");
   hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
   select_us_ *= 1000.f;
@@ -166,16 +158,12 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   int max_iter = 0;
   for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
   if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
-                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_, d_ref_out_, d_ref_dist_,
-                    s) != 0)
+                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_,
+                    h_ref_out_.dev<RefineOutDev>(), h_ref_dist_.dev<float>(), s) != 0)
     throw std::runtime_error("k_refine launch failed");
   hipEventRecord(ev_[3], s);
-  RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
-  float* hd = h_ref_dist_.as<float>();
-  hip_check(hipMemcpyAsync(ho, d_ref_out_, sizeof(RefineOutDev) * n, hipMemcpyDeviceToHost, s), "D2H");
-  if (n_dist > 0)
-    hip_check(hipMemcpyAsync(hd, d_ref_dist_, sizeof(float) * n_dist, hipMemcpyDeviceToHost, s),
-              "D2H");
+  const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
+  const float* hd = h_ref_dist_.as<float>();
   hip_check(hipStreamSynchronize(s), "sync refine");
   hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
   refine_us_ *= 1000.f;

@@ -123,12 +123,7 @@ class GpuIdentBackend : public IdentBackend {
   int cap_markers_ = 0, cap_points_ = 0;
   DeviceArena dev_;
   PinnedBuffer h_sel_jobs_, h_pts_, h_sel_out_, h_ref_jobs_, h_ref_out_, h_ref_dist_;
-  SelectJobDev* d_sel_jobs_ = nullptr;
-  float4* d_pts_ = nullptr;
-  SelectOutDev* d_sel_out_ = nullptr;
   RefineJobDev* d_ref_jobs_ = nullptr;
-  RefineOutDev* d_ref_out_ = nullptr;
-  float* d_ref_dist_ = nullptr;
   RefineWork d_work_{};
   float* d_inv_ratio_ = nullptr;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -104,6 +104,12 @@ class Detector:
                    "cctagx_get_level")
         return _lib.level_view_to_dict(v)
 
+    def set_stage_timing(self, per_kernel: bool) -> None:
+        """Event marks after every front-end kernel (profiling passes) or only
+        around the whole front end (the default)."""
+        _lib.check(self._lib.cctagx_set_stage_timing(self._h, int(per_kernel)),
+                   "cctagx_set_stage_timing")
+
     def stage_times(self) -> list:
         out = []
         n = self._lib.cctagx_stage_count(self._h)

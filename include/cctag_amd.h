@@ -261,7 +261,11 @@ int cctagx_refiner_times(cctagx_refiner* r, float* descend_us, float* grid_us);
 
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
- * events on the library stream; names are static strings. */
+ * events on the library stream; names are static strings.  By default the
+ * front end is timed as one stage ("seeds": start to end); per_kernel != 0
+ * records an event after every kernel (each record is a packet on the
+ * stream, so per-kernel timing is for profiling passes). */
+int cctagx_set_stage_timing(cctagx_handle* h, int per_kernel);
 int cctagx_stage_count(cctagx_handle* h);
 int cctagx_stage_time(cctagx_handle* h, int i, const char** name, float* usec);
 /* Times `iters` back-to-back front ends on the resident frame (no upload)
