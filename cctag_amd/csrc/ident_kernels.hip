@@ -24,9 +24,14 @@ namespace {
 
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
+// A pointer the compiler knows to be in device global memory (the loads
+// become global rather than flat ones).
+typedef const __attribute__((address_space(1))) uint8_t* gptr_u8;
+
 struct Img {
   const uint8_t* p;
   int w, h;
+  __device__ __forceinline__ gptr_u8 gp() const { return (gptr_u8)p; }
   // Identification.hpp:197-222: half of the bilinear value.
   __device__ __forceinline__ float bilinear(float x, float y) const {
     const int px = (int)x, py = (int)y;
@@ -146,7 +151,7 @@ __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, 
     for (int t = 0; t < kWalkBatch; ++t) {
       const bool live = i0 + t < n;
       const int ix = (int)(live ? px[t] : px[0]), iy = (int)(live ? py[t] : py[0]);
-      const uint8_t* q = im.p + ix + iy * im.w;
+      const gptr_u8 q = im.gp() + ix + iy * im.w;
       p00[t] = q[0];
       p01[t] = q[1];
       p10[t] = q[im.w];
@@ -348,12 +353,19 @@ __device__ void grid_homography(const RefineJobDev& J, float cur_x, float cur_y,
   inverse3(Hf, Hi);
 }
 
-// extractSignalUsingHomography (Identification.cpp:216-280) for samples
-// [c*kRefineChunk, ...) of cut k: the stop back-projected by Hi, samples
-// from begin*stop to end*stop in the marker plane (running sums replayed up
-// to the chunk), mapped by H, half bilinear value; 0 and the flag when out.
-__device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, float2 stop,
-                             float begin_sig, float end_sig, int n_samples, int c, float* out) {
+// extractSignalUsingHomography (Identification.cpp:216-280), first part,
+// one thread per cut: the stop back-projected by Hi, the sample positions
+// from begin*stop to end*stop in the marker plane as the reference's running
+// sums, kept at the start of every kRefineChunk-sample chunk (the chunks'
+// threads then resume the sums from there).
+constexpr int kMaxChunks = (kMaxSamples + kRefineChunk - 1) / kRefineChunk;
+struct CutWalk {
+  float2 start[kMaxChunks];
+  float2 step;
+};
+
+__device__ void cut_walk_starts(const float* Hi, float2 stop, float begin_sig, float end_sig,
+                                int n_samples, CutWalk& w) {
   float bx, by;
   apply_h(bx, by, Hi, stop.x, stop.y);
   float xs, ys, xe, ye;
@@ -373,11 +385,24 @@ __device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, flo
   }
   const float stepX = (xe - xs) / ((float)n_samples - 1.f), stepY = (ye - ys) / ((float)n_samples - 1.f);
   float x = xs, y = ys;
-  const int i0 = c * kRefineChunk;
-  for (int i = 0; i < i0; ++i) {
-    x += stepX;
-    y += stepY;
+  const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
+  for (int c = 0; c < nchunk; ++c) {
+    w.start[c] = make_float2(x, y);
+    for (int t = 0; t < kRefineChunk; ++t) {
+      x += stepX;
+      y += stepY;
+    }
   }
+  w.step = make_float2(stepX, stepY);
+}
+
+// Second part for samples [c*kRefineChunk, ...) of a cut: mapped by H, half
+// bilinear value; 0 and the flag when out of the image.
+__device__ bool sample_chunk(const Img& im, const float* H, const CutWalk& w, int n_samples, int c,
+                             float* out) {
+  float x = w.start[c].x, y = w.start[c].y;
+  const float stepX = w.step.x, stepY = w.step.y;
+  const int i0 = c * kRefineChunk;
   const float xmax = (float)(im.w - 1), ymax = (float)(im.h - 1);
   bool oob = false;
   // per group of kLoadBatch samples: positions first, then every corner load
@@ -400,7 +425,7 @@ __device__ bool sample_chunk(const Img& im, const float* H, const float* Hi, flo
 #pragma unroll
     for (int t = 0; t < kLoadBatch; ++t) {
       const int px = in[t] ? (int)xr[t] : 0, py = in[t] ? (int)yr[t] : 0;
-      const uint8_t* p0 = im.p + px + py * im.w;
+      const gptr_u8 p0 = im.gp() + px + py * im.w;
       p00[t] = p0[0];
       p01[t] = p0[1];
       p10[t] = p0[im.w];
@@ -419,13 +444,25 @@ __device__ __forceinline__ int pair_offset(int k, int nc) { return k * nc - k * 
 
 // Row k's share of sum_{k<l valid} sum_i (s_ki - s_li)^2 from the Gram
 // matrix C (packed upper triangle): (m-1) C_kk - 2 sum_{l>k valid} C_kl.
+// The row's entries are read kRowBatch at a time (indices clamped into the
+// row, so the reads need no branch and are in flight together) and
+// subtracted in column order, skipping invalid columns as the plain loop
+// does.
+constexpr int kRowBatch = 8;
 __device__ __forceinline__ double gram_row(const double* C, const uint8_t* valid, int k, int nc,
                                            int m) {
   if (!valid[k]) return 0.0;
   const double* row = C + pair_offset(k, nc);
+  const int last = nc - 1 - k;  // row[0 .. last]
   double acc = (double)(m - 1) * row[0];
-  for (int l = k + 1; l < nc; ++l)
-    if (valid[l]) acc -= 2.0 * row[l - k];
+  for (int d0 = 1; d0 <= last; d0 += kRowBatch) {
+    double v[kRowBatch];
+#pragma unroll
+    for (int j = 0; j < kRowBatch; ++j) v[j] = row[min(d0 + j, last)];
+#pragma unroll
+    for (int j = 0; j < kRowBatch; ++j)
+      if (d0 + j <= last && valid[k + d0 + j]) acc -= 2.0 * v[j];
+  }
   return acc;
 }
 
@@ -593,6 +630,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   __shared__ int oobk[kMaxCuts];
   __shared__ double sC[kMaxPairs];
   __shared__ uint8_t lvalid[kMaxCuts];
+  __shared__ CutWalk walk[kMaxCuts];
   __shared__ RefineStateDev S;
   PickShared& P = U.P;
   auto& sig = U.sig;
@@ -626,11 +664,12 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   }
   if (tid < kMaxCuts) oobk[tid] = 0;
   __syncthreads();
+  if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
+  __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
   for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
     const int k = w / nchunk, c = w - k * nchunk;
-    if (sample_chunk(im, sH, sHi, J.stops[k], J.begin_sig, J.end_sig, n_samples, c, sig[k]))
-      atomicOr(&oobk[k], 1);
+    if (sample_chunk(im, sH, walk[k], n_samples, c, sig[k])) atomicOr(&oobk[k], 1);
   }
   __syncthreads();
   double* C = W.gram + (((size_t)cb * n_jobs + m) * G2MAX + g) * kMaxPairs;
@@ -700,6 +739,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   __shared__ float bar[kMaxSamples];
   __shared__ int valid_cut[kMaxCuts];
   __shared__ int n_valid;
+  __shared__ CutWalk walk[kMaxCuts];
   __shared__ float sH[9], sHi[9];
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
   __shared__ float xs[kMaxSamples];
@@ -736,10 +776,12 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     sHi[tid] = GB.Hi[tid];
   }
   __syncthreads();
+  if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
+  __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
   for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
     const int k = w / nchunk, c = w - k * nchunk;
-    sample_chunk(im, sH, sHi, J.stops[k], J.begin_sig, J.end_sig, n_samples, c, sig[k]);
+    sample_chunk(im, sH, walk[k], n_samples, c, sig[k]);
   }
   __syncthreads();
   if (tid == 0) {
