@@ -692,6 +692,8 @@ __device__ __forceinline__ void update_xy(float dx, float dy, int& x, int& y, fl
   }
 }
 
+constexpr int kDescentBatch = 8;
+
 __device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float gx, float gy,
                        int dir, int nmax, int thr) {
   float e = 0.0f;
@@ -717,17 +719,51 @@ __device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float 
   } else {
     return -1;
   }
+  // Later steps move along fixed (dx, dy): the positions follow from
+  // (x, y, e) alone, so kDescentBatch steps are laid out first, their map
+  // reads issued together (a read outside the level reads its first pixel
+  // and is ignored), then checked in step order as one step at a time would.
+  const float a = ymajor ? fabsf(__fdiv_rn(dx, dy)) : fabsf(__fdiv_rn(dy, dx));
+  sx = (dx > 0.0f) - (dx < 0.0f);
+  sy = (dy > 0.0f) - (dy < 0.0f);
   while (n <= nmax) {
-    if (ymajor) update_xy(dy, dx, y, x, e, sy, sx);
-    else update_xy(dx, dy, x, y, e, sx, sy);
-    n = n + 1;
-    if (!(x >= 0 && x < W && y >= 0 && y < H)) return -1;
-    int r = L.at(x, y);
-    if (r >= 0) return r;
-    const int xs = ymajor ? x : x - sx, ys = ymajor ? y - sy : y;
-    if (!(xs >= 0 && xs < W && ys >= 0 && ys < H)) return -1;
-    r = L.at(xs, ys);
-    if (r >= 0) return r;
+    int r0[kDescentBatch], r1[kDescentBatch];
+    bool in0[kDescentBatch], in1[kDescentBatch], live[kDescentBatch];
+#pragma unroll
+    for (int j = 0; j < kDescentBatch; ++j) {
+      live[j] = n + j <= nmax;
+      e = __fadd_rn(e, a);
+      if (ymajor) {
+        y += sy;
+        if (e >= 0.5f) {
+          x += sx;
+          e = __fsub_rn(e, 1.0f);
+        }
+      } else {
+        x += sx;
+        if (e >= 0.5f) {
+          y += sy;
+          e = __fsub_rn(e, 1.0f);
+        }
+      }
+      const int xs = ymajor ? x : x - sx, ys = ymajor ? y - sy : y;
+      in0[j] = x >= 0 && x < W && y >= 0 && y < H;
+      in1[j] = xs >= 0 && xs < W && ys >= 0 && ys < H;
+      const bool m0 = in0[j] && x < L.w && y < L.h, m1 = in1[j] && xs < L.w && ys < L.h;
+      const int v0 = L.emap[L.pix + (m0 ? y * L.w + x : 0)];
+      const int v1 = L.emap[L.pix + (m1 ? ys * L.w + xs : 0)];
+      r0[j] = m0 ? v0 : -1;
+      r1[j] = m1 ? v1 : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kDescentBatch; ++j) {
+      if (!live[j]) return -1;
+      if (!in0[j]) return -1;
+      if (r0[j] >= 0) return r0[j];
+      if (!in1[j]) return -1;
+      if (r1[j] >= 0) return r1[j];
+    }
+    n += kDescentBatch;
   }
   return -1;
 }

@@ -89,11 +89,6 @@ void EdgeSet::prepare(const LevelEdges& L) {
   bits_ = L.bits;
   const size_t nw = ((size_t)L.w * L.h + 31) / 32;
   rank_.resize(nw);
-  int32_t acc = 0;
-  for (size_t j = 0; j < nw; ++j) {
-    rank_[j] = acc;
-    acc += __builtin_popcount(bits_[j]);
-  }
   before_ = L.before;
   after_ = L.after;
   vindex_ = L.vindex;
@@ -102,6 +97,19 @@ void EdgeSet::prepare(const LevelEdges& L) {
   pin_.assign(L.n, 0);
   angle_.assign(L.n, kNoAngle);
   seeds.assign(L.seeds, L.seeds + L.n_seeds);
+}
+
+int32_t EdgeSet::word_edges(size_t w0, size_t w1) const {
+  int32_t n = 0;
+  for (size_t j = w0; j < w1; ++j) n += __builtin_popcount(bits_[j]);
+  return n;
+}
+
+void EdgeSet::rank_words(size_t w0, size_t w1, int32_t before) {
+  for (size_t j = w0; j < w1; ++j) {
+    rank_[j] = before;
+    before += __builtin_popcount(bits_[j]);
+  }
 }
 
 float EdgeSet::compute_angle(const EdgePoint& e) {

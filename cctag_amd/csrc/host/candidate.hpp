@@ -48,11 +48,17 @@ class EdgeSet {
  public:
   void load(const LevelEdges& L) {
     prepare(L);
+    rank_words(0, n_words(), 0);
     fill(L, 0, L.n);
   }
-  // load() in two steps: sizes, lookup tables and flags, then the points
-  // [begin, end) (independent ranges may be filled concurrently).
+  // load() in steps: sizes and flags; the point index of the first edge of
+  // every 32-pixel word of the bit mask, for words [w0, w1) given the edges
+  // before w0 (word_edges counts the edges of a word range); the points
+  // [begin, end).  Independent ranges may be done concurrently.
   void prepare(const LevelEdges& L);
+  size_t n_words() const { return rank_.size(); }
+  int32_t word_edges(size_t w0, size_t w1) const;
+  void rank_words(size_t w0, size_t w1, int32_t before);
   void fill(const LevelEdges& L, int begin, int end);
   // Point index at (x, y) or -1; the caller guarantees 0 <= x < shape_w and
   // 0 <= y < shape_h (the collection's shape is the level-0 size).

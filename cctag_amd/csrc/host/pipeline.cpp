@@ -28,12 +28,40 @@ void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p
     es[l].prepare(levels[l]);
     level_h[l] = levels[l].h;
   });
-  constexpr int kFillChunk = 16384;
+  // the word ranks (edge counts of word ranges, their running sum, the
+  // ranks) and the points, in pieces small enough to spread over the pool
+  constexpr size_t kRankWords = 8192;
+  constexpr int kFillChunk = 4096;
+  struct WordRange {
+    int l;
+    size_t w0, w1;
+    int32_t before;
+  };
+  std::vector<WordRange> ranges;
+  for (int l = 0; l < nl; ++l)
+    for (size_t w0 = 0; w0 < es[l].n_words(); w0 += kRankWords)
+      ranges.push_back({l, w0, std::min(w0 + kRankWords, es[l].n_words()), 0});
+  parallel_for(pool, (int)ranges.size(), [&](int r) {
+    ranges[r].before = es[ranges[r].l].word_edges(ranges[r].w0, ranges[r].w1);
+  });
+  int32_t run = 0;
+  for (size_t r = 0; r < ranges.size(); ++r) {
+    if (r == 0 || ranges[r].l != ranges[r - 1].l) run = 0;
+    const int32_t n = ranges[r].before;
+    ranges[r].before = run;
+    run += n;
+  }
   std::vector<std::pair<int, int>> chunks;
   for (int l = 0; l < nl; ++l)
     for (int b = 0; b < levels[l].n; b += kFillChunk) chunks.emplace_back(l, b);
-  parallel_for(pool, (int)chunks.size(), [&](int c) {
-    const int l = chunks[c].first, b = chunks[c].second;
+  const int nr = (int)ranges.size();
+  parallel_for(pool, nr + (int)chunks.size(), [&](int c) {
+    if (c < nr) {
+      const WordRange& r = ranges[c];
+      es[r.l].rank_words(r.w0, r.w1, r.before);
+      return;
+    }
+    const int l = chunks[c - nr].first, b = chunks[c - nr].second;
     es[l].fill(levels[l], b, std::min(b + kFillChunk, levels[l].n));
   });
   if (times) times->load += ms_since(t);
