@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -120,6 +122,79 @@ def batched_comparison(frames, Detector, BatchDetector, reps=3):
                                                             "candidates_us", "ident_us")}}
 
 
+def gt_centre_errors(markers, gts):
+    """Distance of every identified marker to the nearest ground-truth centre,
+    and how many carry that marker's ID."""
+    errs, ids_ok = [], 0
+    for m in markers:
+        if m["status"] != 1:
+            continue
+        d, g = min((math.hypot(m["x"] - g.center[0], m["y"] - g.center[1]), g) for g in gts)
+        errs.append(d)
+        ids_ok += int(g.id == m["id"])
+    return errs, ids_ok
+
+
+def gradient_comparison(frames, Detector, BatchDetector, steps, reps=3):
+    """1920x1080 frames with 16 markers: the plain path frame by frame
+    against the batched path, with the reference grid search and with the
+    gradient refinement (its first grid iteration, then `steps` descent
+    steps); frames/s with the frames' uploads and centre error against the
+    synthetic ground truth."""
+    if not frames:
+        return None
+    imgs = [f[0] for f in frames]
+    gts = [f[1] for f in frames]
+    h, w = imgs[0].shape
+    det = Detector(w, h)
+    plain = [det.detect(im) for im in imgs]
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for im in imgs:
+            det.detect(im)
+    plain_s = (time.perf_counter() - t0) / reps
+    det.close()
+    out = {"workload": "%d frames 1920x1080, 16 three-ring markers each" % len(imgs),
+           "gradient_steps": steps}
+
+    def run_batch(gradient_steps):
+        b = BatchDetector([(w, h)] * len(imgs))
+        b.set_refinement(gradient_steps)
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for f, im in enumerate(imgs):
+                b.load(f, im)
+            got = b.detect()
+        dt = (time.perf_counter() - t0) / reps
+        b.close()
+        return got, dt
+
+    def accuracy(res):
+        errs, ids = [], 0
+        for markers, g in zip(res, gts):
+            e, ok = gt_centre_errors(markers, g)
+            errs += e
+            ids += ok
+        return {"identified": len(errs), "ids_correct": ids,
+                "center_err_px_mean": float(np.mean(errs)) if errs else None,
+                "center_err_px_max": float(np.max(errs)) if errs else None}
+
+    grid, grid_s = run_batch(0)
+    grad, grad_s = run_batch(steps)
+    n = len(imgs)
+    out.update({
+        "plain_frames_per_s": n / plain_s, "batched_frames_per_s": n / grid_s,
+        "batched_gradient_frames_per_s": n / grad_s, "speedup_vs_plain": plain_s / grad_s,
+        "batched_identical_to_plain": grid == plain,
+        "same_ids_as_plain": all([m["id"] for m in a] == [m["id"] for m in b]
+                                 for a, b in zip(grad, plain)),
+        "accuracy": {"plain": accuracy(plain), "batched_gradient": accuracy(grad)}})
+    return out
+
+
 def refine_comparison(mosaic, Refiner, steps=20, reps=3):
     """Centre refinement of many candidate markers: batched descent on the
     differentiable score against the reference grid search; centre error
@@ -132,11 +207,14 @@ def refine_comparison(mosaic, Refiner, steps=20, reps=3):
     r.set_image(img)
     r.set_markers(ms)
     r.descend(steps)
+    r.refine(steps)
     r.grid()
-    d_us, g_us = [], []
+    d_us, f_us, g_us = [], [], []
     for _ in range(reps):
         c_desc, _ = r.descend(steps)
         d_us.append(r.times()["descend_us"])
+        c_ref, _ = r.refine(steps)
+        f_us.append(r.times()["descend_us"])
         c_grid, ok = r.grid()
         g_us.append(r.times()["grid_us"])
     r.close()
@@ -148,9 +226,12 @@ def refine_comparison(mosaic, Refiner, steps=20, reps=3):
         return {"mean": float(e.mean()), "median": float(np.median(e)),
                 "p95": float(np.percentile(e, 95))}
     return {"workload": "%d markers, %d cuts x 100 samples, fp32 sampling" % (len(ms), len(ms[0]["stops"])),
-            "steps": steps, "descend_ms": min(d_us) / 1e3, "grid_ms": min(g_us) / 1e3,
-            "speedup": min(g_us) / min(d_us), "grid_ok": int(ok.sum()),
-            "center_err_px": {"initial": err(c0), "descend": err(c_desc),
+            "steps": steps,
+            "refine": "the grid search's first iteration (5 x 5 points around the start), "
+                      "then %d descent steps from the point it picked" % steps,
+            "refine_ms": min(f_us) / 1e3, "descend_ms": min(d_us) / 1e3, "grid_ms": min(g_us) / 1e3,
+            "speedup": min(g_us) / min(f_us), "grid_ok": int(ok.sum()),
+            "center_err_px": {"initial": err(c0), "refine": err(c_ref), "descend": err(c_desc),
                               "grid": err(c_grid, ok)}}
 
 
@@ -171,6 +252,10 @@ def main():
                     help="frames of the uneven-size batch (0 skips the batched comparison)")
     ap.add_argument("--refine-markers", type=int, default=4096,
                     help="markers of the refinement comparison (0 skips it)")
+    ap.add_argument("--gradient-frames", type=int, default=16,
+                    help="1920x1080 frames of the plain / batched / batched+gradient "
+                         "comparison (0 skips it)")
+    ap.add_argument("--gradient-steps", type=int, default=20)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -182,12 +267,13 @@ def main():
     # worker processes before this process touches the GPU
     from cctag_amd import synth
     side = {}
-    if args.batch_frames > 0 or args.refine_markers > 0:
+    if args.batch_frames > 0 or args.refine_markers > 0 or args.gradient_frames > 0:
         import multiprocessing as mp
         with mp.get_context("fork").Pool(max(1, min(16, os.cpu_count() or 1))) as pool:
             mosaic = (pool.apply_async(synth.refine_mosaic, (args.refine_markers,))
                       if args.refine_markers > 0 else None)
             side["batch"] = pool.map(synth.batch_frame, range(args.batch_frames))
+            side["hd"] = pool.map(synth.frame_1080p, range(args.gradient_frames))
             side["mosaic"] = mosaic.get() if mosaic else None
         progress("synthetic inputs ready")
 
@@ -310,6 +396,9 @@ def main():
     progress("batched comparison done")
     refine = refine_comparison(side.get("mosaic"), Refiner)
     progress("refinement comparison done")
+    gradient = gradient_comparison(side.get("hd") or [], Detector, BatchDetector,
+                                   args.gradient_steps)
+    progress("batched gradient comparison done")
 
     line = {
         "metric": "1080p frames/s (16 markers), plain pipeline",
@@ -339,6 +428,7 @@ def main():
         "kernels": kernels,
         "batched": batched,
         "grad_refine": refine,
+        "batched_gradient": gradient,
     }
     det.close()
     print(json.dumps(line))

@@ -200,6 +200,8 @@ def load() -> ctypes.CDLL:
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]
     lib.cctagx_get_detect_stats.argtypes = [vp, P(DetectStats)]
     lib.cctagx_set_host_threads.argtypes = [vp, ctypes.c_int]
+    lib.cctagx_set_refinement.argtypes = [vp, ctypes.c_int]
+    lib.cctagx_batch_set_refinement.argtypes = [vp, ctypes.c_int]
     lib.cctagx_get_host_threads.argtypes = [vp]
     lib.cctagx_batch_create.argtypes = [P(Params), ctypes.c_int, P(c_i32), P(c_i32), P(vp)]
     lib.cctagx_batch_destroy.argtypes = [vp]
@@ -214,6 +216,7 @@ def load() -> ctypes.CDLL:
     lib.cctagx_refiner_cost_grad.argtypes = [vp, vp, ctypes.c_int, vp, vp]
     lib.cctagx_refiner_homography.argtypes = [vp, vp, vp]
     lib.cctagx_refiner_descend.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.cctagx_refiner_refine.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     lib.cctagx_refiner_grid.argtypes = [vp, vp, vp]
     lib.cctagx_refiner_times.argtypes = [vp, P(c_f32), P(c_f32)]
     lib.cctagx_version.restype = ctypes.c_char_p

@@ -207,6 +207,12 @@ int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, size_t st
   return CCTAGX_OK;
 }
 
+int cctagx_batch_set_refinement(cctagx_batch* b, int gradient_steps) {
+  if (!b || gradient_steps < 0 || gradient_steps > 10000) return CCTAGX_ERR_ARG;
+  b->ident.set_gradient_steps(gradient_steps);
+  return CCTAGX_OK;
+}
+
 int cctagx_batch_set_partition(cctagx_batch* b, int filter_variant, int edge_blocks) {
   if (!b || filter_variant < -1 || filter_variant >= kFilterVariants || edge_blocks < 0 ||
       edge_blocks > 65535)

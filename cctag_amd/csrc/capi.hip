@@ -210,6 +210,12 @@ int cctagx_detect_times(cctagx_handle* h, float* host_us, float* select_us, floa
   return CCTAGX_OK;
 }
 
+int cctagx_set_refinement(cctagx_handle* h, int gradient_steps) {
+  if (!h || gradient_steps < 0 || gradient_steps > 10000) return CCTAGX_ERR_ARG;
+  h->ident.set_gradient_steps(gradient_steps);
+  return CCTAGX_OK;
+}
+
 int cctagx_set_host_threads(cctagx_handle* h, int threads) {
   if (!h || threads < 0) return CCTAGX_ERR_ARG;
   if (threads == 0) threads = TaskPool::default_threads();

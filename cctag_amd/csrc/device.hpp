@@ -57,6 +57,33 @@ class DeviceArena {
   size_t size_ = 0;
 };
 
+// Device buffer that only grows.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  int ensure(size_t bytes) {
+    if (bytes <= cap_) return 0;
+    release();
+    CCTAGX_HIP(hipMalloc(&p_, bytes));
+    cap_ = bytes;
+    return 0;
+  }
+  void release() {
+    if (p_) hipFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p_); }
+
+ private:
+  void* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
 // Pinned host buffer, also mapped for the device: kernels may read it (jobs
 // the host wrote before the launch) or write it (results the host reads
 // after synchronizing; allocate such buffers with hipHostMallocCoherent).

@@ -27,7 +27,7 @@ struct cctagx_refiner {
   hipStream_t stream = nullptr;
   int max_markers = 0, n = 0;
   int w = 0, h = 0;
-  DeviceBuffer img, jobs, H, cost, grad, centers;
+  DeviceBuffer img, jobs, H, cost, grad, centers, coarse, start, coarse_of;
   PinnedBuffer h_jobs, h_io;
   GpuIdentBackend ident;
   std::vector<RefineJob> rjobs;
@@ -184,6 +184,7 @@ int cctagx_refiner_set_markers(cctagx_refiner* r, const cctagx_refine_marker* ma
     CCTAGX_HIP(hipMemcpyAsync(r->jobs.as<void>(), hj, sizeof(GradJobDev) * n, hipMemcpyHostToDevice,
                               r->stream));
     CCTAGX_HIP(hipStreamSynchronize(r->stream));
+    if (int rc = to_device(r, r->coarse_of, r->rjob_of.data(), (size_t)n)) return rc;
   }
   return CCTAGX_OK;
 }
@@ -217,7 +218,35 @@ int cctagx_refiner_descend(cctagx_refiner* r, int steps, int fp64, float* center
   if (!r->n) return CCTAGX_ERR_STATE;
   CCTAGX_HIP(hipEventRecord(r->ev[0], r->stream));
   if (launch_grad_descend(r->jobs.as<GradJobDev>(), r->n, steps, r->kc.n_samples, fp64 != 0,
-                          r->centers.as<float>(), r->cost.as<double>(), r->stream))
+                          nullptr, r->centers.as<float>(), r->cost.as<double>(), r->stream))
+    return CCTAGX_ERR_HIP;
+  CCTAGX_HIP(hipEventRecord(r->ev[1], r->stream));
+  CCTAGX_HIP(hipEventSynchronize(r->ev[1]));
+  float ms = 0.f;
+  CCTAGX_HIP(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+  r->descend_us = ms * 1000.f;
+  if (int rc = to_host(r, centers, r->centers, (size_t)2 * r->n)) return rc;
+  return to_host(r, cost, r->cost, r->n);
+}
+
+int cctagx_refiner_refine(cctagx_refiner* r, int steps, int fp64, float* centers, double* cost) {
+  if (!r || steps < 1 || steps > 10000 || !centers || !cost) return CCTAGX_ERR_ARG;
+  if (!r->n) return CCTAGX_ERR_STATE;
+  if (r->coarse.ensure(sizeof(float) * 2 * std::max<size_t>(r->rjobs.size(), 1)) ||
+      r->start.ensure(sizeof(float) * 2 * r->n))
+    return CCTAGX_ERR_HIP;
+  CCTAGX_HIP(hipEventRecord(r->ev[0], r->stream));
+  try {
+    r->ident.coarse_centers(r->rjobs, r->host, r->coarse.as<float>());
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_HIP;
+  }
+  if (launch_grad_start(r->jobs.as<GradJobDev>(), r->n, r->coarse_of.as<int>(), r->coarse.as<float>(),
+                        r->start.as<float>(), r->stream) ||
+      launch_grad_descend(r->jobs.as<GradJobDev>(), r->n, steps, r->kc.n_samples, fp64 != 0,
+                          r->start.as<float>(), r->centers.as<float>(), r->cost.as<double>(),
+                          r->stream))
     return CCTAGX_ERR_HIP;
   CCTAGX_HIP(hipEventRecord(r->ev[1], r->stream));
   CCTAGX_HIP(hipEventSynchronize(r->ev[1]));

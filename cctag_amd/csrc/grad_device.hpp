@@ -25,7 +25,13 @@ int launch_grad_eval(const GradJobDev* jobs, int n, const float* H, int n_sample
                      double* cost, double* grad, hipStream_t s);
 int launch_grad_homography(const GradJobDev* jobs, int n, const float* centers, float* H,
                            hipStream_t s);
+// start (2 per marker) = coarse[coarse_of[m]] when coarse_of[m] >= 0, else
+// the job's c0.
+int launch_grad_start(const GradJobDev* jobs, int n, const int* coarse_of, const float* coarse,
+                      float* start, hipStream_t s);
+// `steps` descent steps per marker from start (2 per marker) or, when start
+// is null, from the job's c0.
 int launch_grad_descend(const GradJobDev* jobs, int n, int steps, int n_samples, bool fp64,
-                        float* centers, double* cost, hipStream_t s);
+                        const float* start, float* centers, double* cost, hipStream_t s);
 
 }  // namespace cctagx

@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256) void k_grad_homography(const GradJobDev* __res
 template <class T>
 __global__ __launch_bounds__(kGradBlock) void k_grad_descend(const GradJobDev* __restrict__ jobs,
                                                              int steps, int n_samples,
+                                                             const float* __restrict__ start,
                                                              float* __restrict__ centers,
                                                              double* __restrict__ cost) {
   __shared__ EvalShared<T> E;
@@ -316,8 +317,8 @@ __global__ __launch_bounds__(kGradBlock) void k_grad_descend(const GradJobDev* _
   const int m = blockIdx.x, tid = threadIdx.x;
   const GradJobDev& J = jobs[m];
   if (tid == 0) {
-    cx = J.c0x;
-    cy = J.c0y;
+    cx = start ? start[2 * m] : J.c0x;
+    cy = start ? start[2 * m + 1] : J.c0y;
     best_x = cx;
     best_y = cy;
     best_cost = DBL_MAX;
@@ -373,7 +374,27 @@ __global__ __launch_bounds__(kGradBlock) void k_grad_descend(const GradJobDev* _
   }
 }
 
+// Per marker the descent's start: its coarse grid centre when it has one,
+// else the job's c0.
+__global__ __launch_bounds__(256) void k_grad_start(const GradJobDev* __restrict__ jobs, int n,
+                                                    const int* __restrict__ coarse_of,
+                                                    const float* __restrict__ coarse,
+                                                    float* __restrict__ start) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int j = coarse_of[m];
+  start[2 * m] = j >= 0 ? coarse[2 * j] : jobs[m].c0x;
+  start[2 * m + 1] = j >= 0 ? coarse[2 * j + 1] : jobs[m].c0y;
+}
+
 }  // namespace
+
+int launch_grad_start(const GradJobDev* jobs, int n, const int* coarse_of, const float* coarse,
+                      float* start, hipStream_t s) {
+  if (n <= 0) return 0;
+  k_grad_start<<<(n + 255) / 256, 256, 0, s>>>(jobs, n, coarse_of, coarse, start);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int launch_grad_eval(const GradJobDev* jobs, int n, const float* H, int n_samples, bool fp64,
                      double* cost, double* grad, hipStream_t s) {
@@ -391,10 +412,12 @@ int launch_grad_homography(const GradJobDev* jobs, int n, const float* centers, 
 }
 
 int launch_grad_descend(const GradJobDev* jobs, int n, int steps, int n_samples, bool fp64,
-                        float* centers, double* cost, hipStream_t s) {
+                        const float* start, float* centers, double* cost, hipStream_t s) {
   if (n <= 0) return 0;
-  if (fp64) k_grad_descend<double><<<n, kGradBlock, 0, s>>>(jobs, steps, n_samples, centers, cost);
-  else k_grad_descend<float><<<n, kGradBlock, 0, s>>>(jobs, steps, n_samples, centers, cost);
+  if (fp64)
+    k_grad_descend<double><<<n, kGradBlock, 0, s>>>(jobs, steps, n_samples, start, centers, cost);
+  else
+    k_grad_descend<float><<<n, kGradBlock, 0, s>>>(jobs, steps, n_samples, start, centers, cost);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -27,6 +27,7 @@ int GpuIdentBackend::init(const std::vector<std::vector<float>>& bank, const Hos
     throw std::runtime_error("marker bank larger than the identification kernels accept");
   for (hipEvent_t& e : ev_)
     if (!e) CCTAGX_HIP(hipEventCreate(&e));
+  if (!jobs_copied_) CCTAGX_HIP(hipEventCreate(&jobs_copied_));
   ensure(64, 64 * 160);
   return 0;
 }
@@ -112,21 +113,16 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
   }
 }
 
-void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParams& p,
-                             std::vector<RefineResult>& out) {
-  out.assign(jobs.size(), RefineResult());
-  const int n = (int)jobs.size();
-  if (n == 0) return;
-  ensure(n, 0);
-  refine_jobs_ = n;
-  refine_samples_ = 0;
-  const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
-  for (const RefineJob& r : jobs)
-    refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
+// Fills the pinned job array for `jobs` (distances packed job after job);
+// returns the number of distance floats.
+int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs) {
+  // the pinned array may still be the source of a queued copy
+  if (jobs_in_flight_) hip_check(hipEventSynchronize(jobs_copied_), "job copy");
+  jobs_in_flight_ = false;
   RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
   const int nb = k_.bank_rows;
   int n_dist = 0;
-  for (int i = 0; i < n; ++i) {
+  for (size_t i = 0; i < jobs.size(); ++i) {
     const RefineJob& r = jobs[i];
     RefineJobDev& d = hj[i];
     const DeviceImage& im = image(r.image);
@@ -152,15 +148,106 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
     n_dist += d.n_cuts * nb;
     for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
   }
+  return n_dist;
+}
+
+void GpuIdentBackend::coarse_centers(const std::vector<RefineJob>& jobs, const HostParams& p,
+                                     float* d_centers) {
+  const int n = (int)jobs.size();
+  if (n == 0) return;
+  ensure(n, 0);
+  pack_refine_jobs(jobs);
+  hip_check(hipMemcpyAsync(d_ref_jobs_, h_ref_jobs_.as<RefineJobDev>(), sizeof(RefineJobDev) * n,
+                           hipMemcpyHostToDevice, stream_),
+            "H2D");
+  hip_check(hipEventRecord(jobs_copied_, stream_), "event");
+  jobs_in_flight_ = true;
+  if (launch_refine_coarse(d_ref_jobs_, n, k_.n_samples, p.imaged_center_n_grid_sample, d_work_,
+                           d_centers, stream_) != 0)
+    throw std::runtime_error("k_refine coarse launch failed");
+}
+
+// Gradient refinement of the jobs already on the device: the grid search's
+// first iteration, grad_steps_ descent steps from the centre it picked, the
+// closed-form homography there and the read-out (results into the same
+// mapped buffers as the grid search's).
+void GpuIdentBackend::refine_gradient(const std::vector<RefineJob>& jobs, const HostParams& p) {
+  const int n = (int)jobs.size();
+  hipStream_t s = stream_;
+  if (h_grad_jobs_.ensure(sizeof(GradJobDev) * n) || d_grad_jobs_.ensure(sizeof(GradJobDev) * n) ||
+      d_coarse_.ensure(sizeof(float) * 2 * n) || d_gcenters_.ensure(sizeof(float) * 2 * n) ||
+      d_gcost_.ensure(sizeof(double) * n) || d_gH_.ensure(sizeof(float) * 9 * n))
+    throw std::runtime_error(std::string("gradient refinement buffers: ") + last_error());
+  GradJobDev* g = h_grad_jobs_.as<GradJobDev>();
+  for (int i = 0; i < n; ++i) {
+    const RefineJob& r = jobs[i];
+    GradJobDev& d = g[i];
+    const DeviceImage& im = image(r.image);
+    d.img = im.p;
+    d.iw = im.w;
+    d.ih = im.h;
+    d.Q11 = r.canonic.m[0][0];
+    d.Q22 = r.canonic.m[1][1];
+    d.Q33 = r.canonic.m[2][2];
+    copy9(d.t_can, r.t_can);
+    copy9(d.t_inv_can, r.t_inv_can);
+    d.begin_sig = r.begin_sig;
+    d.end_sig = r.end_sig;
+    // a quarter of the grid search's first neighbourhood
+    d.alpha0 = 0.25f * r.neighbour0 * r.max_semi_axis;
+    d.n_cuts = (int)r.stops.size();
+    d.c0x = r.center0.x;
+    d.c0y = r.center0.y;
+    for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
+  }
+  hip_check(hipMemcpyAsync(d_grad_jobs_.as<void>(), g, sizeof(GradJobDev) * n,
+                           hipMemcpyHostToDevice, s),
+            "H2D");
+  const int ns = k_.n_samples;
+  if (launch_refine_coarse(d_ref_jobs_, n, ns, p.imaged_center_n_grid_sample, d_work_,
+                           d_coarse_.as<float>(), s) != 0 ||
+      launch_grad_descend(d_grad_jobs_.as<GradJobDev>(), n, grad_steps_, ns, false,
+                          d_coarse_.as<float>(), d_gcenters_.as<float>(), d_gcost_.as<double>(),
+                          s) != 0 ||
+      launch_grad_homography(d_grad_jobs_.as<GradJobDev>(), n, d_gcenters_.as<float>(),
+                             d_gH_.as<float>(), s) != 0 ||
+      launch_refine_readout(d_ref_jobs_, n, ns, d_gH_.as<float>(), d_gcenters_.as<float>(),
+                            d_gcost_.as<double>(), d_inv_ratio_, k_.bank_rows, k_.bank_cols,
+                            d_work_, h_ref_out_.dev<RefineOutDev>(), h_ref_dist_.dev<float>(),
+                            s) != 0)
+    throw std::runtime_error("gradient refinement launch failed");
+}
+
+void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParams& p,
+                             std::vector<RefineResult>& out) {
+  out.assign(jobs.size(), RefineResult());
+  const int n = (int)jobs.size();
+  if (n == 0) return;
+  ensure(n, 0);
+  refine_jobs_ = n;
+  refine_samples_ = 0;
+  const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
+  for (const RefineJob& r : jobs)
+    refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
+  RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
+  const int nb = k_.bank_rows;
+  pack_refine_jobs(jobs);
   hipStream_t s = stream_;
   hip_check(hipMemcpyAsync(d_ref_jobs_, hj, sizeof(RefineJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
+
+  hip_check(hipEventRecord(jobs_copied_, s), "event");
+  jobs_in_flight_ = true;
   hipEventRecord(ev_[2], s);
-  int max_iter = 0;
-  for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
-  if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
-                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_,
-                    h_ref_out_.dev<RefineOutDev>(), h_ref_dist_.dev<float>(), s) != 0)
-    throw std::runtime_error("k_refine launch failed");
+  if (grad_steps_ > 0) {
+    refine_gradient(jobs, p);
+  } else {
+    int max_iter = 0;
+    for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
+    if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
+                      d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_,
+                      h_ref_out_.dev<RefineOutDev>(), h_ref_dist_.dev<float>(), s) != 0)
+      throw std::runtime_error("k_refine launch failed");
+  }
   hipEventRecord(ev_[3], s);
   const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
   const float* hd = h_ref_dist_.as<float>();

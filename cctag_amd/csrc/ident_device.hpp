@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "grad_device.hpp"
 #include "host/ident.hpp"
 
 namespace cctagx {
@@ -78,6 +79,16 @@ struct RefineWork {
 
 // out: one header per job; dist: per job n_cuts x bank_rows distances from
 // its dist_offset on (invalid cuts' rows are not written).
+// The grid search's first iteration only: per job the centre it moves to
+// (centers: 2 per job, device memory).
+int launch_refine_coarse(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
+                         const RefineWork& work, float* centers, hipStream_t s);
+// The read-out of launch_refine's last kernel at given homographies (9 per
+// job), centres (2 per job) and scores instead of the grid search's choice.
+int launch_refine_readout(const RefineJobDev* jobs, int n_jobs, int n_samples, const float* H,
+                          const float* centers, const double* cost, const float* inv_ratio,
+                          int bank_rows, int bank_cols, const RefineWork& work, RefineOutDev* out,
+                          float* dist, hipStream_t s);
 int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samples, int grid_n,
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, float* dist, hipStream_t s);
@@ -100,6 +111,14 @@ class GpuIdentBackend : public IdentBackend {
                    std::vector<CutSelection>& out) override;
   void refine(const std::vector<RefineJob>& jobs, const HostParams& p,
               std::vector<RefineResult>& out) override;
+  // 0: the reference grid search (the default); steps > 0: its first
+  // iteration, then `steps` descent steps on the differentiable score from
+  // the centre it picked, and the read-out there.
+  void set_gradient_steps(int steps) { grad_steps_ = steps; }
+  int gradient_steps() const { return grad_steps_; }
+  // The centres after the grid search's first iteration, into device memory
+  // (2 floats per job), on the backend's stream without synchronizing.
+  void coarse_centers(const std::vector<RefineJob>& jobs, const HostParams& p, float* d_centers);
   // Device time of the last select / refine launches, microseconds.
   float last_select_us() const { return select_us_; }
   float last_refine_us() const { return refine_us_; }
@@ -116,6 +135,7 @@ class GpuIdentBackend : public IdentBackend {
 
  private:
   void ensure(int n_markers, int n_points);
+  int pack_refine_jobs(const std::vector<RefineJob>& jobs);
   hipStream_t stream_ = nullptr;
   std::vector<DeviceImage> images_;
   const DeviceImage& image(int i) const;
@@ -127,6 +147,12 @@ class GpuIdentBackend : public IdentBackend {
   RefineWork d_work_{};
   float* d_inv_ratio_ = nullptr;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t jobs_copied_ = nullptr;  // the last copy out of h_ref_jobs_
+  bool jobs_in_flight_ = false;
+  int grad_steps_ = 0;
+  void refine_gradient(const std::vector<RefineJob>& jobs, const HostParams& p);
+  PinnedBuffer h_grad_jobs_;
+  DeviceBuffer d_grad_jobs_, d_coarse_, d_gcenters_, d_gcost_, d_gH_;
   float select_us_ = 0, refine_us_ = 0;
   int select_jobs_ = 0, refine_jobs_ = 0;
   long long select_samples_ = 0, refine_samples_ = 0;

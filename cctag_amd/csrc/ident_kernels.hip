@@ -730,15 +730,24 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_init(const RefineJobDev
 // Signals at the last iteration's best grid point, then the per-cut
 // statistics and distances of orazioDistanceRobust (Identification.cpp:
 // 88-160).
+// The identification read-out at one homography per marker: the grid
+// search's choice after its last iteration (given == null), or a given
+// homography with its centre and score (given: 9 floats per job, centres 2
+// per job, scores), whose out-of-bounds cuts are the ones leaving the image
+// there.  Signals, median profile, orazio statistics, distance to every
+// bank profile.
 __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
     const float* __restrict__ inv_ratio, int bank_rows, int bank_cols, RefineWork W,
-    RefineOutDev* __restrict__ out, float* __restrict__ dist) {
+    const float* __restrict__ given_H, const float* __restrict__ given_c,
+    const double* __restrict__ given_cost, RefineOutDev* __restrict__ out,
+    float* __restrict__ dist) {
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ float ordered[kMaxSamples][kMaxCuts];  // per sample, the valid cuts' values sorted
   __shared__ float bar[kMaxSamples];
   __shared__ int valid_cut[kMaxCuts];
   __shared__ int n_valid;
+  __shared__ int oob_at[kMaxCuts];
   __shared__ CutWalk walk[kMaxCuts];
   __shared__ float sH[9], sHi[9];
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
@@ -757,32 +766,50 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   const int m = blockIdx.x, tid = threadIdx.x, n_jobs = gridDim.x;
   const RefineJobDev& J = jobs[m];
   RefineOutDev& O = out[m];
-  if (J.n_iter == 0) {
+  const bool given = given_H != nullptr;
+  if (!given && J.n_iter == 0) {
     if (tid == 0) O.ok = 0;
     return;
   }
   for (int q = tid; q < bank_rows * kMaxRatios; q += kRefineBlock) ratio[q] = inv_ratio[q];
-  // the selection after the last iteration
-  state_at(J, J.n_iter, m, n_jobs, grid_n, W, P, S);
+  const int nc = J.n_cuts;
+  if (given) {
+    if (tid == 0) {
+      const double c = given_cost[m];
+      S.ok = c < DBL_MAX && isfinite(c);
+      S.cx = given_c[2 * m];
+      S.cy = given_c[2 * m + 1];
+      S.min_res = (float)c;
+      for (int r = 0; r < 9; ++r) sH[r] = given_H[9 * m + r];
+      inverse3(sH, sHi);
+    }
+    if (tid < kMaxCuts) oob_at[tid] = 0;
+    __syncthreads();
+  } else {
+    // the selection after the last iteration
+    state_at(J, J.n_iter, m, n_jobs, grid_n, W, P, S);
+    if (S.ok && tid < 9) {
+      const RefineGridDev& GB =
+          W.grid[((size_t)((J.n_iter - 1) & 1) * n_jobs + m) * G2MAX + S.best_g];
+      sH[tid] = GB.H[tid];
+      sHi[tid] = GB.Hi[tid];
+    }
+    __syncthreads();
+  }
   if (!S.ok) {
     if (tid == 0) O.ok = 0;
     return;
   }
-  const int nc = J.n_cuts;
   const Img im{J.img, J.iw, J.ih};
-  const RefineGridDev& GB = W.grid[((size_t)((J.n_iter - 1) & 1) * n_jobs + m) * G2MAX + S.best_g];
-  if (tid < 9) {
-    sH[tid] = GB.H[tid];
-    sHi[tid] = GB.Hi[tid];
-  }
-  __syncthreads();
   if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
   __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
   for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
     const int k = w / nchunk, c = w - k * nchunk;
-    sample_chunk(im, sH, walk[k], n_samples, c, sig[k]);
+    if (sample_chunk(im, sH, walk[k], n_samples, c, sig[k]) && given) atomicOr(&oob_at[k], 1);
   }
+  __syncthreads();
+  if (given && tid < nc) S.sticky[tid] = oob_at[tid] != 0;
   __syncthreads();
   if (tid == 0) {
     O.ok = 1;
@@ -907,7 +934,42 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   }
 }
 
+// The centre the grid search moves to after its first iteration (the job's
+// start when that iteration found no readable grid point).
+__global__ __launch_bounds__(256) void k_refine_coarse_out(const RefineJobDev* __restrict__ jobs,
+                                                           int n_jobs, RefineWork W,
+                                                           float* __restrict__ centers) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_jobs) return;
+  const RefineJobDev& J = jobs[m];
+  const RefineStateDev& S = W.state[(size_t)n_jobs + m];  // parity of iteration 1
+  const bool moved = J.n_iter >= 1 && S.ok;
+  centers[2 * m] = moved ? S.cx : J.c0x;
+  centers[2 * m + 1] = moved ? S.cy : J.c0y;
+}
+
 }  // namespace
+
+int launch_refine_coarse(const RefineJobDev* jobs, int n_jobs, int n_samples, int grid_n,
+                         const RefineWork& work, float* centers, hipStream_t s) {
+  if (n_jobs <= 0) return 0;
+  const dim3 grid(grid_n * grid_n, n_jobs);
+  k_refine_init<<<n_jobs, kRefineBlock, 0, s>>>(jobs, grid_n, work);
+  k_refine_iter<<<grid, kRefineBlock, 0, s>>>(jobs, 0, n_samples, grid_n, 0, work);
+  k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, 0, grid_n, work);
+  k_refine_coarse_out<<<(n_jobs + 255) / 256, 256, 0, s>>>(jobs, n_jobs, work, centers);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_refine_readout(const RefineJobDev* jobs, int n_jobs, int n_samples, const float* H,
+                          const float* centers, const double* cost, const float* inv_ratio,
+                          int bank_rows, int bank_cols, const RefineWork& work, RefineOutDev* out,
+                          float* dist, hipStream_t s) {
+  if (n_jobs <= 0) return 0;
+  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, kMaxGrid, inv_ratio, bank_rows,
+                                                 bank_cols, work, H, centers, cost, out, dist);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
                        int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s) {
@@ -929,7 +991,8 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
       k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work);
   }
   k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
-                                                 bank_cols, work, out, dist);
+                                                 bank_cols, work, nullptr, nullptr, nullptr, out,
+                                                 dist);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

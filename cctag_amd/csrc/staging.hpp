@@ -15,33 +15,6 @@
 
 namespace cctagx {
 
-// Device buffer that only grows.
-class DeviceBuffer {
- public:
-  DeviceBuffer() = default;
-  DeviceBuffer(const DeviceBuffer&) = delete;
-  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  ~DeviceBuffer() { release(); }
-  int ensure(size_t bytes) {
-    if (bytes <= cap_) return 0;
-    release();
-    CCTAGX_HIP(hipMalloc(&p_, bytes));
-    cap_ = bytes;
-    return 0;
-  }
-  void release() {
-    if (p_) hipFree(p_);
-    p_ = nullptr;
-    cap_ = 0;
-  }
-  template <class T>
-  T* as() const { return static_cast<T*>(p_); }
-
- private:
-  void* p_ = nullptr;
-  size_t cap_ = 0;
-};
-
 // One contiguous piece to gather: `bytes` from `src` to packed offset `dst`
 // (a multiple of 16).
 struct CopySeg {

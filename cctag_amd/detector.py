@@ -84,6 +84,11 @@ class Detector:
                                                  ctypes.byref(ref)), "cctagx_detect_times")
         return {"host_us": host.value, "select_us": sel.value, "refine_us": ref.value}
 
+    def set_refinement(self, gradient_steps: int = 0) -> None:
+        """0: the reference grid search; n > 0: its first iteration, then n
+        descent steps on the differentiable score (cctagx_set_refinement)."""
+        _lib.check(self._lib.cctagx_set_refinement(self._h, gradient_steps), "cctagx_set_refinement")
+
     def set_host_threads(self, threads: int) -> None:
         """Candidate-stage host threads (0 = default); results do not change."""
         _lib.check(self._lib.cctagx_set_host_threads(self._h, threads), "cctagx_set_host_threads")
@@ -168,6 +173,11 @@ class BatchDetector:
             raise ValueError("frame %d must be a %dx%d uint8 image" % (frame, w, h))
         _lib.check(self._lib.cctagx_batch_load(self._h, frame, img.ctypes.data, img.strides[0]),
                    "cctagx_batch_load")
+
+    def set_refinement(self, gradient_steps: int = 0) -> None:
+        """As Detector.set_refinement, for every frame of the batch."""
+        _lib.check(self._lib.cctagx_batch_set_refinement(self._h, gradient_steps),
+                   "cctagx_batch_set_refinement")
 
     def set_partition(self, filter_variant: int = -1, edge_blocks: int = 0) -> None:
         _lib.check(self._lib.cctagx_batch_set_partition(self._h, filter_variant, edge_blocks),
@@ -260,6 +270,14 @@ class Refiner:
         _lib.check(self._lib.cctagx_refiner_homography(self._h, c.ctypes.data, H.ctypes.data),
                    "cctagx_refiner_homography")
         return H.reshape(self.n, 3, 3)
+
+    def refine(self, steps: int = 20, fp64: bool = False):
+        """The grid search's first iteration, then `steps` descent steps."""
+        c = np.zeros((self.n, 2), np.float32)
+        cost = np.zeros(self.n, np.float64)
+        _lib.check(self._lib.cctagx_refiner_refine(self._h, steps, int(fp64), c.ctypes.data,
+                                                   cost.ctypes.data), "cctagx_refiner_refine")
+        return c, cost
 
     def descend(self, steps: int = 20, fp64: bool = False):
         c = np.zeros((self.n, 2), np.float32)

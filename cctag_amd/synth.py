@@ -273,6 +273,12 @@ def config_frame(config: int, k: int = 0):
     return make_frame(CONFIGS[config], 1000 * config + k)
 
 
+def frame_1080p(k: int):
+    """Frame k of the 1920x1080 16-marker configuration (frame 0 is the one
+    the plain-path benchmark detects)."""
+    return config_frame(2, k)
+
+
 def batch_frame(k: int, seed: int = 4000):
     """Frame k of the uneven-size batch (size drawn from BATCH_SIZES)."""
     rng = np.random.Generator(np.random.PCG64(seed + k))

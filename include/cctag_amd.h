@@ -147,6 +147,15 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
 int cctagx_set_host_threads(cctagx_handle* h, int threads);
 int cctagx_get_host_threads(cctagx_handle* h);
 
+/* Centre refinement of the identification (no counterpart in the reference,
+ * which always runs the grid search of refineConicFamilyGlob,
+ * Identification.cpp:813-1158): 0, the default, runs that grid search;
+ * steps > 0 runs its first iteration and then `steps` descent steps on the
+ * differentiable score (cctagx_refiner_refine) and reads the ID out at the
+ * centre reached.  Centres then differ from the grid search's by a fraction
+ * of a pixel and an ID can in rare cases differ. */
+int cctagx_set_refinement(cctagx_handle* h, int gradient_steps);
+
 /* Breakdown of the last cctagx_detect: host wall time after the front end
  * (staging, candidate stage, identification control) and device time of the
  * two identification kernels, microseconds.  The reference's logtime::Mgmt
@@ -200,6 +209,8 @@ int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, size_t st
  * kernels.  -1 / 0 leave the choice to the autotuner, which times the
  * candidates on the first detection's frames before capturing the graph. */
 int cctagx_batch_set_partition(cctagx_batch* b, int filter_variant, int edge_blocks);
+/* cctagx_set_refinement for the batch. */
+int cctagx_batch_set_refinement(cctagx_batch* b, int gradient_steps);
 /* Detects every loaded frame.  Writes up to max_out markers, frame by frame
  * (each frame sorted by id); frame_first[f] .. frame_first[f+1] (n_frames + 1
  * entries) index frame f's markers, and *n_out is the total, which may
@@ -254,9 +265,14 @@ int cctagx_refiner_cost_grad(cctagx_refiner* r, const float* H, int fp64, double
 int cctagx_refiner_homography(cctagx_refiner* r, const float* centers, float* H);
 /* `steps` batched descent steps from center0: centres (n x 2) and scores (n). */
 int cctagx_refiner_descend(cctagx_refiner* r, int steps, int fp64, float* centers, double* cost);
+/* The grid search's first iteration (its 5 x 5 neighbourhood of center0,
+ * the centre it moves to), then `steps` descent steps from there: centres
+ * (n x 2) and scores (n).  The coarse step keeps the descent in the right
+ * basin when center0 is several pixels off. */
+int cctagx_refiner_refine(cctagx_refiner* r, int steps, int fp64, float* centers, double* cost);
 /* The reference grid search on the same markers: centres (n x 2) and ok flags (n). */
 int cctagx_refiner_grid(cctagx_refiner* r, float* centers, int32_t* ok);
-/* Device time of the last descend and grid calls, microseconds. */
+/* Device time of the last descend (or refine) and grid calls, microseconds. */
 int cctagx_refiner_times(cctagx_refiner* r, float* descend_us, float* grid_us);
 
 /* ---- timing ------------------------------------------------------------

@@ -46,3 +46,22 @@ def test_batch_partitions_agree(fv, eb):
         b.set_partition(fv, eb)
         print("partition", fv, eb, flush=True)
         assert b.detect() == ref
+
+
+@pytest.mark.gpu
+def test_batch_gradient_refinement_matches_plain_gradient_refinement():
+    """Batched detection with the gradient refinement gives every frame what
+    the plain path gives it alone with the same refinement."""
+    imgs = frames.small_batch()
+    ref = []
+    for img in imgs:
+        h, w = img.shape
+        with Detector(w, h) as det:
+            det.set_refinement(20)
+            ref.append(det.detect(img))
+    with BatchDetector([(im.shape[1], im.shape[0]) for im in imgs]) as b:
+        b.set_refinement(20)
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+    assert got == ref

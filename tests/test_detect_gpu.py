@@ -86,3 +86,37 @@ def test_detect_host_threads_do_not_change_result():
         assert det.host_threads() == 1
         b = det.detect(img)
     assert a == b
+
+
+GRADIENT_CENTER_TOL_PX = 0.25  # gradient refinement vs grid search, imaged centre
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", [1, 2])
+def test_gradient_refinement_mode(config):
+    """The identification's first grid iteration plus 20 descent steps in
+    place of the full grid search: the same markers, IDs and statuses,
+    centres within a fraction of a pixel of the grid search's and no further
+    from the ground truth on average; switching back restores the reference
+    result exactly."""
+    img, gts = frames.config_frame(config)
+    print("gradient refinement config", config, img.shape, flush=True)
+    h, w = img.shape
+    with Detector(w, h) as det:
+        grid = det.detect(img)
+        det.set_refinement(20)
+        grad = det.detect(img)
+        det.set_refinement(0)
+        again = det.detect(img)
+    assert again == grid
+    assert len(grad) == len(grid)
+    for g, r in zip(grad, grid):
+        assert g["id"] == r["id"] and g["status"] == r["status"], (g, r)
+        if r["status"] == 1:
+            d = math.hypot(g["x"] - r["x"], g["y"] - r["y"])
+            assert d <= GRADIENT_CENTER_TOL_PX, ("centre", d, g, r)
+    e_grid, ok_grid = gt_errors(grid, gts)
+    e_grad, ok_grad = gt_errors(grad, gts)
+    print("centre error vs ground truth: grid", np.mean(e_grid), "gradient", np.mean(e_grad), flush=True)
+    assert ok_grad == ok_grid == len(e_grid)
+    assert np.mean(e_grad) <= np.mean(e_grid) + 0.02

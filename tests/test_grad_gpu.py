@@ -8,6 +8,7 @@ import math
 import numpy as np
 import pytest
 
+from cctag_amd import synth
 from cctag_amd.detector import Refiner
 from tests import frames, grad_data, oracle_lib
 from tests.test_grad_cpu import fd_grad
@@ -98,3 +99,26 @@ def test_descent_against_ground_truth_and_grid_search(config):
     assert eg.mean() < e0.mean()
     assert eg.mean() <= er.mean() + 0.05
     assert eg.max() < 0.5
+
+
+@pytest.mark.gpu
+def test_coarse_start_refinement_on_a_mosaic():
+    """The grid search's first iteration followed by 20 descent steps reaches
+    the grid search's centre accuracy on a mosaic of strongly tilted markers,
+    including those whose starting centre is pixels off, where plain descent
+    can settle in another basin."""
+    img, ms = synth.refine_mosaic(512)
+    gt = np.array([m["center_gt"] for m in ms])
+    r = Refiner(len(ms))
+    with r:
+        r.set_image(img)
+        r.set_markers(ms)
+        c_ref, cost = r.refine(20)
+        c_grid, ok = r.grid()
+    e_ref = np.hypot(*(np.asarray(c_ref, np.float64) - gt).T)
+    e_grid = np.hypot(*(np.asarray(c_grid, np.float64) - gt).T)
+    print("coarse+descent", e_ref.mean(), e_ref.max(), "grid", e_grid[ok].mean(), e_grid[ok].max(),
+          flush=True)
+    assert ok.all() and np.isfinite(cost).all()
+    assert e_ref.mean() <= e_grid.mean() * 1.05
+    assert e_ref.max() < 0.5
