@@ -169,8 +169,9 @@ def gradient_comparison(frames, Detector, BatchDetector, steps, reps=3):
                 b.load(f, im)
             got = b.detect()
         dt = (time.perf_counter() - t0) / reps
+        refine_us = b.stats()["refine_us"]
         b.close()
-        return got, dt
+        return got, dt, refine_us
 
     def accuracy(res):
         errs, ids = [], 0
@@ -182,12 +183,13 @@ def gradient_comparison(frames, Detector, BatchDetector, steps, reps=3):
                 "center_err_px_mean": float(np.mean(errs)) if errs else None,
                 "center_err_px_max": float(np.max(errs)) if errs else None}
 
-    grid, grid_s = run_batch(0)
-    grad, grad_s = run_batch(steps)
+    grid, grid_s, grid_us = run_batch(0)
+    grad, grad_s, grad_us = run_batch(steps)
     n = len(imgs)
     out.update({
         "plain_frames_per_s": n / plain_s, "batched_frames_per_s": n / grid_s,
         "batched_gradient_frames_per_s": n / grad_s, "speedup_vs_plain": plain_s / grad_s,
+        "refine_us_per_batch": {"grid": grid_us, "gradient": grad_us},
         "batched_identical_to_plain": grid == plain,
         "same_ids_as_plain": all([m["id"] for m in a] == [m["id"] for m in b]
                                  for a, b in zip(grad, plain)),
@@ -255,7 +257,9 @@ def main():
     ap.add_argument("--gradient-frames", type=int, default=16,
                     help="1920x1080 frames of the plain / batched / batched+gradient "
                          "comparison (0 skips it)")
-    ap.add_argument("--gradient-steps", type=int, default=20)
+    ap.add_argument("--gradient-steps", type=int, default=30,
+                    help="descent steps of the batched+gradient detection (after the grid "
+                         "search's first iteration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
