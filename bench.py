@@ -122,6 +122,19 @@ def batched_comparison(frames, Detector, BatchDetector, reps=3):
                                                             "candidates_us", "ident_us")}}
 
 
+def measured_traffic(path: str):
+    """HBM bytes per detection of the refinement sequence from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against a copy), or
+    None when the summary is absent."""
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            if line.startswith("traffic_bytes_per_detection:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def gt_centre_errors(markers, gts):
     """Distance of every identified marker to the nearest ground-truth centre,
     and how many carry that marker's ID."""
@@ -359,7 +372,9 @@ def main():
         achieved = flops / (avg[dom] * 1e-6) / 1e12
         roof = {"kernel": "k_refine", "bound": "mfma", "achieved": achieved,
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": None, "avg_us": avg[dom], "algorithmic_per_launch": flops}
+                "traffic": measured_traffic("profiles/refine_1080p16_pmc.md"),
+                "traffic_source": "profiles/refine_1080p16_pmc.md",
+                "avg_us": avg[dom], "algorithmic_per_launch": flops}
     elif dom == "select_cuts":
         flops = work["select_samples"] / K * FLOP_PER_SELECT_SAMPLE
         achieved = flops / (avg[dom] * 1e-6) / 1e12

@@ -320,6 +320,7 @@ constexpr int kLoadBatch = 5;     // samples whose image reads are issued togeth
 static_assert(kRefineChunk % kLoadBatch == 0, "whole load batches per chunk");
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
 constexpr int kFuseMaxBlocks = 2048;  // grid blocks up to which the selection is fused
+constexpr int kXcds = 8;               // accelerator complex dies of the MI355X
 
 // getNearbyPoints point g + computeHomographyFromEllipseAndImagedCenter
 // (Identification.cpp:739-798, 1071-1109) and the inverse.
@@ -635,7 +636,19 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   PickShared& P = U.P;
   auto& sig = U.sig;
 
-  const int g = blockIdx.x, m = blockIdx.y, tid = threadIdx.x, n_jobs = gridDim.y;
+  // Workgroups are dealt to the 8 XCDs in turn, each XCD with its own L2:
+  // renumber them so that an XCD runs consecutive (grid point, marker)
+  // pairs, i.e. whole markers, and each marker's image area is fetched into
+  // one L2 only.
+  const int n_jobs = gridDim.y, tid = threadIdx.x;
+  int g, m;
+  {
+    const int n = gridDim.x * gridDim.y, hw = blockIdx.y * gridDim.x + blockIdx.x;
+    const int x = hw % kXcds, k = hw / kXcds, q = n / kXcds, r = n % kXcds;
+    const int logical = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+    g = logical % gridDim.x;
+    m = logical / gridDim.x;
+  }
   const RefineJobDev& J = jobs[m];
   if (it >= J.n_iter) return;
   const int cb = it & 1;
