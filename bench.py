@@ -135,6 +135,39 @@ def measured_traffic(path: str):
     return None
 
 
+def pipes_comparison(img, Detector, ref, configs=((2, 8), (4, 4)), steps=40):
+    """Several pipes detecting at once from their own host threads (the
+    reference's pipeId: one pipe per camera or stream), the frame resident in
+    each; whole-job frames/s and whether every detection equals `ref`."""
+    import threading
+    h, w = img.shape
+    out = []
+    for pipes, threads in configs:
+        dets = [Detector(w, h) for _ in range(pipes)]
+        for d in dets:
+            d.set_host_threads(threads)
+            d.load(img)
+            d.detect_loaded()
+        results = [[] for _ in dets]
+
+        def run(d, res):
+            for _ in range(steps):
+                res.append(d.detect_loaded())
+        ths = [threading.Thread(target=run, args=(d, r)) for d, r in zip(dets, results)]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        dt = time.perf_counter() - t0
+        for d in dets:
+            d.close()
+        out.append({"pipes": pipes, "host_threads_per_pipe": threads,
+                    "frames_per_s": pipes * steps / dt,
+                    "identical": all(x == ref for r in results for x in r)})
+    return out
+
+
 def gt_centre_errors(markers, gts):
     """Distance of every identified marker to the nearest ground-truth centre,
     and how many carry that marker's ID."""
@@ -345,6 +378,9 @@ def main():
             work[k] += ds[k]
     det.set_stage_timing(False)
 
+    pipes = pipes_comparison(img, Detector, markers)
+    progress("concurrent pipes done")
+
     # the drop-in call with the frame in host memory (upload included)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -448,6 +484,7 @@ def main():
         "batched": batched,
         "grad_refine": refine,
         "batched_gradient": gradient,
+        "concurrent_pipes": pipes,
     }
     det.close()
     print(json.dumps(line))
