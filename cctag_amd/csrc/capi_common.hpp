@@ -18,6 +18,8 @@ inline int validate_params(const cctagx_params* p) {
   if (p->number_of_processed_multires_layers < 1 || p->number_of_processed_multires_layers > 4)
     return CCTAGX_ERR_UNSUPPORTED;
   if (p->dist_search < 0 || p->min_votes_to_select_candidate < 1) return CCTAGX_ERR_ARG;
+  // the linking's convexity test reads the first and last of the window
+  if (p->window_size_on_inner_elliptic_segment < 1) return CCTAGX_ERR_ARG;
   // fixed bounds of the identification kernels and of the candidate stage's
   // 64-bit per-candidate masks
   if (p->imaged_center_n_grid_sample < 3 || p->imaged_center_n_grid_sample > kMaxGrid ||

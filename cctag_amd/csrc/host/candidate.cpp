@@ -190,13 +190,74 @@ class VisitedSet {
   size_t count_ = 0;
 };
 
+// The segment of one linking: the seed, at most 100 points appended on one
+// side and 100 prepended on the other (edge_linking_dir's step bound), in a
+// fixed buffer; and the points it flags (the seed, then each direction's
+// share of the segment).
+class SegBuf {
+ public:
+  void push_back(int v) {
+    if (tail_ == kCap) throw std::length_error("edge linking segment");
+    buf_[tail_++] = v;
+  }
+  void push_front(int v) {
+    if (head_ == 0) throw std::length_error("edge linking segment");
+    buf_[--head_] = v;
+  }
+  size_t size() const { return (size_t)(tail_ - head_); }
+  const int* begin() const { return buf_ + head_; }
+  const int* end() const { return buf_ + tail_; }
+
+ private:
+  static constexpr int kCap = 256;
+  int buf_[kCap];
+  int head_ = kCap / 2, tail_ = kCap / 2;
+};
+
+class MarkBuf {
+ public:
+  void push_back(int v) {
+    if (n_ == kCap) throw std::length_error("edge linking marks");
+    buf_[n_++] = v;
+  }
+  const int* begin() const { return buf_; }
+  const int* end() const { return buf_ + n_; }
+
+ private:
+  static constexpr int kCap = 1 + 2 * 256;
+  int buf_[kCap];
+  int n_ = 0;
+};
+
+// The last `window` gradient angles of a linking direction (a ring).
+class AngleWindow {
+ public:
+  explicit AngleWindow(size_t window) : w_(window) {
+    thread_local std::vector<float> storage;
+    if (storage.size() < w_) storage.resize(w_);
+    v_ = storage.data();
+  }
+  void push(float a) {
+    v_[(first_ + n_) % w_] = a;
+    if (n_ < w_) ++n_;
+    else first_ = (first_ + 1) % w_;
+  }
+  size_t size() const { return n_; }
+  float front() const { return v_[first_]; }
+  float back() const { return v_[(first_ + n_ - 1) % w_]; }
+
+ private:
+  size_t w_, first_ = 0, n_ = 0;
+  float* v_;
+};
+
 // Vote.cpp:255-394: convex edge linking from a seed in both directions.
 // Points the reference flags processed_in are appended to `marks` (the
 // flags themselves are set by the caller, in seed order).
 static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, int dir,
-                             std::deque<int>& seg, size_t window, float average_vote_min,
-                             std::vector<int>& marks) {
-  std::deque<float> phi;
+                             SegBuf& seg, size_t window, float average_vote_min,
+                             MarkBuf& marks) {
+  AngleWindow phi(window);
   size_t i = 0;
   bool found = true;
   int stop = 0;
@@ -206,8 +267,7 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   const int yoff[8] = {0, dir * -1, dir * -1, dir * -1, 0, dir * 1, dir * 1, dir * 1};
   while ((i < maxLength) && found && (averageVote >= average_vote_min)) {
     const float angle = es.gradient_angle(p);
-    phi.push_back(angle);
-    if (phi.size() > window) phi.pop_front();
+    phi.push(angle);
     const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
     int j = 0;
     stop = 0;
@@ -257,8 +317,8 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   }
 }
 
-static void edge_linking(const EdgeSet& es, std::deque<int>& seg, int pmax, size_t window,
-                         float average_vote_min, std::vector<int>& marks) {
+static void edge_linking(const EdgeSet& es, SegBuf& seg, int pmax, size_t window,
+                         float average_vote_min, MarkBuf& marks) {
   VisitedSet processed;
   if (pmax >= 0) {
     seg.push_back(pmax);
@@ -270,7 +330,7 @@ static void edge_linking(const EdgeSet& es, std::deque<int>& seg, int pmax, size
 }
 
 // Vote.cpp:396-412.
-static void children_of(const EdgeSet& es, const std::deque<int>& edges, std::vector<int>& children) {
+static void children_of(const EdgeSet& es, const std::vector<int>& edges, std::vector<int>& children) {
   size_t voteMax = 1;
   for (int e : edges) voteMax = std::max((int)voteMax, es.voters_size(e));
   for (int e : edges) {
@@ -631,7 +691,7 @@ static bool add_candidate_flow(const EdgeSet& es, const std::vector<int>& fc,
 namespace {
 struct Candidate {
   int seed = -1;
-  std::deque<int> convex_segment;
+  std::vector<int> convex_segment;
   std::vector<int> outer_points;
   Ellipse outer;
   std::vector<int> filtered_children;
@@ -728,8 +788,8 @@ static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::ve
 // seed's linking is then applied in seed order.
 namespace {
 struct SeedLink {
-  std::deque<int> seg;
-  std::vector<int> marks;  // points the linking flags processed_in
+  SegBuf seg;
+  MarkBuf marks;  // points the linking flags processed_in
   std::exception_ptr error;
 };
 // A seed's linking in loop one.
@@ -747,7 +807,7 @@ static void construct_flow_component(int seed, EdgeSet& es, SeedLink& link,
   for (int q : link.marks) es.set_processed_in(q, true);
   auto cand = std::make_unique<Candidate>();
   cand->seed = seed;
-  cand->convex_segment = std::move(link.seg);
+  cand->convex_segment.assign(link.seg.begin(), link.seg.end());
   int nReceivedVote = 0, nVotedPoints = 0;
   for (int q : cand->convex_segment) {
     const int vs = es.voters_size(q);
