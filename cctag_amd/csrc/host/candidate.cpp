@@ -56,8 +56,10 @@ inline bool bits_test(const uint64_t& w, uint64_t m) {
 // exit.  Testing a null point throws std::out_of_range like the reference.
 class AuxFlags {
  public:
-  explicit AuxFlags(size_t n) : f_(storage()) {
+  // n: the point count; expect: about how many points will be set
+  AuxFlags(size_t n, size_t expect) : f_(storage()) {
     if (f_.size() < n) f_.resize(n, 0);
+    touched_.reserve(expect);
   }
   ~AuxFlags() {
     for (int q : touched_) f_[q] = 0;
@@ -393,6 +395,8 @@ static void outlier_removal(const EdgeSet& es, const std::vector<int>& children,
   const float f = 1.f;
   std::vector<Pt2> pts;
   std::vector<float> weights;
+  pts.reserve(nSub + 1);
+  if (weightedType == kInvGradWeight) weights.reserve(nSub + 1);
   size_t k = 0, iEdge = 0;
   for (int c : children) {
     if (iEdge == (size_t)(k * step)) {
@@ -538,15 +542,17 @@ static bool ellipse_growing_init(const EdgeSet& es, const std::vector<int>& fc, 
   return false;
 }
 
-// EllipseGrowing.cpp:327-368, with an explicit stack in the recursion's order.
+// EllipseGrowing.cpp:327-368, with an explicit stack in the recursion's order
+// (the caller's, reused from call to call).
+struct HullFrame {
+  int x, y, i;
+};
 static void connected_point(std::vector<int>& pts, uint64_t mask, EdgeSet& es, const Ellipse& qin,
-                            const Ellipse& qout, int x0, int y0) {
+                            const Ellipse& qout, int x0, int y0, std::vector<HullFrame>& stack) {
   static const int xoff[8] = {1, 1, 0, -1, -1, -1, 0, 1};
   static const int yoff[8] = {0, -1, -1, -1, 0, 1, 1, 1};
-  struct Frame {
-    int x, y, i;
-  };
-  std::vector<Frame> stack;
+  using Frame = HullFrame;
+  stack.clear();
   bits_set(es.pts[es.at(x0, y0)].processed, mask);
   stack.push_back({x0, y0, 0});
   while (!stack.empty()) {
@@ -577,9 +583,11 @@ static void ellipse_hull(EdgeSet& es, std::vector<int>& pts, const Ellipse& e, f
   Ellipse qin, qout;
   compute_hull(e, delta, qin, qout);
   const size_t n0 = pts.size();
+  std::vector<HullFrame> stack;
+  stack.reserve(64);
   for (size_t i = 0; i < n0; ++i) {
     const EdgePoint& p = es.pts[pts[i]];
-    connected_point(pts, mask, es, qin, qout, p.x, p.y);
+    connected_point(pts, mask, es, qin, qout, p.x, p.y, stack);
   }
 }
 
@@ -646,8 +654,10 @@ static bool add_candidate_flow(const EdgeSet& es, const std::vector<int>& fc,
                                const std::vector<int>& outer_pts, const Ellipse& outer,
                                std::vector<std::vector<Pt2>>& cctagPoints, size_t numCircles) {
   cctagPoints.resize(numCircles);
+  cctagPoints.back().reserve(cctagPoints.back().size() + outer_pts.size());
   for (int e : outer_pts) cctagPoints.back().push_back(pt_of(es.pts[e]));
-  AuxFlags aux(es.pts.size());
+  for (size_t c = 0; c + 1 < numCircles; ++c) cctagPoints[c].reserve(cctagPoints[c].size() + fc.size());
+  AuxFlags aux(es.pts.size(), fc.size() * numCircles);
   size_t nGradientOut = 0, nAddedPoint = 0;
   for (int p0 : fc) {
     int dir = -1;
@@ -716,8 +726,10 @@ static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::ve
   const float SRef = median_ref(distRef);
   size_t cnt = 0;
   std::vector<float> dist, anotherDist;
+  std::vector<Pt2> points;
+  points.reserve(8);
   while (cnt < 100) {
-    std::vector<Pt2> points;
+    points.clear();
     int perm[5];
     rng.rand_5_k(perm, outerPts.size());
     for (int i = 0; i < 4; ++i) points.emplace_back(pts[perm[i]].x, pts[perm[i]].y);
@@ -857,6 +869,7 @@ static void complete_flow_component(LoopTwoTask& t, EdgeSet& es, size_t run_id,
     ellipse_growing2(es, filtered, outer, outerEllipse, p.ellipse_growing_elliptic_hull_width,
                      run_id, goodInit);
     std::vector<float> vDist;
+    vDist.reserve(outer.size());
     for (int q : outer) vDist.push_back(distance_point_ellipse((float)es.pts[q].x, (float)es.pts[q].y, outerEllipse));
     SmFinal = median_ref(vDist);
     if (SmFinal > p.thr_median_distance_ellipse) return;
