@@ -57,7 +57,7 @@ def frontend_bytes(name, lv):
     for l, c in enumerate(lv):
         N, E, V, S = c["N"], c["E"], c["V"], c["S"]
         per = {
-            "pyramid": N,                                  # level 0 read, coarser levels written
+            "pyramid": N + (4 * N if l == 0 else 0),       # level 0 read + its quads written, coarser levels written
             "filter": N + 4 * N,                           # u8 in, dx/dy int16 out
             "nms": 4 * N + N + 4 * N + N,                  # dx,dy in; cls, lab, root flag out
             "cc_local": N + 4 * N,                         # cls in, labels out

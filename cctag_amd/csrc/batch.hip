@@ -171,7 +171,7 @@ int cctagx_batch_create(const cctagx_params* params, int n_frames, const int32_t
     if (int rc = b->ident.init(b->bank, b->host, b->stream)) return rc;
     std::vector<DeviceImage> images;
     for (const PlaneDesc& d : b->fe.planes().planes)
-      if (d.level == 0) images.push_back(DeviceImage{b->fe.buffers().img + d.pix, d.w, d.h});
+      if (d.level == 0) images.push_back(DeviceImage{b->fe.buffers().quad + d.pix, d.w, d.h});
     b->ident.set_images(images);
     const int threads = TaskPool::default_threads();
     if (threads > 1) b->pool = std::make_unique<TaskPool>(threads);

@@ -185,7 +185,7 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   h->ident.clear_stats();
   try {
     const PlaneDesc& d0 = h->fe.planes().planes[0];
-    h->ident.set_image(h->fe.buffers().img + d0.pix, d0.w, d0.h);
+    h->ident.set_image(h->fe.buffers().quad + d0.pix, d0.w, d0.h);
     h->host_times = HostTimes();
     detect_markers(h->views, h->host, h->bank, &h->ident, h->markers, h->pool.get(),
                    &h->host_times, &h->workspace);

@@ -26,7 +26,7 @@ struct PlaneDesc {
   int32_t tile_f;           // first block of the default filter grid (64x32 tiles)
   int32_t tile_s;           // first block of the stencil grids (64x16 tiles)
   int32_t tile_c;           // first block of the compaction grid (4096-px chunks)
-  int32_t tile_p;           // first block of the pyramid grid (level-0 planes only)
+  int32_t tile_p;           // first block of the pyramid / quad grid (level-0 planes only)
 };
 
 // Per-plane counters written by the GPU.
@@ -64,6 +64,8 @@ struct FrontEndParams {
 // Device buffers of one plane set.
 struct FrontEndBuffers {
   uint8_t* img;        // Npix
+  uint32_t* quad;      // Npix, level-0 planes only: pixel (x, y)'s 2x2 neighbourhood, bytes
+                       // (x, y), (x+1, y), (x, y+1), (x+1, y+1), indices clamped to the plane
   int16_t* dx;         // Npix
   int16_t* dy;         // Npix
   uint8_t* cls;        // Npix: bit0 candidate (passes NMS and low), bit1 strong

@@ -52,7 +52,7 @@ bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_leve
       d.tile_c = tc;
       tc += cdiv(lw * lh, kChunk);
       d.tile_p = tp;
-      if (l == 0 && n_levels > 1) tp += cdiv(lw, kPyrTile) * cdiv(lh, kPyrTile);
+      if (l == 0) tp += cdiv(lw, kPyrTile) * cdiv(lh, kPyrTile);
       out.planes.push_back(d);
       pix += align256(lw * lh);
     }
@@ -72,6 +72,7 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t N = (size_t)ps.total_pix;
   const size_t NP = ps.planes.size();
   const size_t o_img = arena_.reserve(N);
+  const size_t o_quad = arena_.reserve(4 * N);
   const size_t o_dx = arena_.reserve(2 * N);
   const size_t o_dy = arena_.reserve(2 * N);
   const size_t o_cls = arena_.reserve(N);
@@ -105,6 +106,7 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   if (int rc = arena_.allocate()) return rc;
   FrontEndBuffers& b = launch_.buf;
   b.img = arena_.at<uint8_t>(o_img);
+  b.quad = arena_.at<uint32_t>(o_quad);
   b.dx = arena_.at<int16_t>(o_dx);
   b.dy = arena_.at<int16_t>(o_dy);
   b.cls = arena_.at<uint8_t>(o_cls);

@@ -55,13 +55,17 @@ __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 // OpenCV's fast 2x2 area path: vector lanes (the first floor(w/8)*8 outputs
 // of a row) compute (a+b+c+d+2)>>2, the scalar tail rounds (a+b+c+d)*0.25f
 // half to even.  One block turns a 64x64 level-0 tile into 32x32, 16x16 and
-// 8x8 tiles of levels 1..3 through LDS.
+// 8x8 tiles of levels 1..3 through LDS.  It also writes the tile's quads:
+// per level-0 pixel its 2x2 neighbourhood as one 32-bit word, so that the
+// identification's bilinear samples take one aligned load instead of four
+// byte loads.
 __device__ __forceinline__ uint8_t area2(int s, int x, int vec_end) {
   return (uint8_t)(x < vec_end ? (s + 2) >> 2 : (int)rintf((float)s * 0.25f));
 }
 
 __global__ __launch_bounds__(256) void k_pyramid(const PlaneDesc* __restrict__ P, int np,
-                                                  int n_levels, uint8_t* __restrict__ img) {
+                                                  int n_levels, uint8_t* __restrict__ img,
+                                                  uint32_t* __restrict__ quad) {
   __shared__ uint8_t s1[32][33];
   __shared__ uint8_t s2[16][17];
   const int p0 = plane_of<F_TILE_P>(P, np, blockIdx.x);
@@ -70,6 +74,16 @@ __global__ __launch_bounds__(256) void k_pyramid(const PlaneDesc* __restrict__ P
   const int tx_n = (d0.w + kPyrTile - 1) / kPyrTile;
   const int X0 = (t % tx_n) * kPyrTile, Y0 = (t / tx_n) * kPyrTile;
   const int tid = threadIdx.x;
+  for (int k = tid; k < kPyrTile * kPyrTile; k += 256) {
+    const int x = X0 + (k % kPyrTile), y = Y0 + k / kPyrTile;
+    if (x < d0.w && y < d0.h) {
+      const uint8_t* r0 = img + d0.pix + y * d0.w + x;
+      const int dx = x + 1 < d0.w ? 1 : 0, dy = y + 1 < d0.h ? d0.w : 0;
+      quad[d0.pix + y * d0.w + x] = (uint32_t)r0[0] | (uint32_t)r0[dx] << 8 |
+                                    (uint32_t)r0[dy] << 16 | (uint32_t)r0[dy + dx] << 24;
+    }
+  }
+  if (n_levels < 2) return;
   // level 1: 32x32, 4 outputs per thread, straight from global level 0
   {
     const PlaneDesc d1 = P[p0 + 1];
@@ -1127,8 +1141,8 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   };
 
   mark("start", true);
-  if (L.n_levels > 1 && L.n_tiles_p > 0) {
-    k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img);
+  if (L.n_tiles_p > 0) {
+    k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img, b.quad);
     mark("pyramid");
   }
   {

@@ -27,7 +27,7 @@ struct cctagx_refiner {
   hipStream_t stream = nullptr;
   int max_markers = 0, n = 0;
   int w = 0, h = 0;
-  DeviceBuffer img, jobs, H, cost, grad, centers, coarse, start, coarse_of;
+  DeviceBuffer img, quad, jobs, H, cost, grad, centers, coarse, start, coarse_of;
   PinnedBuffer h_jobs, h_io;
   GpuIdentBackend ident;
   std::vector<RefineJob> rjobs;
@@ -117,13 +117,17 @@ int cctagx_refiner_destroy(cctagx_refiner* r) {
 int cctagx_refiner_set_image(cctagx_refiner* r, const uint8_t* gray, int width, int height,
                              size_t stride) {
   if (!r || !gray || width < 2 || height < 2 || stride < (size_t)width) return CCTAGX_ERR_ARG;
-  if (r->img.ensure((size_t)width * height)) return CCTAGX_ERR_HIP;
+  if ((size_t)width * height > (size_t)INT32_MAX / 4) return CCTAGX_ERR_ARG;
+  if (r->img.ensure((size_t)width * height) || r->quad.ensure(4 * (size_t)width * height))
+    return CCTAGX_ERR_HIP;
   CCTAGX_HIP(hipMemcpy2DAsync(r->img.as<uint8_t>(), width, gray, stride, width, height,
                               hipMemcpyHostToDevice, r->stream));
+  if (launch_quads(r->img.as<uint8_t>(), width, height, r->quad.as<uint32_t>(), r->stream))
+    return CCTAGX_ERR_HIP;
   CCTAGX_HIP(hipStreamSynchronize(r->stream));
   r->w = width;
   r->h = height;
-  r->ident.set_image(r->img.as<uint8_t>(), width, height);
+  r->ident.set_image(r->quad.as<uint32_t>(), width, height);
   r->n = 0;
   return CCTAGX_OK;
 }
@@ -145,7 +149,7 @@ int cctagx_refiner_set_markers(cctagx_refiner* r, const cctagx_refine_marker* ma
       e.canonic_form(canonic, t_can, t_inv);
       GradJobDev& d = hj[i];
       std::memset(&d, 0, sizeof(d));
-      d.img = r->img.as<uint8_t>();
+      d.quad = r->quad.as<uint32_t>();
       d.iw = r->w;
       d.ih = r->h;
       d.Q11 = canonic.m[0][0];

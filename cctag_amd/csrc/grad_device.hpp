@@ -10,7 +10,8 @@
 namespace cctagx {
 
 struct GradJobDev {
-  const uint8_t* img;  // the marker's image (level 0), row-major, width iw
+  const uint32_t* quad;  // the marker's image (level 0) as quads (bytes (x, y), (x+1, y),
+                         // (x, y+1), (x+1, y+1) of word (x, y)), row-major, width iw
   int iw, ih;
   float Q11, Q22, Q33;             // canonic form of the outer ellipse
   float t_can[9], t_inv_can[9];    // image <-> canonic frame
@@ -21,6 +22,8 @@ struct GradJobDev {
   float2 stops[kMaxCuts];
 };
 
+// The quads of a w x h image (w, h >= 2) into quad (w * h words).
+int launch_quads(const uint8_t* img, int w, int h, uint32_t* quad, hipStream_t s);
 int launch_grad_eval(const GradJobDev* jobs, int n, const float* H, int n_samples, bool fp64,
                      double* cost, double* grad, hipStream_t s);
 int launch_grad_homography(const GradJobDev* jobs, int n, const float* centers, float* H,

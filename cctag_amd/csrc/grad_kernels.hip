@@ -128,9 +128,9 @@ __device__ __forceinline__ bool sample_at(const GradJobDev& J, const T* Hs, T qx
   const T x = w0 / w2, y = w1 / w2;
   if (!(x >= (T)0 && x < (T)(J.iw - 1) && y >= (T)0 && y < (T)(J.ih - 1))) return false;
   const int ix = (int)x, iy = (int)y;
-  const uint8_t* p = J.img + (size_t)iy * J.iw + ix;
+  const uint32_t q = J.quad[(size_t)iy * J.iw + ix];
   const T fx = x - (T)ix, fy = y - (T)iy;
-  const T a = p[0], b = p[1], c = p[J.iw], d = p[J.iw + 1];
+  const T a = (T)(q & 255u), b = (T)((q >> 8) & 255u), c = (T)((q >> 16) & 255u), d = (T)(q >> 24);
   val = (a * ((T)1 - fx) * ((T)1 - fy) + b * fx * ((T)1 - fy) + c * ((T)1 - fx) * fy + d * fx * fy) / (T)2;
   dx = ((b - a) * ((T)1 - fy) + (d - c) * fy) / (T)2;
   dy = ((c - a) * ((T)1 - fx) + (d - b) * fx) / (T)2;
@@ -387,7 +387,25 @@ __global__ __launch_bounds__(256) void k_grad_start(const GradJobDev* __restrict
   start[2 * m + 1] = j >= 0 ? coarse[2 * j + 1] : jobs[m].c0y;
 }
 
+// One word per pixel: its 2x2 neighbourhood, indices clamped to the image.
+__global__ __launch_bounds__(256) void k_quads(const uint8_t* __restrict__ img, int w, int h,
+                                               uint32_t* __restrict__ quad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w * h) return;
+  const int y = i / w, x = i - y * w;
+  const uint8_t* r0 = img + i;
+  const int dx = x + 1 < w ? 1 : 0, dy = y + 1 < h ? w : 0;
+  quad[i] = (uint32_t)r0[0] | (uint32_t)r0[dx] << 8 | (uint32_t)r0[dy] << 16 |
+            (uint32_t)r0[dy + dx] << 24;
+}
+
 }  // namespace
+
+int launch_quads(const uint8_t* img, int w, int h, uint32_t* quad, hipStream_t s) {
+  if (w < 2 || h < 2) return -1;
+  k_quads<<<(w * h + 255) / 256, 256, 0, s>>>(img, w, h, quad);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int launch_grad_start(const GradJobDev* jobs, int n, const int* coarse_of, const float* coarse,
                       float* start, hipStream_t s) {

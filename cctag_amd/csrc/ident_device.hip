@@ -89,7 +89,7 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
     if ((int)j.outer_points.size() > kMaxOuterPoints)
       throw std::runtime_error("more outer points than the identification kernels accept");
     const DeviceImage& im = image(j.image);
-    hj[i] = SelectJobDev{im.p, im.w, im.h, j.cx, j.cy, j.ca, j.sa, j.ea, j.eb, j.scale,
+    hj[i] = SelectJobDev{im.quad, im.w, im.h, j.cx, j.cy, j.ca, j.sa, j.ea, j.eb, j.scale,
                          (int)j.outer_points.size(), off};
     for (const Pt2& q : j.outer_points) hp[off++] = make_float4(q.x, q.y, q.dx, q.dy);
   }
@@ -126,7 +126,7 @@ int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs) {
     const RefineJob& r = jobs[i];
     RefineJobDev& d = hj[i];
     const DeviceImage& im = image(r.image);
-    d.img = im.p;
+    d.quad = im.quad;
     d.iw = im.w;
     d.ih = im.h;
     d.Q11 = r.canonic.m[0][0];
@@ -183,7 +183,7 @@ void GpuIdentBackend::refine_gradient(const std::vector<RefineJob>& jobs, const 
     const RefineJob& r = jobs[i];
     GradJobDev& d = g[i];
     const DeviceImage& im = image(r.image);
-    d.img = im.p;
+    d.quad = im.quad;
     d.iw = im.w;
     d.ih = im.h;
     d.Q11 = r.canonic.m[0][0];

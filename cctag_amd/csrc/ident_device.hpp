@@ -17,7 +17,7 @@ constexpr int kMaxBankRows = 128;  // rows of the 4-crown bank
 constexpr int kMaxRefineSamples = 32;  // numSamplesOuterEdgePointsRefinement bound
 
 struct SelectJobDev {
-  const uint8_t* img;  // the marker's frame (level 0), row-major, width iw
+  const uint32_t* quad;  // the marker's frame (level 0) as quads, row-major, width iw
   int iw, ih;
   float cx, cy;
   double ca, sa;
@@ -31,7 +31,7 @@ struct SelectOutDev {
 };
 
 struct RefineJobDev {
-  const uint8_t* img;  // the marker's frame (level 0), row-major, width iw
+  const uint32_t* quad;  // the marker's frame (level 0) as quads, row-major, width iw
   int iw, ih;
   float Q11, Q22, Q33;
   float t_can[9], t_inv_can[9], cond[9], cond_inv[9];
@@ -93,9 +93,10 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, float* dist, hipStream_t s);
 
-// A device image the identification jobs sample.
+// A device image the identification jobs sample, as quads: word (x, y)
+// holds pixels (x, y), (x+1, y), (x, y+1), (x+1, y+1) in bytes 0..3.
 struct DeviceImage {
-  const uint8_t* p;
+  const uint32_t* quad;
   int w, h;
 };
 
@@ -106,7 +107,7 @@ class GpuIdentBackend : public IdentBackend {
   int init(const std::vector<std::vector<float>>& bank, const HostParams& p, hipStream_t s);
   // Images that CutJob::image / RefineJob::image index.
   void set_images(const std::vector<DeviceImage>& images) { images_ = images; }
-  void set_image(const uint8_t* dev_img, int w, int h) { images_.assign(1, DeviceImage{dev_img, w, h}); }
+  void set_image(const uint32_t* dev_quad, int w, int h) { images_.assign(1, DeviceImage{dev_quad, w, h}); }
   void select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
                    std::vector<CutSelection>& out) override;
   void refine(const std::vector<RefineJob>& jobs, const HostParams& p,

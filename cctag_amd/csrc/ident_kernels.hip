@@ -26,31 +26,26 @@ __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
 // A pointer the compiler knows to be in device global memory (the loads
 // become global rather than flat ones).
-typedef const __attribute__((address_space(1))) uint8_t* gptr_u8;
+typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
 
+// The frame as quads: word (x, y) holds pixels (x, y), (x+1, y), (x, y+1),
+// (x+1, y+1) in bytes 0..3, so a bilinear sample is one aligned load.
 struct Img {
-  const uint8_t* p;
+  const uint32_t* q;
   int w, h;
-  __device__ __forceinline__ gptr_u8 gp() const { return (gptr_u8)p; }
-  // Identification.hpp:197-222: half of the bilinear value.
-  __device__ __forceinline__ float bilinear(float x, float y) const {
-    const int px = (int)x, py = (int)y;
-    const uint8_t* p0 = p + px + py * w;
-    const float fx = x - px, fy = y - py;
-    const float fx1 = 1.0f - fx, fy1 = 1.0f - fy;
-    const float w1 = fx1 * fy1, w2 = fx * fy1, w3 = fx1 * fy, w4 = fx * fy;
-    return ((float)p0[0] * w1 + (float)p0[1] * w2 + (float)p0[w] * w3 + (float)p0[w + 1] * w4) / 2;
-  }
+  __device__ __forceinline__ uint32_t quad(int x, int y) const { return ((gptr_u32)q)[x + y * w]; }
 };
 
-// Img::bilinear with the four corner pixels already loaded.
-__device__ __forceinline__ float bilinear_of(float x, float y, uint8_t a, uint8_t b, uint8_t c,
-                                             uint8_t d) {
+// Identification.hpp:197-222: half of the bilinear value at (x, y), from
+// the quad of pixel (floor x, floor y).
+__device__ __forceinline__ float bilinear_of(float x, float y, uint32_t quad) {
   const int px = (int)x, py = (int)y;
   const float fx = x - px, fy = y - py;
   const float fx1 = 1.0f - fx, fy1 = 1.0f - fy;
   const float w1 = fx1 * fy1, w2 = fx * fy1, w3 = fx1 * fy, w4 = fx * fy;
-  return ((float)a * w1 + (float)b * w2 + (float)c * w3 + (float)d * w4) / 2;
+  const float a = (float)(quad & 255u), b = (float)((quad >> 8) & 255u),
+              c = (float)((quad >> 16) & 255u), d = (float)(quad >> 24);
+  return (a * w1 + b * w2 + c * w3 + d * w4) / 2;
 }
 
 __device__ __forceinline__ void apply_h(float& xr, float& yr, const float* H, float x, float y) {
@@ -146,20 +141,15 @@ __device__ bool cut_walk(const Img& im, float sx, float sy, float ex, float ey, 
       y += stepY;
     }
     if (!inside) return false;
-    uint8_t p00[kWalkBatch], p01[kWalkBatch], p10[kWalkBatch], p11[kWalkBatch];
+    uint32_t qd[kWalkBatch];
 #pragma unroll
     for (int t = 0; t < kWalkBatch; ++t) {
       const bool live = i0 + t < n;
-      const int ix = (int)(live ? px[t] : px[0]), iy = (int)(live ? py[t] : py[0]);
-      const gptr_u8 q = im.gp() + ix + iy * im.w;
-      p00[t] = q[0];
-      p01[t] = q[1];
-      p10[t] = q[im.w];
-      p11[t] = q[im.w + 1];
+      qd[t] = im.quad((int)(live ? px[t] : px[0]), (int)(live ? py[t] : py[0]));
     }
 #pragma unroll
     for (int t = 0; t < kWalkBatch; ++t)
-      if (i0 + t < n) f(i0 + t, bilinear_of(px[t], py[t], p00[t], p01[t], p10[t], p11[t]));
+      if (i0 + t < n) f(i0 + t, bilinear_of(px[t], py[t], qd[t]));
   }
   return true;
 }
@@ -188,7 +178,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
   __shared__ float w_max[NW];
   __shared__ float ref_sig[kMaxRefineSamples][kSelectBlock];  // the thread's refinement cut
   const SelectJobDev J = jobs[blockIdx.x];
-  const Img im{J.img, J.iw, J.ih};
+  const Img im{J.quad, J.iw, J.ih};
   const int n = J.n_outer;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   bool valid = false;
@@ -422,20 +412,13 @@ __device__ bool sample_chunk(const Img& im, const float* H, const CutWalk& w, in
       x += stepX;
       y += stepY;
     }
-    uint8_t p00[kLoadBatch], p01[kLoadBatch], p10[kLoadBatch], p11[kLoadBatch];
+    uint32_t qd[kLoadBatch];
 #pragma unroll
-    for (int t = 0; t < kLoadBatch; ++t) {
-      const int px = in[t] ? (int)xr[t] : 0, py = in[t] ? (int)yr[t] : 0;
-      const gptr_u8 p0 = im.gp() + px + py * im.w;
-      p00[t] = p0[0];
-      p01[t] = p0[1];
-      p10[t] = p0[im.w];
-      p11[t] = p0[im.w + 1];
-    }
+    for (int t = 0; t < kLoadBatch; ++t) qd[t] = in[t] ? im.quad((int)xr[t], (int)yr[t]) : im.quad(0, 0);
 #pragma unroll
     for (int t = 0; t < kLoadBatch; ++t) {
       if (i0 + t0 + t >= n_samples) continue;
-      out[i0 + t0 + t] = in[t] ? bilinear_of(xr[t], yr[t], p00[t], p01[t], p10[t], p11[t]) : 0.f;
+      out[i0 + t0 + t] = in[t] ? bilinear_of(xr[t], yr[t], qd[t]) : 0.f;
     }
   }
   return oob;
@@ -661,7 +644,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   }
   if (!S.ok) return;
   const int nc = J.n_cuts;
-  const Img im{J.img, J.iw, J.ih};
+  const Img im{J.quad, J.iw, J.ih};
   RefineGridDev& GD = W.grid[((size_t)cb * n_jobs + m) * G2MAX + g];
   if (tid == 0) {
     float Hf[9], Hi[9], qx, qy;
@@ -813,7 +796,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     if (tid == 0) O.ok = 0;
     return;
   }
-  const Img im{J.img, J.iw, J.ih};
+  const Img im{J.quad, J.iw, J.ih};
   if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
   __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
