@@ -54,11 +54,12 @@ struct RefineOutDev {
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
                        int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s);
 // Grid-search state of one marker between iteration launches.
+static_assert(kMaxCuts <= 32, "one bit per cut in the flag words");
 struct RefineStateDev {
   float cx, cy, neighbour, min_res;
   int ok, best_g;
-  int pad[2];
-  uint8_t sticky[kMaxCuts];
+  uint32_t sticky;     // bit k: cut k left the image at an earlier grid point
+  int pad;
 };
 // One grid point of the current iteration.
 struct RefineGridDev {
@@ -66,7 +67,7 @@ struct RefineGridDev {
   float gx, gy;
   double cost_local;   // residual sum assuming only this point's own flags
   int m_local;         // valid cuts under that assumption
-  uint8_t oob[kMaxCuts];
+  uint32_t oob;        // bit k: a sample of cut k is outside the image
 };
 constexpr int kMaxPairs = kMaxCuts * (kMaxCuts + 1) / 2;
 // Two sets (by iteration parity): an iteration reads the previous set and

@@ -86,16 +86,53 @@ __device__ __forceinline__ void condition(float& x, float& y, const float* T) {
 }
 
 // boost::accumulators immediate mean and variance, one sample at a time.
+// x / n, correctly rounded, for a float x and an integer 1 <= n <=
+// kMaxSamples, from inv_n = 1/n rounded to double: the double product is
+// within 2^-52 (relative) of x/n; x/n, a 24-bit significand over an
+// integer below 2^8, is never a midpoint between two floats and lies more
+// than 2^-33 (relative) away from every midpoint, so rounding the product to
+// float rounds x/n.  The argument needs a result in the normal range: for a
+// subnormal one (div_n_tiny) the caller takes the division.
+__device__ __forceinline__ double div_n_product(float x, double inv_n) { return (double)x * inv_n; }
+__device__ __forceinline__ bool div_n_tiny(double p) { return p != 0.0 && fabs(p) < 0x1p-126; }
+
+// 1/n in double for n = 1 .. kMaxSamples (entry 0 unused), by the block.
+__device__ __forceinline__ void fill_inv(double* inv) {
+  for (int q = threadIdx.x; q <= kMaxSamples; q += blockDim.x) inv[q] = 1.0 / (double)q;
+}
+
+// Running mean and variance, one value at a time (at most kMaxSamples).
+// The reciprocals a step needs are read from fill_inv's table one step
+// ahead, so the table's latency stays off the chain of divisions.
 struct MeanVar {
   float mean = 0.f, var = 0.f;
   unsigned cnt = 0;
+  const double* inv;
+  double inv_cnt, inv_prev;  // 1/(cnt + 1) and 1/cnt for the next add
+  __device__ explicit MeanVar(const double* table) : inv(table), inv_cnt(table[1]), inv_prev(0.0) {}
+  // the subnormal fallbacks are branched to once per wavefront and step
   __device__ __forceinline__ void add(float v) {
     ++cnt;
-    mean = (mean * (float)(cnt - 1) + v) / (float)cnt;
+    const double inv_next = inv[min(cnt + 1, (unsigned)kMaxSamples)];
+    const float before = (float)(cnt - 1);
+    const float xm = mean * before + v;
+    const double pm = div_n_product(xm, inv_cnt);
+    mean = (float)pm;
+    if (__builtin_expect(__ballot(div_n_tiny(pm)) != 0, 0))
+      if (div_n_tiny(pm)) mean = xm / (float)cnt;
     if (cnt > 1) {
       const float t = v - mean;
-      var = var * (float)(cnt - 1) / (float)cnt + t * t / (float)(cnt - 1);
+      const float xa = var * before, xb = t * t;
+      const double pa = div_n_product(xa, inv_cnt), pb = div_n_product(xb, inv_prev);
+      float a = (float)pa, b = (float)pb;
+      if (__builtin_expect(__ballot(div_n_tiny(pa) || div_n_tiny(pb)) != 0, 0)) {
+        if (div_n_tiny(pa)) a = xa / (float)cnt;
+        if (div_n_tiny(pb)) b = xb / before;
+      }
+      var = a + b;
     }
+    inv_prev = inv_cnt;
+    inv_cnt = inv_next;
   }
 };
 
@@ -177,6 +214,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
   __shared__ int w_cnt[NW], w_cand[NW];
   __shared__ float w_max[NW];
   __shared__ float ref_sig[kMaxRefineSamples][kSelectBlock];  // the thread's refinement cut
+  __shared__ double inv[kMaxSamples + 1];
+  fill_inv(inv);
+  __syncthreads();
   const SelectJobDev J = jobs[blockIdx.x];
   const Img im{J.quad, J.iw, J.ih};
   const int n = J.n_outer;
@@ -186,7 +226,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
   if (t < n) {
     s = pts[J.pt_offset + t];
-    MeanVar mv;
+    MeanVar mv(inv);
     valid = cut_walk(im, J.cx, J.cy, s.x, s.y, start_sig, 1.f, n_samples,
                      [&](int, float v) { mv.add(v); });
     var = mv.var;
@@ -211,7 +251,6 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
     nc += w_cnt[w];
     var_max = fmaxf(var_max, w_max[w]);
   }
-  if (t == 0) out[blockIdx.x].n_collected = nc;
   bool cand = false;
   if (valid) {
     // pointOnEllipse (EllipseFromPoints.cpp:29-49), double like the C cos/sin
@@ -286,9 +325,12 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
     const int at = (int)((float)t * step);
     if (at < ni) out[blockIdx.x].stops[t] = stop[idx_of_rank[at]];
   }
+  // (out is host memory: written after the last barrier, which would
+  // otherwise wait for the stores to reach the host)
   if (t == 0) {
     int nsel = 0;
     while (nsel < select && (int)((float)nsel * step) < ni) ++nsel;
+    out[blockIdx.x].n_collected = nc;
     out[blockIdx.x].n_selected = nsel;
   }
 }
@@ -305,8 +347,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 // every block of the next launch (and by the final kernel), so an
 // iteration costs one launch.
 constexpr int kRefineBlock = 256;
+constexpr int kFinalBlock = 1024;  // one block per marker: every loop of the read-out in one pass
 constexpr int kRefineChunk = 10;  // samples per sampling work item
-constexpr int kLoadBatch = 5;     // samples whose image reads are issued together
+constexpr int kLoadBatch = 10;    // samples whose image reads are issued together
 static_assert(kRefineChunk % kLoadBatch == 0, "whole load batches per chunk");
 constexpr int G2MAX = kMaxGrid * kMaxGrid;
 constexpr int kFuseMaxBlocks = 2048;  // grid blocks up to which the selection is fused
@@ -427,15 +470,14 @@ __device__ bool sample_chunk(const Img& im, const float* H, const CutWalk& w, in
 __device__ __forceinline__ int pair_offset(int k, int nc) { return k * nc - k * (k - 1) / 2; }
 
 // Row k's share of sum_{k<l valid} sum_i (s_ki - s_li)^2 from the Gram
-// matrix C (packed upper triangle): (m-1) C_kk - 2 sum_{l>k valid} C_kl.
-// The row's entries are read kRowBatch at a time (indices clamped into the
-// row, so the reads need no branch and are in flight together) and
-// subtracted in column order, skipping invalid columns as the plain loop
-// does.
+// matrix C (packed upper triangle): (m-1) C_kk - 2 sum_{l>k valid} C_kl,
+// valid cuts being the set bits of `valid`.  The row's entries are read
+// kRowBatch at a time (indices clamped into the row, so the reads need no
+// branch and are in flight together) and subtracted in column order,
+// skipping invalid columns as the plain loop does.
 constexpr int kRowBatch = 8;
-__device__ __forceinline__ double gram_row(const double* C, const uint8_t* valid, int k, int nc,
-                                           int m) {
-  if (!valid[k]) return 0.0;
+__device__ __forceinline__ double gram_row(const double* C, uint32_t valid, int k, int nc, int m) {
+  if (!((valid >> k) & 1u)) return 0.0;
   const double* row = C + pair_offset(k, nc);
   const int last = nc - 1 - k;  // row[0 .. last]
   double acc = (double)(m - 1) * row[0];
@@ -445,7 +487,7 @@ __device__ __forceinline__ double gram_row(const double* C, const uint8_t* valid
     for (int j = 0; j < kRowBatch; ++j) v[j] = row[min(d0 + j, last)];
 #pragma unroll
     for (int j = 0; j < kRowBatch; ++j)
-      if (d0 + j <= last && valid[k + d0 + j]) acc -= 2.0 * v[j];
+      if (d0 + j <= last && ((valid >> (k + d0 + j)) & 1u)) acc -= 2.0 * v[j];
   }
   return acc;
 }
@@ -457,107 +499,88 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Selection step of an iteration (what the reference's loop does after
-// evaluating its grid): the sticky out-of-bounds flags in grid-point order,
-// each point's residual (its block's own when the valid cuts are the ones it
-// assumed, otherwise recomputed from its Gram matrix with the same row sums
-// and reduction order), the first minimum, the next centre.  Called by a
-// whole block; every block that calls it with the same inputs gets the
-// same state.
-struct PickShared {
-  uint8_t oobl[G2MAX][kMaxCuts];
-  uint8_t valid[G2MAX][kMaxCuts];
-  uint8_t sticky_prev[kMaxCuts];
-  int mvalid[G2MAX];
-  int same[G2MAX];
-  float res[G2MAX];
-  double cost_local[G2MAX];
-  int m_local[G2MAX];
-  double part[G2MAX][64];  // row shares of the points that are recomputed
-};
+__device__ __forceinline__ uint32_t cut_bits(int nc) { return nc >= 32 ? ~0u : (1u << nc) - 1u; }
 
+// Selection step of an iteration (what the reference's loop does after
+// evaluating its grid), by wavefront 0 with lane g on grid point g: the
+// sticky out-of-bounds flags in grid-point order (a prefix OR over the
+// lanes), each point's residual (its block's own when the valid cuts are the
+// ones it assumed, otherwise recomputed from its Gram matrix with the same
+// row sums and reduction order), the first minimum, the next centre.  Called
+// by a whole block (one barrier); every block that calls it with the same
+// inputs gets the same state.
 __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateDev& prev,
-                           const RefineGridDev* GM, const double* gram_m, PickShared& P,
-                           RefineStateDev& S) {
+                           const RefineGridDev* GM, const double* gram_m, RefineStateDev& S) {
   const int tid = threadIdx.x, nc = J.n_cuts, G2 = grid_n * grid_n;
-  for (int q = tid; q < G2 * nc; q += kRefineBlock) {
-    const int gg = q / nc, k = q - gg * nc;
-    P.oobl[gg][k] = GM[gg].oob[k];
-  }
-  if (tid < G2) {
-    P.cost_local[tid] = GM[tid].cost_local;
-    P.m_local[tid] = GM[tid].m_local;
-  }
-  if (tid < kMaxCuts) P.sticky_prev[tid] = prev.sticky[tid];
-  __syncthreads();
-  if (tid < nc) {
-    uint8_t run = P.sticky_prev[tid];
-    for (int gg = 0; gg < G2; ++gg) {
-      run |= P.oobl[gg][tid];
-      P.valid[gg][tid] = !run;
+  if (tid < 64) {
+    const uint32_t cuts = cut_bits(nc);
+    const bool on = tid < G2;
+    const uint32_t sp = prev.sticky;
+    uint32_t own = 0;
+    double acc = 0.0;
+    int ml = 0;
+    float gx = 0.f, gy = 0.f;
+    if (on) {
+      const RefineGridDev& G = GM[tid];
+      own = G.oob;
+      acc = G.cost_local;
+      ml = G.m_local;
+      gx = G.gx;
+      gy = G.gy;
     }
-    S.sticky[tid] = run;
-  } else if (tid < kMaxCuts) {
-    S.sticky[tid] = P.sticky_prev[tid];
-  }
-  __syncthreads();
-  if (tid < G2) {
-    int cnt = 0;
-    bool same = true;
-    for (int k = 0; k < nc; ++k) {
-      cnt += P.valid[tid][k];
-      same = same && (P.valid[tid][k] == (uint8_t)!(P.sticky_prev[k] | P.oobl[tid][k]));
+    uint32_t run = own;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(run, o, 64);
+      if (tid >= o) run |= v;
     }
-    P.mvalid[tid] = cnt;
-    P.same[tid] = same && cnt == P.m_local[tid];
-  }
-  __syncthreads();
-  // points whose valid cuts differ from the ones their block assumed: row
-  // shares from the Gram matrices, all rows at once, then the wavefront
-  // reduction tree of wave_sum replayed in LDS (same additions, same order)
-  for (int q = tid; q < G2 * 64; q += kRefineBlock) {
-    const int gg = q >> 6, k = q & 63;
-    if (P.same[gg]) continue;
-    P.part[gg][k] = k < nc ? gram_row(gram_m + (size_t)gg * kMaxPairs, P.valid[gg], k, nc,
-                                      P.mvalid[gg])
-                           : 0.0;
-  }
-  __syncthreads();
-  for (int o = 32; o > 0; o >>= 1) {
-    for (int q = tid; q < G2 * o; q += kRefineBlock) {
-      const int gg = q / o, k = q - gg * o;
-      if (!P.same[gg]) P.part[gg][k] += P.part[gg][k + o];
+    run |= sp;
+    const uint32_t valid = ~run & cuts;
+    const int mv = __popc(valid);
+    const bool same = valid == (~(sp | own) & cuts) && mv == ml;
+    // points whose valid cuts differ from the ones their block assumed, one
+    // at a time over the whole wavefront, as the block itself sums
+    for (unsigned long long redo = __ballot(on && !same); redo; redo &= redo - 1) {
+      const int gg = __builtin_ctzll(redo);
+      const uint32_t vg = __shfl(valid, gg, 64);
+      const int mg = __shfl(mv, gg, 64);
+      const double part = tid < nc ? gram_row(gram_m + (size_t)gg * kMaxPairs, vg, tid, nc, mg) : 0.0;
+      const double sum = __shfl(wave_sum(part), 0, 64);
+      if (tid == gg) acc = sum;
     }
-    __syncthreads();
-  }
-  if (tid < G2) {
-    const double acc = P.same[tid] ? P.cost_local[tid] : P.part[tid][0];
-    const int mv = P.mvalid[tid];
-    P.res[tid] = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int b = -1;
-    float mr = FLT_MAX;
-    for (int gg = 0; gg < G2; ++gg)
-      if (P.mvalid[gg] >= 2 && P.res[gg] < mr) {
-        mr = P.res[gg];
-        b = gg;
+    const float res = mv < 2 ? FLT_MAX : (float)acc / (float)(mv * (mv - 1) / 2);
+    // the first minimum among the points with two valid cuts and a residual
+    // below FLT_MAX: lexicographic (residual, index) minimum over the lanes
+    const bool pickable = on && mv >= 2 && res < FLT_MAX;
+    float key = pickable ? res : INFINITY;
+    int idx = pickable ? tid : 64;
+    for (int o = 32; o > 0; o >>= 1) {
+      const float k2 = __shfl_xor(key, o, 64);
+      const int i2 = __shfl_xor(idx, o, 64);
+      if (k2 < key || (k2 == key && i2 < idx)) {
+        key = k2;
+        idx = i2;
       }
-    S.neighbour = prev.neighbour;
-    if (b < 0) {
-      S.ok = 0;
-      S.best_g = prev.best_g;
-      S.cx = prev.cx;
-      S.cy = prev.cy;
-      S.min_res = prev.min_res;
-    } else {
-      S.ok = 1;
-      S.best_g = b;
-      S.cx = GM[b].gx;
-      S.cy = GM[b].gy;
-      S.neighbour /= (float)((grid_n - 1) / 2);
-      S.min_res = P.res[b];
+    }
+    const uint32_t sticky = __shfl(run, G2 - 1, 64);
+    const int b = idx < 64 ? idx : -1;
+    const float bx = __shfl(gx, b < 0 ? 0 : b, 64), by = __shfl(gy, b < 0 ? 0 : b, 64);
+    if (tid == 0) {
+      S.sticky = sticky;
+      S.neighbour = prev.neighbour;
+      if (b < 0) {
+        S.ok = 0;
+        S.best_g = prev.best_g;
+        S.cx = prev.cx;
+        S.cy = prev.cy;
+        S.min_res = prev.min_res;
+      } else {
+        S.ok = 1;
+        S.best_g = b;
+        S.cx = bx;
+        S.cy = by;
+        S.neighbour /= (float)((grid_n - 1) / 2);
+        S.min_res = key;
+      }
     }
   }
   __syncthreads();
@@ -567,7 +590,7 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
 // first one, otherwise the selection over the previous iteration's grid
 // (buffers of parity (it - 1) & 1).
 __device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int grid_n,
-                         const RefineWork& W, PickShared& P, RefineStateDev& S) {
+                         const RefineWork& W, RefineStateDev& S) {
   const int tid = threadIdx.x;
   if (it == 0) {
     if (tid == 0) {
@@ -577,8 +600,8 @@ __device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int g
       S.min_res = FLT_MAX;
       S.ok = 1;
       S.best_g = 0;
+      S.sticky = 0;
     }
-    if (tid < kMaxCuts) S.sticky[tid] = 0;
     __syncthreads();
     return;
   }
@@ -590,7 +613,7 @@ __device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int g
     return;
   }
   pick_state(J, grid_n, prev, W.grid + ((size_t)pb * n_jobs + m) * G2MAX,
-             W.gram + ((size_t)pb * n_jobs + m) * G2MAX * kMaxPairs, P, S);
+             W.gram + ((size_t)pb * n_jobs + m) * G2MAX * kMaxPairs, S);
 }
 
 // Iteration `it` of the grid search: block (g, m) takes marker m's state --
@@ -606,18 +629,11 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, int fused,
     RefineWork W) {
   __shared__ float sH[9], sHi[9];
-  // the selection's scratch is dead once the sampling starts
-  __shared__ union {
-    PickShared P;
-    float sig[kMaxCuts][kMaxSamples + 1];
-  } U;
+  __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
   __shared__ double sC[kMaxPairs];
-  __shared__ uint8_t lvalid[kMaxCuts];
   __shared__ CutWalk walk[kMaxCuts];
   __shared__ RefineStateDev S;
-  PickShared& P = U.P;
-  auto& sig = U.sig;
 
   // Workgroups are dealt to the 8 XCDs in turn, each XCD with its own L2:
   // renumber them so that an XCD runs consecutive (grid point, marker)
@@ -636,7 +652,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   if (it >= J.n_iter) return;
   const int cb = it & 1;
   if (fused) {
-    state_at(J, it, m, n_jobs, grid_n, W, P, S);
+    state_at(J, it, m, n_jobs, grid_n, W, S);
     if (g == 0 && tid == 0) W.state[(size_t)cb * n_jobs + m] = S;
   } else {
     if (tid == 0) S = W.state[(size_t)cb * n_jobs + m];
@@ -682,21 +698,19 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     C[q] = acc;
     sC[q] = acc;
   }
-  if (tid < nc) {
-    GD.oob[tid] = oobk[tid] != 0;
-    lvalid[tid] = !(S.sticky[tid] | (oobk[tid] != 0));
-  }
   __syncthreads();
   // residual of this grid point if no earlier grid point of the iteration
   // flags another cut (the selection checks that and recomputes otherwise)
   if (tid < 64) {
-    int mv = 0;
-    for (int k = 0; k < nc; ++k) mv += lvalid[k];
-    const double part = tid < nc ? gram_row(sC, lvalid, tid, nc, mv) : 0.0;
+    const uint32_t oob = (uint32_t)__ballot(tid < nc && oobk[tid] != 0);
+    const uint32_t valid = ~(S.sticky | oob) & cut_bits(nc);
+    const int mv = __popc(valid);
+    const double part = tid < nc ? gram_row(sC, valid, tid, nc, mv) : 0.0;
     const double sum = wave_sum(part);
     if (tid == 0) {
       GD.cost_local = sum;
       GD.m_local = mv;
+      GD.oob = oob;
     }
   }
 }
@@ -705,21 +719,19 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
 // iteration it), one block per marker; k_refine_init gives iteration 0's.
 __global__ __launch_bounds__(kRefineBlock) void k_refine_pick(const RefineJobDev* __restrict__ jobs,
                                                               int it, int grid_n, RefineWork W) {
-  __shared__ PickShared P;
   __shared__ RefineStateDev S;
   const int m = blockIdx.x, n_jobs = gridDim.x;
   const RefineJobDev& J = jobs[m];
   if (it + 1 > J.n_iter) return;
-  state_at(J, it + 1, m, n_jobs, grid_n, W, P, S);
+  state_at(J, it + 1, m, n_jobs, grid_n, W, S);
   if (threadIdx.x == 0) W.state[(size_t)((it + 1) & 1) * n_jobs + m] = S;
 }
 
 __global__ __launch_bounds__(kRefineBlock) void k_refine_init(const RefineJobDev* __restrict__ jobs,
                                                               int grid_n, RefineWork W) {
-  __shared__ PickShared P;
   __shared__ RefineStateDev S;
   const int m = blockIdx.x;
-  state_at(jobs[m], 0, m, gridDim.x, grid_n, W, P, S);
+  state_at(jobs[m], 0, m, gridDim.x, grid_n, W, S);
   if (threadIdx.x == 0) W.state[m] = S;
 }
 
@@ -732,7 +744,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_init(const RefineJobDev
 // per job, scores), whose out-of-bounds cuts are the ones leaving the image
 // there.  Signals, median profile, orazio statistics, distance to every
 // bank profile.
-__global__ __launch_bounds__(kRefineBlock) void k_refine_final(
+__global__ __launch_bounds__(kFinalBlock) void k_refine_final(
     const RefineJobDev* __restrict__ jobs, int n_samples, int grid_n,
     const float* __restrict__ inv_ratio, int bank_rows, int bank_cols, RefineWork W,
     const float* __restrict__ given_H, const float* __restrict__ given_c,
@@ -749,16 +761,11 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   __shared__ float stat_mub[kMaxCuts], stat_muw[kMaxCuts], stat_var[kMaxCuts];
   __shared__ float xs[kMaxSamples];
   __shared__ float ratio[kMaxBankRows * kMaxRatios];
-  __shared__ uint8_t parity[kMaxBankRows][kMaxSamples];
-  // a sample's term of the distance, for each ring parity; the selection's
-  // scratch is dead by then
-  __shared__ union {
-    PickShared P;
-    float term[2][kMaxCuts][kMaxSamples + 1];
-  } U;
+  __shared__ uint8_t parity[kMaxSamples][kMaxBankRows];  // [sample][bank row]
+  // a sample's term of the distance, for each ring parity
+  __shared__ float term[2][kMaxCuts][kMaxSamples + 1];
   __shared__ RefineStateDev S;
-  PickShared& P = U.P;
-  auto& term = U.term;
+  __shared__ double inv[kMaxSamples + 1];
   const int m = blockIdx.x, tid = threadIdx.x, n_jobs = gridDim.x;
   const RefineJobDev& J = jobs[m];
   RefineOutDev& O = out[m];
@@ -767,7 +774,8 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     if (tid == 0) O.ok = 0;
     return;
   }
-  for (int q = tid; q < bank_rows * kMaxRatios; q += kRefineBlock) ratio[q] = inv_ratio[q];
+  for (int q = tid; q < bank_rows * kMaxRatios; q += kFinalBlock) ratio[q] = inv_ratio[q];
+  fill_inv(inv);
   const int nc = J.n_cuts;
   if (given) {
     if (tid == 0) {
@@ -783,7 +791,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     __syncthreads();
   } else {
     // the selection after the last iteration
-    state_at(J, J.n_iter, m, n_jobs, grid_n, W, P, S);
+    state_at(J, J.n_iter, m, n_jobs, grid_n, W, S);
     if (S.ok && tid < 9) {
       const RefineGridDev& GB =
           W.grid[((size_t)((J.n_iter - 1) & 1) * n_jobs + m) * G2MAX + S.best_g];
@@ -800,52 +808,65 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
   __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
-  for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
+  for (int w = tid; w < nc * nchunk; w += kFinalBlock) {
     const int k = w / nchunk, c = w - k * nchunk;
     if (sample_chunk(im, sH, walk[k], n_samples, c, sig[k]) && given) atomicOr(&oob_at[k], 1);
   }
   __syncthreads();
-  if (given && tid < nc) S.sticky[tid] = oob_at[tid] != 0;
-  __syncthreads();
-  if (tid == 0) {
-    O.ok = 1;
-    O.cx = S.cx;
-    O.cy = S.cy;
-    for (int r = 0; r < 9; ++r) O.H[r] = sH[r];
-    O.min_res = S.min_res;
+  // The outputs live in host memory: they are stored after the last
+  // barrier, which would otherwise wait for the stores to reach the host.
+  if (given && tid < 64) {
+    const uint32_t oob = (uint32_t)__ballot(tid < nc && oob_at[tid] != 0);
+    if (tid == 0) S.sticky = oob;
   }
-  if (tid < nc) O.oob[tid] = S.sticky[tid];
-  if (tid == 0) {
-    int mv = 0;
-    for (int k = 0; k < nc; ++k)
-      if (!S.sticky[k]) valid_cut[mv++] = k;
-    n_valid = mv;
+  __syncthreads();
+  const uint32_t sticky = S.sticky;
+  if (tid < 64) {
+    const uint32_t valid = ~sticky & cut_bits(nc);
+    if (tid < nc && ((valid >> tid) & 1u)) valid_cut[__popc(valid & ((1u << tid) - 1u))] = tid;
+    if (tid == 0) n_valid = __popc(valid);
   }
   // orazioDistanceRobust statistics per valid cut (Identification.cpp:88-121)
-  if (tid < nc && !S.sticky[tid]) {
+  if (tid < nc && !((sticky >> tid) & 1u)) {
+    // the signal is read kStatBatch samples at a time (indices clamped,
+    // so the reads are in flight together), then consumed in order
+    constexpr int kStatBatch = 8;
     const float* sg = sig[tid];
-    MeanVar mv;
-    for (int i = 30; i < n_samples; ++i) mv.add(sg[i]);
+    MeanVar mv(inv);
+    for (int i0 = 30; i0 < n_samples; i0 += kStatBatch) {
+      float v[kStatBatch];
+#pragma unroll
+      for (int j = 0; j < kStatBatch; ++j) v[j] = sg[min(i0 + j, n_samples - 1)];
+#pragma unroll
+      for (int j = 0; j < kStatBatch; ++j)
+        if (i0 + j < n_samples) mv.add(v[j]);
+    }
     float sumInf = 0, sumSup = 0;
     unsigned nInf = 0, nSup = 0;
     bool acc = false;
-    for (int i = 0; i < n_samples; ++i) {
-      const float v = sg[i];
-      if (!acc && v < mv.mean) acc = true;
-      if (acc) {
-        if (v < mv.mean) {
-          sumInf += v;
-          ++nInf;
-        } else {
-          sumSup += v;
-          ++nSup;
+    for (int i0 = 0; i0 < n_samples; i0 += kStatBatch) {
+      float v[kStatBatch];
+#pragma unroll
+      for (int j = 0; j < kStatBatch; ++j) v[j] = sg[min(i0 + j, n_samples - 1)];
+#pragma unroll
+      for (int j = 0; j < kStatBatch; ++j) {
+        if (i0 + j >= n_samples) break;
+        if (!acc && v[j] < mv.mean) acc = true;
+        if (acc) {
+          if (v[j] < mv.mean) {
+            sumInf += v[j];
+            ++nInf;
+          } else {
+            sumSup += v[j];
+            ++nSup;
+          }
         }
       }
     }
     stat_var[tid] = mv.var;
     stat_muw[tid] = sumSup / (float)nSup;
     stat_mub[tid] = sumInf / (float)nInf;
-  } else if (tid == kRefineBlock - 64) {
+  } else if (tid == kFinalBlock - 64) {
     // sample positions x_i along the profile (another wavefront)
     const float stepX = (J.end_sig - J.begin_sig) / ((float)n_samples - 1.f);
     float x = J.begin_sig;
@@ -860,7 +881,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   // order statistic, or the two middle ones summed in float and halved in
   // double, as the host's partial_sort_copy median does
   const int mv = n_valid;
-  for (int q = tid; q < n_samples * mv; q += kRefineBlock) {
+  for (int q = tid; q < n_samples * mv; q += kFinalBlock) {
     const int i = q / mv, j = q - i * mv;
     const float v = sig[valid_cut[j]][i];
     int rank = 0;
@@ -871,7 +892,7 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     ordered[i][rank] = v;
   }
   __syncthreads();
-  for (int i = tid; i < n_samples; i += kRefineBlock) {
+  for (int i = tid; i < n_samples; i += kFinalBlock) {
     float b = 0.f;
     if (mv % 2) {
       b = ordered[i][mv / 2];
@@ -882,34 +903,30 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
     bar[i] = b;
   }
   __syncthreads();
+  float bar_lo = FLT_MAX, bar_hi = -FLT_MAX;  // lane 0's are the block's
   if (tid < 64) {
-    float lo = FLT_MAX, hi = -FLT_MAX;
     for (int i = tid; i < n_samples; i += 64) {
-      lo = fminf(lo, bar[i]);
-      hi = fmaxf(hi, bar[i]);
+      bar_lo = fminf(bar_lo, bar[i]);
+      bar_hi = fmaxf(bar_hi, bar[i]);
     }
     for (int o = 32; o > 0; o >>= 1) {
-      lo = fminf(lo, __shfl_xor(lo, o, 64));
-      hi = fmaxf(hi, __shfl_xor(hi, o, 64));
-    }
-    if (tid == 0) {
-      O.bar_min = lo;
-      O.bar_max = hi;
+      bar_lo = fminf(bar_lo, __shfl_xor(bar_lo, o, 64));
+      bar_hi = fmaxf(bar_hi, __shfl_xor(bar_hi, o, 64));
     }
   }
   // ring parity of every bank profile at every sample position, and each
   // sample's term under either parity (:124-160): the division is done once
   // per sample instead of once per sample and bank row
-  for (int q = tid; q < bank_rows * n_samples; q += kRefineBlock) {
-    const int idc = q / n_samples, i = q - idc * n_samples;
+  for (int q = tid; q < bank_rows * n_samples; q += kFinalBlock) {
+    const int i = q / bank_rows, idc = q - i * bank_rows;
     int ldum = 0;
     for (int j = 0; j < bank_cols; ++j)
       if (ratio[idc * kMaxRatios + j] <= xs[i]) ++ldum;
-    parity[idc][i] = (uint8_t)(ldum & 1);
+    parity[i][idc] = (uint8_t)(ldum & 1);
   }
-  for (int q = tid; q < nc * n_samples; q += kRefineBlock) {
+  for (int q = tid; q < nc * n_samples; q += kFinalBlock) {
     const int k = q / n_samples, i = q - k * n_samples;
-    if (S.sticky[k]) continue;
+    if ((sticky >> k) & 1u) continue;
     const float v = sig[k][i], v2 = 2.0f * stat_var[k];
     const float tb = fmaxf(v - stat_mub[k], 0.f);
     const float tw = fminf(v - stat_muw[k], 0.f);
@@ -918,16 +935,36 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_final(
   }
   __syncthreads();
   // distance of each valid cut to each bank profile, summed in sample order
-  for (int q = tid; q < nc * bank_rows; q += kRefineBlock) {
+  for (int q = tid; q < nc * bank_rows; q += kFinalBlock) {
     const int k = q / bank_rows, idc = q - k * bank_rows;
-    if (S.sticky[k]) continue;
-    const uint8_t* par = parity[idc];
+    if ((sticky >> k) & 1u) continue;
     const float* t0 = term[0][k];
     const float* t1 = term[1][k];
     float distance = 0;
-    for (int i = 0; i < n_samples; ++i) distance += par[i] ? t1[i] : t0[i];
+    constexpr int kSumBatch = 8;  // terms read together, then added in sample order
+    for (int i0 = 0; i0 < n_samples; i0 += kSumBatch) {
+      float v[kSumBatch];
+#pragma unroll
+      for (int j = 0; j < kSumBatch; ++j) {
+        const int i = min(i0 + j, n_samples - 1);
+        v[j] = parity[i][idc] ? t1[i] : t0[i];
+      }
+#pragma unroll
+      for (int j = 0; j < kSumBatch; ++j)
+        if (i0 + j < n_samples) distance += v[j];
+    }
     dist[J.dist_offset + k * bank_rows + idc] = distance;
   }
+  if (tid == 0) {
+    O.ok = 1;
+    O.cx = S.cx;
+    O.cy = S.cy;
+    for (int r = 0; r < 9; ++r) O.H[r] = sH[r];
+    O.min_res = S.min_res;
+    O.bar_min = bar_lo;
+    O.bar_max = bar_hi;
+  }
+  if (tid < nc) O.oob[tid] = (uint8_t)((sticky >> tid) & 1u);
 }
 
 // The centre the grid search moves to after its first iteration (the job's
@@ -962,7 +999,7 @@ int launch_refine_readout(const RefineJobDev* jobs, int n_jobs, int n_samples, c
                           int bank_rows, int bank_cols, const RefineWork& work, RefineOutDev* out,
                           float* dist, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, kMaxGrid, inv_ratio, bank_rows,
+  k_refine_final<<<n_jobs, kFinalBlock, 0, s>>>(jobs, n_samples, kMaxGrid, inv_ratio, bank_rows,
                                                  bank_cols, work, H, centers, cost, out, dist);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -986,7 +1023,7 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
     if (!fused && it + 1 < max_iter)
       k_refine_pick<<<n_jobs, kRefineBlock, 0, s>>>(jobs, it, grid_n, work);
   }
-  k_refine_final<<<n_jobs, kRefineBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
+  k_refine_final<<<n_jobs, kFinalBlock, 0, s>>>(jobs, n_samples, grid_n, inv_ratio, bank_rows,
                                                  bank_cols, work, nullptr, nullptr, nullptr, out,
                                                  dist);
   return hipGetLastError() == hipSuccess ? 0 : -2;
