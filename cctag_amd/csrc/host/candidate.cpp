@@ -231,27 +231,96 @@ class MarkBuf {
   int n_ = 0;
 };
 
-// The last `window` gradient angles of a linking direction (a ring).
-class AngleWindow {
+// The last `window` points of a linking direction (a ring), whose gradient
+// angles the reference keeps.
+class PointWindow {
  public:
-  explicit AngleWindow(size_t window) : w_(window) {
-    thread_local std::vector<float> storage;
+  explicit PointWindow(size_t window) : w_(window) {
+    thread_local std::vector<int> storage;
     if (storage.size() < w_) storage.resize(w_);
     v_ = storage.data();
   }
-  void push(float a) {
-    v_[(first_ + n_) % w_] = a;
+  void push(int q) {
+    size_t at = first_ + n_;
+    if (at >= w_) at -= w_;
+    v_[at] = q;
+    last_ = at;
     if (n_ < w_) ++n_;
-    else first_ = (first_ + 1) % w_;
+    else if (++first_ == w_) first_ = 0;
   }
   size_t size() const { return n_; }
-  float front() const { return v_[first_]; }
-  float back() const { return v_[(first_ + n_ - 1) % w_]; }
+  int front() const { return v_[first_]; }
+  int back() const { return v_[last_]; }
 
  private:
-  size_t w_, first_ = 0, n_ = 0;
-  float* v_;
+  size_t w_, first_ = 0, n_ = 0, last_ = 0;
+  int* v_;
 };
+
+// The linking's two decisions on gradient angles (Vote.cpp:290-340), where
+// an angle is atan2 of the gradient mapped to [0, 2pi) in float.  Both only
+// ask on which side of a few directions an angle or an angle difference
+// lies, so away from those directions they follow from the gradient vectors
+// themselves: the sine and cosine of a difference are the normalised cross
+// and dot products (exact integers in double), an angle's sector is its
+// nearest multiple of pi/4.  The float angles differ from the exact ones by
+// less than 1e-6 rad; within kAngleMargin = 1e-4 of a boundary the
+// reference's float expressions are evaluated instead, so every decision is
+// the one the float angles give.
+constexpr double kAngleMargin = 1e-4;
+
+// round(((angle + pi/4) / 2pi) * 8) - 1 modulo 8: the first neighbour
+// direction to try.
+static int linking_shift(const EdgeSet& es, int q) {
+  const EdgePoint& e = es.pts[q];
+  const double gx = e.gx, gy = e.gy, n = std::sqrt(gx * gx + gy * gy);
+  if (n > 0.0) {
+    static const double r = 0.70710678118654752;
+    static const double cx[8] = {1, r, 0, -r, -1, -r, 0, r}, cy[8] = {0, r, 1, r, 0, -r, -1, -r};
+    int best = 0;
+    double d1 = -2.0 * n, d2 = -2.0 * n;  // largest and second largest projections
+    for (int k = 0; k < 8; ++k) {
+      const double d = gx * cx[k] + gy * cy[k];
+      if (d > d1) {
+        d2 = d1;
+        d1 = d;
+        best = k;
+      } else if (d > d2) {
+        d2 = d;
+      }
+    }
+    // next to a sector boundary the two nearest directions project alike:
+    // d1 - d2 = 2 sin(pi/8) n sin(distance to the boundary)
+    if (d1 - d2 > 0.7653668647301796 * n * kAngleMargin) return best;
+  }
+  const float angle = es.gradient_angle(q);
+  const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
+  return ((shifting % 8) + 8) % 8;
+}
+
+// The convexity test between the oldest (front) and newest (back) point of
+// the window: with a full window dir * sin(back - front) < 0, otherwise
+// s < -0.707 with c > 0, or s < 0 with c < 0, for s = dir * sin(back -
+// front) and c = cos(back - front).
+static bool convexity_lost(const EdgeSet& es, int front, int back, int dir, bool full) {
+  const EdgePoint &f = es.pts[front], &b = es.pts[back];
+  const double fx = f.gx, fy = f.gy, bx = b.gx, by = b.gy;
+  const double nn = std::sqrt((fx * fx + fy * fy) * (bx * bx + by * by));
+  if (nn > 0.0) {
+    const double sn = dir * (by * fx - bx * fy) / nn, cs = (bx * fx + by * fy) / nn;
+    const double s_thr = (double)-0.707f;
+    if (full) {
+      if (std::fabs(sn) > kAngleMargin) return sn < 0.0;
+    } else if (std::fabs(sn) > kAngleMargin && std::fabs(cs) > kAngleMargin &&
+               std::fabs(sn - s_thr) > kAngleMargin) {
+      return ((sn < s_thr) && (cs > 0.0)) || ((sn < 0.0) && (cs < 0.0));
+    }
+  }
+  const float d = es.gradient_angle(back) - es.gradient_angle(front);
+  if (full) return dir * std::sin(d) < 0.0f;
+  const float s = dir * std::sin(d), c = std::cos(d);
+  return ((s < -0.707f) && (c > 0.f)) || ((s < 0.f) && (c < 0.f));
+}
 
 // Vote.cpp:255-394: convex edge linking from a seed in both directions.
 // Points the reference flags processed_in are appended to `marks` (the
@@ -259,7 +328,7 @@ class AngleWindow {
 static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, int dir,
                              SegBuf& seg, size_t window, float average_vote_min,
                              MarkBuf& marks) {
-  AngleWindow phi(window);
+  PointWindow phi(window);
   size_t i = 0;
   bool found = true;
   int stop = 0;
@@ -268,9 +337,8 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   static const int xoff[8] = {1, 1, 0, -1, -1, -1, 0, 1};
   const int yoff[8] = {0, dir * -1, dir * -1, dir * -1, 0, dir * 1, dir * 1, dir * 1};
   while ((i < maxLength) && found && (averageVote >= average_vote_min)) {
-    const float angle = es.gradient_angle(p);
-    phi.push(angle);
-    const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
+    phi.push(p);
+    const int shifting = linking_shift(es, p);
     int j = 0;
     stop = 0;
     while (!stop) {
@@ -282,13 +350,8 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
         const int sy = es.pts[p].y + yoff[k];
         if (sx >= 0 && sx < es.shape_w() && sy >= 0 && sy < es.shape_h() && es.at(sx, sy) >= 0 &&
             !processed.contains(packxy(sx, sy))) {
-          if (phi.size() == window) {
-            if (dir * std::sin(phi.back() - phi.front()) < 0.0f) stop = kConvexityLost;
-          } else {
-            const float s = dir * std::sin(phi.back() - phi.front());
-            const float c = std::cos(phi.back() - phi.front());
-            if (((s < -0.707f) && (c > 0.f)) || ((s < 0.f) && (c < 0.f))) stop = kConvexityLost;
-          }
+          if (convexity_lost(es, phi.front(), phi.back(), dir, phi.size() == window))
+            stop = kConvexityLost;
           if (stop == 0) {
             processed.insert(packxy(es.pts[p].x, es.pts[p].y));
             p = es.at(sx, sy);

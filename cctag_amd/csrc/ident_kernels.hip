@@ -848,19 +848,18 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
       float v[kStatBatch];
 #pragma unroll
       for (int j = 0; j < kStatBatch; ++j) v[j] = sg[min(i0 + j, n_samples - 1)];
+      // without branches: the signal is non-negative, so adding +0 to a
+      // sum leaves it unchanged
 #pragma unroll
       for (int j = 0; j < kStatBatch; ++j) {
-        if (i0 + j >= n_samples) break;
-        if (!acc && v[j] < mv.mean) acc = true;
-        if (acc) {
-          if (v[j] < mv.mean) {
-            sumInf += v[j];
-            ++nInf;
-          } else {
-            sumSup += v[j];
-            ++nSup;
-          }
-        }
+        const bool live = i0 + j < n_samples;
+        const bool lo = live && v[j] < mv.mean;
+        acc = acc || lo;
+        const bool take_lo = acc && lo, take_hi = live && acc && !lo;
+        sumInf += take_lo ? v[j] : 0.f;
+        nInf += take_lo;
+        sumSup += take_hi ? v[j] : 0.f;
+        nSup += take_hi;
       }
     }
     stat_var[tid] = mv.var;
