@@ -97,7 +97,6 @@ void EdgeSet::prepare(const LevelEdges& L) {
   vlist_ = L.vlist;
   pts.resize(L.n);
   pin_.assign(L.n, 0);
-  angle_.assign(L.n, kNoAngle);
   seeds.assign(L.seeds, L.seeds + L.n_seeds);
 }
 
@@ -114,9 +113,6 @@ void EdgeSet::rank_words(size_t w0, size_t w1, int32_t before) {
   }
 }
 
-float EdgeSet::compute_angle(const EdgePoint& e) {
-  return std::fmod(std::atan2(e.gy, e.gx) + 2.0f * kPi, 2.0f * kPi);
-}
 
 void EdgeSet::fill(const LevelEdges& L, int begin, int end) {
   for (int i = begin; i < end; ++i) {
@@ -257,23 +253,23 @@ class PointWindow {
   int* v_;
 };
 
-// The linking's two decisions on gradient angles (Vote.cpp:290-340), where
-// an angle is atan2 of the gradient mapped to [0, 2pi) in float.  Both only
-// ask on which side of a few directions an angle or an angle difference
-// lies, so away from those directions they follow from the gradient vectors
-// themselves: the sine and cosine of a difference are the normalised cross
-// and dot products (exact integers in double), an angle's sector is its
-// nearest multiple of pi/4.  The float angles differ from the exact ones by
-// less than 1e-6 rad; within kAngleMargin = 1e-4 of a boundary the
-// reference's float expressions are evaluated instead, so every decision is
-// the one the float angles give.
+// The linking's two decisions on gradient angles.  Both only ask on which
+// side of a few directions an angle or an angle difference lies, so away
+// from those directions they follow from the gradient vectors themselves:
+// the sine and cosine of a difference are the normalised cross and dot
+// products (exact integers in double), an angle's sector is its nearest
+// multiple of pi/4.  The float angles differ from the exact ones by less
+// than 1e-6 rad; within kAngleMargin = 1e-4 of a boundary the float
+// expressions are evaluated instead, so every decision is the one the float
+// angles give.
 constexpr double kAngleMargin = 1e-4;
 
-// round(((angle + pi/4) / 2pi) * 8) - 1 modulo 8: the first neighbour
-// direction to try.
-static int linking_shift(const EdgeSet& es, int q) {
-  const EdgePoint& e = es.pts[q];
-  const double gx = e.gx, gy = e.gy, n = std::sqrt(gx * gx + gy * gy);
+float gradient_angle_of(float gx, float gy) {
+  return std::fmod(std::atan2(gy, gx) + 2.0f * kPi, 2.0f * kPi);
+}
+
+int linking_shift(float gxf, float gyf) {
+  const double gx = gxf, gy = gyf, n = std::sqrt(gx * gx + gy * gy);
   if (n > 0.0) {
     static const double r = 0.70710678118654752;
     static const double cx[8] = {1, r, 0, -r, -1, -r, 0, r}, cy[8] = {0, r, 1, r, 0, -r, -1, -r};
@@ -293,18 +289,13 @@ static int linking_shift(const EdgeSet& es, int q) {
     // d1 - d2 = 2 sin(pi/8) n sin(distance to the boundary)
     if (d1 - d2 > 0.7653668647301796 * n * kAngleMargin) return best;
   }
-  const float angle = es.gradient_angle(q);
+  const float angle = gradient_angle_of(gxf, gyf);
   const int shifting = (int)std::round(((angle + kPi / 4.0f) / (2.0f * kPi)) * 8.0f) - 1;
   return ((shifting % 8) + 8) % 8;
 }
 
-// The convexity test between the oldest (front) and newest (back) point of
-// the window: with a full window dir * sin(back - front) < 0, otherwise
-// s < -0.707 with c > 0, or s < 0 with c < 0, for s = dir * sin(back -
-// front) and c = cos(back - front).
-static bool convexity_lost(const EdgeSet& es, int front, int back, int dir, bool full) {
-  const EdgePoint &f = es.pts[front], &b = es.pts[back];
-  const double fx = f.gx, fy = f.gy, bx = b.gx, by = b.gy;
+bool convexity_lost(float fxf, float fyf, float bxf, float byf, int dir, bool full) {
+  const double fx = fxf, fy = fyf, bx = bxf, by = byf;
   const double nn = std::sqrt((fx * fx + fy * fy) * (bx * bx + by * by));
   if (nn > 0.0) {
     const double sn = dir * (by * fx - bx * fy) / nn, cs = (bx * fx + by * fy) / nn;
@@ -316,7 +307,7 @@ static bool convexity_lost(const EdgeSet& es, int front, int back, int dir, bool
       return ((sn < s_thr) && (cs > 0.0)) || ((sn < 0.0) && (cs < 0.0));
     }
   }
-  const float d = es.gradient_angle(back) - es.gradient_angle(front);
+  const float d = gradient_angle_of(bxf, byf) - gradient_angle_of(fxf, fyf);
   if (full) return dir * std::sin(d) < 0.0f;
   const float s = dir * std::sin(d), c = std::cos(d);
   return ((s < -0.707f) && (c > 0.f)) || ((s < 0.f) && (c < 0.f));
@@ -338,7 +329,7 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   const int yoff[8] = {0, dir * -1, dir * -1, dir * -1, 0, dir * 1, dir * 1, dir * 1};
   while ((i < maxLength) && found && (averageVote >= average_vote_min)) {
     phi.push(p);
-    const int shifting = linking_shift(es, p);
+    const int shifting = linking_shift(es.pts[p].gx, es.pts[p].gy);
     int j = 0;
     stop = 0;
     while (!stop) {
@@ -350,7 +341,8 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
         const int sy = es.pts[p].y + yoff[k];
         if (sx >= 0 && sx < es.shape_w() && sy >= 0 && sy < es.shape_h() && es.at(sx, sy) >= 0 &&
             !processed.contains(packxy(sx, sy))) {
-          if (convexity_lost(es, phi.front(), phi.back(), dir, phi.size() == window))
+          const EdgePoint &f = es.pts[phi.front()], &b = es.pts[phi.back()];
+          if (convexity_lost(f.gx, f.gy, b.gx, b.gy, dir, phi.size() == window))
             stop = kConvexityLost;
           if (stop == 0) {
             processed.insert(packxy(es.pts[p].x, es.pts[p].y));

@@ -80,22 +80,6 @@ class EdgeSet {
   // Types.hpp:142-155: an invalid index throws std::out_of_range.
   void set_processed_in(int i, bool f) { pin_.at(check(i)) = f; }
   bool test_processed_in(int i) const { return pin_.at(check(i)); }
-  // fmod(atan2(gy, gx) + 2 pi, 2 pi) of point i (Vote.cpp:296), computed on
-  // first use and kept; concurrent callers store the same bits.
-  float gradient_angle(int i) const {
-    uint32_t bits = __atomic_load_n(&angle_[i], __ATOMIC_RELAXED);
-    if (bits == kNoAngle) {
-      const float a = compute_angle(pts[i]);
-      __builtin_memcpy(&bits, &a, 4);
-      __atomic_store_n(&angle_[i], bits, __ATOMIC_RELAXED);
-      return a;
-    }
-    float a;
-    __builtin_memcpy(&a, &bits, 4);
-    return a;
-  }
-  static float compute_angle(const EdgePoint& e);
-
   std::vector<EdgePoint> pts;
   std::vector<int32_t> seeds;
 
@@ -109,8 +93,6 @@ class EdgeSet {
   std::vector<int32_t> rank_;  // set bits before each word
   const int32_t *before_ = nullptr, *after_ = nullptr, *vindex_ = nullptr, *vlist_ = nullptr;
   std::vector<uint8_t> pin_;
-  static constexpr uint32_t kNoAngle = 0xFFFFFFFFu;  // a NaN no angle computation yields
-  mutable std::vector<uint32_t> angle_;
 };
 
 // The detection parameters the host stages read (Params.hpp:108-199).
@@ -184,6 +166,18 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
 // one task per marker (own rand_5_k stream seeded 271828).
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
                      TaskPool* pool);
+
+// The edge linking's decisions on gradient angles (Vote.cpp:290-340), as
+// functions of the gradients: the first neighbour direction to try,
+// round(((angle + pi/4) / 2pi) * 8) - 1 modulo 8, and the convexity test
+// between the window's oldest (f) and newest (b) gradient direction (with a
+// full window dir * sin(b - f) < 0, otherwise s < -0.707 with c > 0 or s < 0
+// with c < 0 for s = dir * sin(b - f), c = cos(b - f)), angles being
+// gradient_angle_of in float.  They give the float expressions' results.
+// fmod(atan2(gy, gx) + 2 pi, 2 pi) (Vote.cpp:296).
+float gradient_angle_of(float gx, float gy);
+int linking_shift(float gx, float gy);
+bool convexity_lost(float fx, float fy, float bx, float by, int dir, bool full);
 
 // Multiresolution.cpp:172-193 over a whole list: `order` names markers in
 // arrival order; the result names the kept ones (an entry replaced by a

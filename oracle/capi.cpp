@@ -5,6 +5,7 @@
 #include "oracle.hpp"
 
 #include <chrono>
+#include <cmath>
 #include <exception>
 #include <memory>
 #include <vector>
@@ -160,6 +161,32 @@ int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w
 int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
                   cctagx_marker* out, int max_out) {
   return oracle_detect_threads(params, img, w, h, out, max_out, 1);
+}
+
+// The edge linking's angle decisions of the shared candidate stage
+// (cctagx::linking_shift, cctagx::convexity_lost) against the reference's
+// float expressions on gradient angles (Vote.cpp:296-340), for n cases
+// g[5i .. 5i+4] = (gx_front, gy_front, gx_back, gy_back, dir): the number of
+// cases in which any of the three decisions differs.  The first neighbour
+// direction is compared modulo 8, the only way the linking uses it.
+int oracle_linking_mismatches(const float* g, int n) {
+  const float pi = 3.14159265358979323846f;
+  auto angle = [&](float gx, float gy) { return std::fmod(std::atan2(gy, gx) + 2.0f * pi, 2.0f * pi); };
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    const float fx = g[5 * i], fy = g[5 * i + 1], bx = g[5 * i + 2], by = g[5 * i + 3];
+    const int dir = g[5 * i + 4] < 0.f ? -1 : 1;
+    const float af = angle(fx, fy), ab = angle(bx, by);
+    const int shifting = (int)std::round(((af + pi / 4.0f) / (2.0f * pi)) * 8.0f) - 1;
+    const bool full = dir * std::sin(ab - af) < 0.0f;
+    const float s = dir * std::sin(ab - af), c = std::cos(ab - af);
+    const bool part = ((s < -0.707f) && (c > 0.f)) || ((s < 0.f) && (c < 0.f));
+    if (cctagx::linking_shift(fx, fy) != ((shifting % 8) + 8) % 8 ||
+        cctagx::convexity_lost(fx, fy, bx, by, dir, true) != full ||
+        cctagx::convexity_lost(fx, fy, bx, by, dir, false) != part)
+      ++bad;
+  }
+  return bad;
 }
 
 // Wall time of the full CPU detection (front end + candidates + identification)

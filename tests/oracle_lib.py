@@ -48,6 +48,7 @@ def load():
     o.oracle_time_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
+    o.oracle_linking_mismatches.argtypes = [vp, ctypes.c_int]
     _o = o
     return o
 
@@ -162,3 +163,11 @@ def grad_cost(img: np.ndarray, H, stops, n_samples: int = 100, begin: float = 0.
     c = o.oracle_grad_cost(img.ctypes.data, w, h, Hd.ctypes.data, st.ctypes.data, len(st), n_samples,
                            begin, end, g.ctypes.data if grad else None, ctypes.byref(nv))
     return c, (g.reshape(3, 3) if grad else None), nv.value
+
+
+def linking_mismatches(cases: np.ndarray) -> int:
+    """Cases (n x 5 float32: front gradient, back gradient, dir) in which the
+    candidate stage's linking decisions differ from the float-angle
+    expressions."""
+    c = np.ascontiguousarray(cases, dtype=np.float32)
+    return load().oracle_linking_mismatches(c.ctypes.data, c.shape[0])
