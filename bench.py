@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="bound of the CPU baseline sample (whole frames, at least one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pipes", action="store_true",
+                    help="skip the concurrent-pipes comparison")
     ap.add_argument("--batch-frames", type=int, default=64,
                     help="frames of the uneven-size batch (0 skips the batched comparison)")
     ap.add_argument("--refine-markers", type=int, default=4096,
@@ -378,7 +380,7 @@ def main():
             work[k] += ds[k]
     det.set_stage_timing(False)
 
-    pipes = pipes_comparison(img, Detector, markers)
+    pipes = None if args.no_pipes else pipes_comparison(img, Detector, markers)
     progress("concurrent pipes done")
 
     # the drop-in call with the frame in host memory (upload included)
