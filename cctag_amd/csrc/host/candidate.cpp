@@ -16,6 +16,7 @@
 #include <exception>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <unordered_set>
@@ -1032,6 +1033,8 @@ static void loop_two_iteration(std::vector<Marker>& markers, const EdgeSet& es,
   }
 }
 
+static void rescale_marker(Marker& m, const EdgeSet& level0);
+
 void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
                        const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
                        HostTimes* times) {
@@ -1161,38 +1164,151 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     });
   }
   lap(&HostTimes::loop_one);
-  // Loop two: one task per (level, candidate) (Detection.cpp:629-658).
+  // Loops two and three and the level-0 re-fit as one flow of tasks
+  // (Detection.cpp:629-688, Multiresolution.cpp:319-401): one loop-two task
+  // per (level, candidate); a level's segment labels are assigned by the
+  // task that ends its loop two, which then queues the level's loop-three
+  // tasks; each marker a loop-three task finds above level 0 is queued for
+  // its re-fit.  A thread takes loop-two tasks first, then queued ones.  The
+  // loop-two tasks of a level flip their own bits of its points' candidate
+  // masks, the labels write its points' segment numbers, loop three and the
+  // re-fit only read the edge sets, and every task writes only its own
+  // candidate or marker, so the results (and their order) are those of the
+  // three loops run one after another.
   std::vector<std::vector<LoopTwoTask>> two(nl);
-  std::vector<std::pair<int, int>> jobs;
+  std::vector<std::pair<int, int>> jobs2;
   for (int l = 0; l < nl; ++l) {
     const size_t n = std::min(loopOne[l].size(), (size_t)p.maximum_nb_candidates_loop_two);
     two[l].resize(n);
     for (size_t i = 0; i < n; ++i) {
       two[l][i].cand = std::move(*loopOne[l][i]);
-      jobs.emplace_back(l, (int)i);
+      jobs2.emplace_back(l, (int)i);
     }
   }
-  parallel_for(pool, (int)jobs.size(), [&](int j) {
-    const int l = jobs[j].first, i = jobs[j].second;
-    complete_flow_component(two[l][i], levels[l], (size_t)i, p);
-  });
-  lap(&HostTimes::loop_two);
   std::vector<std::vector<Candidate>> loopTwo(nl);
-  for (int l = 0; l < nl; ++l) assign_labels(levels[l], two[l], loopTwo[l]);
-  lap(&HostTimes::labels);
-  // Loop three: one task per (level, candidate) (Detection.cpp:675-688).
-  jobs.clear();
+  std::vector<std::vector<std::vector<Marker>>> found(nl);  // [level][candidate]
+  struct Queued {
+    int l, i, k;  // k < 0: loop three of candidate i of level l; else re-fit of its marker k
+  };
+  std::mutex qm;
+  std::deque<Queued> queue;
+  std::exception_ptr error;  // the first exception a task raised (under qm)
+  const int n2 = (int)jobs2.size();
+  std::atomic<int> next2{0}, pending{n2};  // pending: tasks queued, running or to hand out
+  std::unique_ptr<std::atomic<int>[]> left2(new std::atomic<int>[nl]);
+  for (int l = 0; l < nl; ++l) left2[l].store((int)two[l].size(), std::memory_order_relaxed);
+  using Tick = Clock::duration::rep;
+  std::atomic<Tick> end2{0}, end3{0}, labels_ticks{0};
+  auto stamp = [](std::atomic<Tick>& at) {
+    const Tick now = Clock::now().time_since_epoch().count();
+    Tick cur = at.load(std::memory_order_relaxed);
+    while (now > cur && !at.compare_exchange_weak(cur, now, std::memory_order_relaxed)) {
+    }
+  };
+  auto guarded = [&](auto&& fn) {
+    try {
+      fn();
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(qm);
+      if (!error) error = std::current_exception();
+    }
+  };
+  // labels of level l, then its loop-three tasks
+  auto finish_level = [&](int l) {
+    const auto a = Clock::now();
+    guarded([&] { assign_labels(levels[l], two[l], loopTwo[l]); });
+    labels_ticks.fetch_add((Clock::now() - a).count(), std::memory_order_relaxed);
+    found[l].resize(loopTwo[l].size());
+    const int n = (int)loopTwo[l].size();
+    if (n == 0) return;
+    // the candidates that will search for another segment (the long tasks)
+    // go to the front of the queue, as do the re-fits a finished task asks
+    // for (the last step of its chain)
+    std::vector<uint8_t> long_task(n, 0);
+    if (p.search_for_another_segment)
+      for (int i = 0; i < n; ++i) {
+        const Candidate& c = loopTwo[l][i];
+        const float quality = (float)c.outer_points.size() / (float)rasterize_ellipse_perimeter(c.outer);
+        long_task[i] = quality > 0.25 && quality < 0.7;
+      }
+    pending.fetch_add(n, std::memory_order_relaxed);
+    std::lock_guard<std::mutex> lk(qm);
+    for (int i = n - 1; i >= 0; --i)
+      if (long_task[i]) queue.push_front(Queued{l, i, -1});
+    for (int i = 0; i < n; ++i)
+      if (!long_task[i]) queue.push_back(Queued{l, i, -1});
+  };
+  for (int l = 0; l < nl; ++l)
+    if (two[l].empty()) finish_level(l);
+  auto work = [&](int) {
+    for (int spins = 0;;) {
+      const int k = next2.fetch_add(1, std::memory_order_relaxed);
+      if (k < n2) {
+        const int l = jobs2[k].first, i = jobs2[k].second;
+        guarded([&] { complete_flow_component(two[l][i], levels[l], (size_t)i, p); });
+        if (left2[l].fetch_sub(1, std::memory_order_acq_rel) == 1) finish_level(l);
+        stamp(end2);
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      Queued t{-1, -1, -1};
+      {
+        std::lock_guard<std::mutex> lk(qm);
+        if (!queue.empty()) {
+          t = queue.front();
+          queue.pop_front();
+        }
+      }
+      if (t.l >= 0 && t.k < 0) {
+        std::vector<Marker>& out = found[t.l][t.i];
+        guarded([&] {
+          Pcg32 rng(271828u);
+          loop_two_iteration(out, levels[t.l], loopTwo[t.l], (size_t)t.i, t.l,
+                             (float)std::pow(2.0, t.l), p, rng);
+        });
+        int refits = 0;
+        for (Marker& m : out) {
+          if (m.pyramid_level > 0) ++refits;
+          else m.rescaled_outer_ellipse_points = m.points.back();
+        }
+        if (refits > 0) {
+          pending.fetch_add(refits, std::memory_order_relaxed);
+          std::lock_guard<std::mutex> lk(qm);
+          for (int m = (int)out.size() - 1; m >= 0; --m)
+            if (out[m].pyramid_level > 0) queue.push_front(Queued{t.l, t.i, m});
+        }
+        stamp(end3);
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      if (t.l >= 0) {
+        guarded([&] { rescale_marker(found[t.l][t.i][t.k], levels[0]); });
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      if (pending.load(std::memory_order_acquire) == 0) return;
+      if (++spins % 64 == 0) std::this_thread::yield();
+    }
+  };
+  const auto t_flow = Clock::now();
+  parallel_for(pool, pool ? pool->threads() : 1, work);
+  if (error) std::rethrow_exception(error);
   for (int l = nl - 1; l >= 0; --l)
-    for (size_t i = 0; i < loopTwo[l].size(); ++i) jobs.emplace_back(l, (int)i);
-  std::vector<std::vector<Marker>> found(jobs.size());
-  parallel_for(pool, (int)jobs.size(), [&](int j) {
-    const int l = jobs[j].first, i = jobs[j].second;
-    Pcg32 rng(271828u);
-    loop_two_iteration(found[j], levels[l], loopTwo[l], (size_t)i, l, (float)std::pow(2.0, l), p, rng);
-  });
-  for (std::vector<Marker>& f : found)
-    for (Marker& m : f) markers.push_back(std::move(m));
-  lap(&HostTimes::loop_three);
+    for (std::vector<Marker>& f : found[l])
+      for (Marker& m : f) markers.push_back(std::move(m));
+  if (times) {
+    // loop two: until its last task ended; loop three: from there until its
+    // last task ended; re-fit: the rest.  The labels' own time (inside the
+    // other two) is reported beside them.
+    using Ms = std::chrono::duration<double, std::milli>;
+    const Clock::time_point e2(Clock::duration(end2.load())), e3(Clock::duration(end3.load()));
+    const Clock::time_point t_end = Clock::now();
+    const Clock::time_point a2 = std::max(e2, t_flow), a3 = std::max(e3, a2);
+    times->loop_two += Ms(a2 - t_flow).count();
+    times->loop_three += Ms(a3 - a2).count();
+    times->rescale += Ms(t_end - a3).count();
+    times->labels += Ms(Clock::duration(labels_ticks.load())).count();
+  }
 }
 
 // Multiresolution.cpp:43-170.
@@ -1226,42 +1342,44 @@ static bool intersect_line_two_ellipses(int y, const Ellipse& qIn, const Ellipse
   return true;
 }
 
+// The re-fit of one marker of a level above 0 (own rand_5_k stream).
+static void rescale_marker(Marker& m, const EdgeSet& level0) {
+  Pcg32 rng(271828u);
+  const float scale = m.scale;
+  Ellipse rescaled = m.rescaled_outer_ellipse;
+  std::vector<int> hull;
+  Ellipse qIn, qOut;
+  compute_hull(rescaled, scale, qIn, qOut);
+  const float yCenter = rescaled.cy();
+  const int maxY = std::max((int)yCenter, 0);
+  const int minY = std::min((int)yCenter, level0.shape_h() - 1);
+  for (int y = maxY; y < level0.shape_h(); ++y)
+    if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
+  for (int y = minY; y >= 0; --y)
+    if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
+  if (hull.size() < 5) return;
+  std::vector<int> refit;
+  float SmFinal = 1e+10f;
+  outlier_removal(level0, hull, refit, SmFinal, 20.0f, kNoWeight, 60, rng);
+  if (refit.size() < 5) return;
+  try {
+    fit_ellipse(as_points(level0, refit), rescaled);
+    std::vector<Pt2> pts;
+    for (int q : refit) pts.push_back(pt_of(level0.pts[q]));
+    m.center_img = Pt2(m.center_img.x * scale, m.center_img.y * scale);
+    m.rescaled_outer_ellipse = rescaled;
+    m.rescaled_outer_ellipse_points = pts;
+  } catch (...) {
+  }
+}
+
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
                      TaskPool* pool) {
   (void)p;
   parallel_for(pool, (int)markers.size(), [&](int im) {
     Marker& m = markers[im];
-    if (m.pyramid_level > 0) {
-      Pcg32 rng(271828u);
-      const float scale = m.scale;
-      Ellipse rescaled = m.rescaled_outer_ellipse;
-      std::vector<int> hull;
-      Ellipse qIn, qOut;
-      compute_hull(rescaled, scale, qIn, qOut);
-      const float yCenter = rescaled.cy();
-      const int maxY = std::max((int)yCenter, 0);
-      const int minY = std::min((int)yCenter, level0.shape_h() - 1);
-      for (int y = maxY; y < level0.shape_h(); ++y)
-        if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
-      for (int y = minY; y >= 0; --y)
-        if (!intersect_line_two_ellipses(y, qIn, qOut, level0, hull)) break;
-      if (hull.size() < 5) return;
-      std::vector<int> refit;
-      float SmFinal = 1e+10f;
-      outlier_removal(level0, hull, refit, SmFinal, 20.0f, kNoWeight, 60, rng);
-      if (refit.size() < 5) return;
-      try {
-        fit_ellipse(as_points(level0, refit), rescaled);
-        std::vector<Pt2> pts;
-        for (int q : refit) pts.push_back(pt_of(level0.pts[q]));
-        m.center_img = Pt2(m.center_img.x * scale, m.center_img.y * scale);
-        m.rescaled_outer_ellipse = rescaled;
-        m.rescaled_outer_ellipse_points = pts;
-      } catch (...) {
-      }
-    } else {
-      m.rescaled_outer_ellipse_points = m.points.back();
-    }
+    if (m.pyramid_level > 0) rescale_marker(m, level0);
+    else m.rescaled_outer_ellipse_points = m.points.back();
   });
 }
 

@@ -143,13 +143,16 @@ struct Marker {
 
 // cctagDetectionFromEdges (Detection.cpp:568-693) for every level at once,
 // appending markers coarse level first.  Loop one (seed -> flow component)
-// runs per level in seed order; loop two (completeFlowComponent) and the
-// final candidate loop run as independent tasks over all levels on `pool`
-// (serially when pool is null), each task with its own rand_5_k stream
-// seeded 271828 and its own auxiliary point flags; the segment labels of
-// loop two are assigned afterwards in candidate order.  The result does not
-// depend on the number of threads.
-// Wall time of the host stages of one detection, milliseconds.
+// runs per level in seed order; loop two (completeFlowComponent), the final
+// candidate loop and the level-0 re-fit run as independent tasks over all
+// levels on `pool` (serially when pool is null), each task with its own
+// rand_5_k stream seeded 271828 and its own auxiliary point flags; a level's
+// segment labels are assigned in candidate order once its loop two is done.
+// The result does not depend on the number of threads.
+// Wall time of the host stages of one detection, milliseconds.  Loop two,
+// loop three and the re-fit overlap: loop_two runs until the last loop-two
+// task ends, loop_three from there until the last loop-three task ends,
+// rescale the rest; labels is the label assignment's own time inside them.
 struct HostTimes {
   double load = 0, loop_one = 0, loop_two = 0, labels = 0, loop_three = 0, rescale = 0,
          ident = 0, finish = 0;
@@ -158,12 +161,16 @@ struct HostTimes {
   double ident_prep = 0, ident_select = 0, ident_refine = 0, ident_post = 0;
 };
 
+// The candidate loops of every level (Detection.cpp:603-688) and the level-0
+// re-fit of the markers found above level 0 (Multiresolution.cpp:319-401):
+// the markers in level order (coarsest first), then candidate order.
 void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& level_h,
                        const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
                        HostTimes* times = nullptr);
 
 // Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges,
-// one task per marker (own rand_5_k stream seeded 271828).
+// one task per marker (own rand_5_k stream seeded 271828); what
+// detect_candidates does for the markers it finds.
 void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
                      TaskPool* pool);
 

@@ -66,9 +66,6 @@ void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p
   });
   if (times) times->load += ms_since(t);
   detect_candidates(es, level_h, p, pool, markers, times);
-  t = Clock::now();
-  rescale_markers(markers, es[0], p, pool);
-  if (times) times->rescale += ms_since(t);
 }
 
 void finish_markers(std::vector<Marker>& markers, std::vector<Marker>& out) {
