@@ -23,6 +23,20 @@ const char* last_error();
     }                                                                             \
   } while (0)
 
+// Waits for the work queued on `s` by polling it.  The runtime's blocking
+// wait may put the thread to sleep, and a sleeping thread wakes well after a
+// short launch sequence has ended; the detection path waits on the GPU
+// several times per frame, each time for less than a millisecond.
+inline hipError_t stream_wait(hipStream_t s) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e != hipErrorNotReady) return e;
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+}
+
 // One device allocation carved into 256-byte aligned sub-arrays.
 class DeviceArena {
  public:

@@ -101,7 +101,7 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
     throw std::runtime_error("k_select_cuts launch failed");
   hipEventRecord(ev_[1], s);
   const SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
-  hip_check(hipStreamSynchronize(s), "sync select");
+  hip_check(stream_wait(s), "sync select");
   hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
   select_us_ *= 1000.f;
   for (int i = 0; i < n; ++i) {
@@ -251,7 +251,7 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   hipEventRecord(ev_[3], s);
   const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
   const float* hd = h_ref_dist_.as<float>();
-  hip_check(hipStreamSynchronize(s), "sync refine");
+  hip_check(stream_wait(s), "sync refine");
   hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
   refine_us_ *= 1000.f;
   for (int i = 0; i < n; ++i) {

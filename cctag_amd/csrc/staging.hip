@@ -53,7 +53,7 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
   if (count > 0)
     CCTAGX_HIP(hipMemcpyAsync(cnt, b.counts + first, sizeof(PlaneCounts) * count,
                               hipMemcpyDeviceToHost, s));
-  CCTAGX_HIP(hipStreamSynchronize(s));
+  CCTAGX_HIP(stream_wait(s));
   counts_.assign(cnt, cnt + count);
 
   // Segment list: per plane, the arrays in a fixed order.
@@ -106,7 +106,7 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
     CCTAGX_HIP(hipMemcpyAsync(h_packed_.as<uint8_t>(), d_packed_.as<uint8_t>(), at,
                               hipMemcpyDeviceToHost, s));
   }
-  CCTAGX_HIP(hipStreamSynchronize(s));
+  CCTAGX_HIP(stream_wait(s));
 
   const uint8_t* base = h_packed_.as<uint8_t>();
   views.resize(count);
