@@ -319,7 +319,7 @@ bool convexity_lost(float fxf, float fyf, float bxf, float byf, int dir, bool fu
 // flags themselves are set by the caller, in seed order).
 static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, int dir,
                              SegBuf& seg, size_t window, float average_vote_min,
-                             MarkBuf& marks) {
+                             MarkBuf& marks, uint8_t* visiting) {
   PointWindow phi(window);
   size_t i = 0;
   bool found = true;
@@ -348,6 +348,7 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
           if (stop == 0) {
             processed.insert(packxy(es.pts[p].x, es.pts[p].y));
             p = es.at(sx, sy);
+            if (visiting) __atomic_store_n(&visiting[p], (uint8_t)1, __ATOMIC_RELAXED);
             if (dir > 0) seg.push_back(p);
             else seg.push_front(p);
             averageVote = (averageVote * seg.size() + es.voters_size(p)) / (seg.size() + 1.f);
@@ -375,15 +376,17 @@ static void edge_linking_dir(const EdgeSet& es, VisitedSet& processed, int p, in
   }
 }
 
+// visiting (optional): a byte per point, set for every point the linking
+// walks onto as it goes.
 static void edge_linking(const EdgeSet& es, SegBuf& seg, int pmax, size_t window,
-                         float average_vote_min, MarkBuf& marks) {
+                         float average_vote_min, MarkBuf& marks, uint8_t* visiting) {
   VisitedSet processed;
   if (pmax >= 0) {
     seg.push_back(pmax);
     marks.push_back(pmax);
     processed.insert(packxy(es.pts[pmax].x, es.pts[pmax].y));
-    edge_linking_dir(es, processed, pmax, 1, seg, window, average_vote_min, marks);
-    edge_linking_dir(es, processed, pmax, -1, seg, window, average_vote_min, marks);
+    edge_linking_dir(es, processed, pmax, 1, seg, window, average_vote_min, marks, visiting);
+    edge_linking_dir(es, processed, pmax, -1, seg, window, average_vote_min, marks, visiting);
   }
 }
 
@@ -864,9 +867,10 @@ struct SeedLink {
 enum SeedState : int { kFree = 0, kTaken = 1, kLinked = 2 };
 }  // namespace
 
-static void link_seed(int seed, const EdgeSet& es, const HostParams& p, SeedLink& out) {
+static void link_seed(int seed, const EdgeSet& es, const HostParams& p, SeedLink& out,
+                      uint8_t* visiting) {
   edge_linking(es, out.seg, seed, p.window_size_on_inner_elliptic_segment, p.average_vote_min,
-               out.marks);
+               out.marks, visiting);
 }
 
 static void construct_flow_component(int seed, EdgeSet& es, SeedLink& link,
@@ -1053,8 +1057,9 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
   // flagged by an earlier seed's linking is skipped, so the flow components
   // are committed in seed order by one task while the others link seeds
   // ahead of it (a linking reads only the edge data).  A linker passes over
-  // a seed that the commits so far or any finished linking would flag; the
-  // committer links any seed nobody has taken.
+  // a seed that the commits so far would flag, or that another linking
+  // (finished or under way) has walked onto -- most likely flagged by a
+  // commit to come; the committer links any seed nobody has taken.
   std::vector<std::vector<std::unique_ptr<Candidate>>> loopOne(nl);
   std::vector<size_t> n_seeds(nl, 0);
   for (int l = 0; l < nl; ++l) {
@@ -1084,8 +1089,8 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     std::unique_ptr<std::atomic<int>[]> state(new std::atomic<int>[n]);  // kFree .. kLinked
     for (int k = 0; k < n; ++k) state[k].store(kFree, std::memory_order_relaxed);
     // the commits' processed_in flags as the linkers see them, and the
-    // points some finished linking would flag if its seed is accepted
-    // (plain bytes accessed atomically)
+    // points some linking has walked onto or would flag if its seed is
+    // accepted (a hint only; plain bytes accessed atomically)
     std::vector<std::vector<uint8_t>> flagged(nl), tentative(nl);
     for (int l = 0; l < nl; ++l) {
       flagged[l].assign(levels[l].pts.size(), 0);
@@ -1094,12 +1099,12 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     std::atomic<int> next{0}, committed{0};
     std::atomic<bool> done{false};
     const int tasks = pool ? pool->threads() : 1;
-    const int lookahead = 16 * tasks;
+    const int lookahead = 64 * tasks;
     auto link_one = [&](int k) {
       const int l = order[k].first;
       links[k].reset(new SeedLink());
       try {
-        link_seed(levels[l].seeds[order[k].second], levels[l], p, *links[k]);
+        link_seed(levels[l].seeds[order[k].second], levels[l], p, *links[k], tentative[l].data());
       } catch (...) {
         links[k]->error = std::current_exception();
       }
