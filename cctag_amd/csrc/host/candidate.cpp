@@ -1043,7 +1043,12 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
                        const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
                        HostTimes* times) {
   using Clock = std::chrono::steady_clock;
-  const auto t0 = Clock::now();
+  auto t0 = Clock::now();
+  auto lap = [&](double HostTimes::*field) {
+    const auto t1 = Clock::now();
+    if (times) times->*field += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t0 = t1;
+  };
   const int nl = (int)levels.size();
   // Loop one (Detection.cpp:603-627): seeds by received votes
   // (Multiresolution.cpp:239-242, std::sort: its order among equal votes is
@@ -1074,33 +1079,129 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
                                        const std::pair<int32_t, int32_t>& b) { return a.first > b.first; });
     for (size_t i = 0; i < kv.size(); ++i) es.seeds[i] = kv[i].second;
   });
+  {
+    std::vector<std::pair<int, int>> order;  // (level, seed rank), level by level
+    for (int l = 0; l < nl; ++l)
+      for (size_t i = 0; i < n_seeds[l]; ++i) order.emplace_back(l, (int)i);
+    const int n = (int)order.size();
+    // a linking's result is allocated by the thread that links the seed
+    std::vector<std::unique_ptr<SeedLink>> links(n);
+    std::unique_ptr<std::atomic<int>[]> state(new std::atomic<int>[n]);  // kFree .. kLinked
+    for (int k = 0; k < n; ++k) state[k].store(kFree, std::memory_order_relaxed);
+    // the commits' processed_in flags as the linkers see them, and the
+    // points some linking has walked onto or would flag if its seed is
+    // accepted (a hint only; plain bytes accessed atomically)
+    std::vector<std::vector<uint8_t>> flagged(nl), tentative(nl);
+    for (int l = 0; l < nl; ++l) {
+      flagged[l].assign(levels[l].pts.size(), 0);
+      tentative[l].assign(levels[l].pts.size(), 0);
+    }
+    std::atomic<int> next{0}, committed{0};
+    std::atomic<bool> done{false};
+    const int tasks = pool ? pool->threads() : 1;
+    const int lookahead = 64 * tasks;
+    auto link_one = [&](int k) {
+      const int l = order[k].first;
+      links[k].reset(new SeedLink());
+      try {
+        link_seed(levels[l].seeds[order[k].second], levels[l], p, *links[k], tentative[l].data());
+      } catch (...) {
+        links[k]->error = std::current_exception();
+      }
+      for (const int q : links[k]->marks) __atomic_store_n(&tentative[l][q], (uint8_t)1, __ATOMIC_RELAXED);
+      state[k].store(kLinked, std::memory_order_release);
+    };
+    auto linker = [&] {
+      for (;;) {
+        if (done.load(std::memory_order_acquire)) return;
+        const int k = next.fetch_add(1, std::memory_order_relaxed);
+        if (k >= n) return;
+        while (k - committed.load(std::memory_order_acquire) > lookahead) {
+          if (done.load(std::memory_order_acquire)) return;
+          std::this_thread::yield();
+        }
+        int expect = kFree;
+        if (!state[k].compare_exchange_strong(expect, kTaken)) continue;
+        // a seed an earlier linking covers is most likely skipped: leave it
+        // to the committer, which links it itself if it is not
+        const int l = order[k].first, seed = levels[l].seeds[order[k].second];
+        if (__atomic_load_n(&flagged[l][seed], __ATOMIC_RELAXED) ||
+            __atomic_load_n(&tentative[l][seed], __ATOMIC_RELAXED)) {
+          state[k].store(kFree, std::memory_order_release);
+          continue;
+        }
+        link_one(k);
+      }
+    };
+    auto committer = [&] {
+      struct Done {
+        std::atomic<bool>& d;
+        ~Done() { d.store(true, std::memory_order_release); }
+      } guard{done};
+      for (int k = 0; k < n; ++k) {
+        const int l = order[k].first;
+        EdgeSet& es = levels[l];
+        const int seed = es.seeds[order[k].second];
+        if (!es.test_processed_in(seed)) {
+          // link it here unless a linker holds it; a linker that passes it
+          // over hands it back free
+          for (;;) {
+            int expect = kFree;
+            if (state[k].compare_exchange_strong(expect, kTaken)) {
+              link_one(k);
+              break;
+            }
+            if (expect == kLinked) break;
+            std::this_thread::yield();
+          }
+          std::atomic_thread_fence(std::memory_order_acquire);
+          SeedLink& link = *links[k];
+          if (link.error) std::rethrow_exception(link.error);
+          construct_flow_component(seed, es, link, loopOne[l]);
+          for (const int q : link.marks) __atomic_store_n(&flagged[l][q], (uint8_t)1, __ATOMIC_RELAXED);
+        }
+        committed.store(k + 1, std::memory_order_release);
+      }
+    };
+    parallel_for(pool, tasks, [&](int t) {
+      if (t == 0) committer();
+      else linker();
+    });
+  }
+  lap(&HostTimes::loop_one);
   // Loops two and three and the level-0 re-fit as one flow of tasks
   // (Detection.cpp:629-688, Multiresolution.cpp:319-401): one loop-two task
-  // per (level, candidate), queued when loop one has committed the level's
-  // last seed; a level's segment labels are assigned by the task that ends
-  // its loop two, which then queues the level's loop-three tasks (long ones
-  // first); each marker a loop-three task finds above level 0 is queued for
-  // its re-fit.  Threads take loop-two tasks before the others; linkers that
-  // have no seed left to take join in.  Loop one writes only the seeds'
-  // level (its processed_in flags and flow components); the loop-two tasks
-  // of a level flip their own bits of its points' candidate masks, the
-  // labels write its points' segment numbers, loop three and the re-fit only
-  // read the edge sets, and every task writes only its own candidate or
-  // marker, so the results (and their order) are those of the loops run one
-  // after another.
+  // per (level, candidate); a level's segment labels are assigned by the
+  // task that ends its loop two, which then queues the level's loop-three
+  // tasks; each marker a loop-three task finds above level 0 is queued for
+  // its re-fit.  A thread takes loop-two tasks first, then queued ones.  The
+  // loop-two tasks of a level flip their own bits of its points' candidate
+  // masks, the labels write its points' segment numbers, loop three and the
+  // re-fit only read the edge sets, and every task writes only its own
+  // candidate or marker, so the results (and their order) are those of the
+  // three loops run one after another.
   std::vector<std::vector<LoopTwoTask>> two(nl);
+  std::vector<std::pair<int, int>> jobs2;
+  for (int l = 0; l < nl; ++l) {
+    const size_t n = std::min(loopOne[l].size(), (size_t)p.maximum_nb_candidates_loop_two);
+    two[l].resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      two[l][i].cand = std::move(*loopOne[l][i]);
+      jobs2.emplace_back(l, (int)i);
+    }
+  }
   std::vector<std::vector<Candidate>> loopTwo(nl);
   std::vector<std::vector<std::vector<Marker>>> found(nl);  // [level][candidate]
   struct Queued {
     int l, i, k;  // k < 0: loop three of candidate i of level l; else re-fit of its marker k
   };
   std::mutex qm;
-  std::deque<std::pair<int, int>> queue2;  // loop-two tasks (level, candidate)
-  std::deque<Queued> queue;                // loop-three tasks and re-fits
-  std::exception_ptr error;                // the first exception a task raised (under qm)
-  std::atomic<int> pending{0};             // tasks queued or running
-  std::atomic<bool> all_queued{false};     // every level's loop two is queued
+  std::deque<Queued> queue;
+  std::exception_ptr error;  // the first exception a task raised (under qm)
+  const int n2 = (int)jobs2.size();
+  std::atomic<int> next2{0}, pending{n2};  // pending: tasks queued, running or to hand out
   std::unique_ptr<std::atomic<int>[]> left2(new std::atomic<int>[nl]);
+  for (int l = 0; l < nl; ++l) left2[l].store((int)two[l].size(), std::memory_order_relaxed);
   using Tick = Clock::duration::rep;
   std::atomic<Tick> end2{0}, end3{0}, labels_ticks{0};
   auto stamp = [](std::atomic<Tick>& at) {
@@ -1142,182 +1243,60 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     for (int i = 0; i < n; ++i)
       if (!long_task[i]) queue.push_back(Queued{l, i, -1});
   };
-  // loop one is done with level l: its loop-two tasks
-  auto queue_level = [&](int l) {
-    const size_t n = std::min(loopOne[l].size(), (size_t)p.maximum_nb_candidates_loop_two);
-    two[l].resize(n);
-    for (size_t i = 0; i < n; ++i) two[l][i].cand = std::move(*loopOne[l][i]);
-    left2[l].store((int)n, std::memory_order_relaxed);
-    if (n == 0) {
-      finish_level(l);
-      return;
-    }
-    pending.fetch_add((int)n, std::memory_order_relaxed);
-    std::lock_guard<std::mutex> lk(qm);
-    for (size_t i = 0; i < n; ++i) queue2.emplace_back(l, (int)i);
-  };
-  // one queued task, if any
-  auto flow_step = [&]() -> bool {
-    std::pair<int, int> t2{-1, -1};
-    Queued t{-1, -1, -1};
-    {
-      std::lock_guard<std::mutex> lk(qm);
-      if (!queue2.empty()) {
-        t2 = queue2.front();
-        queue2.pop_front();
-      } else if (!queue.empty()) {
-        t = queue.front();
-        queue.pop_front();
-      }
-    }
-    if (t2.first >= 0) {
-      const int l = t2.first, i = t2.second;
-      guarded([&] { complete_flow_component(two[l][i], levels[l], (size_t)i, p); });
-      if (left2[l].fetch_sub(1, std::memory_order_acq_rel) == 1) finish_level(l);
-      stamp(end2);
-    } else if (t.l >= 0 && t.k < 0) {
-      std::vector<Marker>& out = found[t.l][t.i];
-      guarded([&] {
-        Pcg32 rng(271828u);
-        loop_two_iteration(out, levels[t.l], loopTwo[t.l], (size_t)t.i, t.l,
-                           (float)std::pow(2.0, t.l), p, rng);
-      });
-      int refits = 0;
-      for (Marker& m : out) {
-        if (m.pyramid_level > 0) ++refits;
-        else m.rescaled_outer_ellipse_points = m.points.back();
-      }
-      if (refits > 0) {
-        pending.fetch_add(refits, std::memory_order_relaxed);
-        std::lock_guard<std::mutex> lk(qm);
-        for (int m = (int)out.size() - 1; m >= 0; --m)
-          if (out[m].pyramid_level > 0) queue.push_front(Queued{t.l, t.i, m});
-      }
-      stamp(end3);
-    } else if (t.l >= 0) {
-      guarded([&] { rescale_marker(found[t.l][t.i][t.k], levels[0]); });
-    } else {
-      return false;
-    }
-    pending.fetch_sub(1, std::memory_order_release);
-    return true;
-  };
-  auto flow_worker = [&] {
+  for (int l = 0; l < nl; ++l)
+    if (two[l].empty()) finish_level(l);
+  auto work = [&](int) {
     for (int spins = 0;;) {
-      if (flow_step()) continue;
-      if (all_queued.load(std::memory_order_acquire) && pending.load(std::memory_order_acquire) == 0)
-        return;
+      const int k = next2.fetch_add(1, std::memory_order_relaxed);
+      if (k < n2) {
+        const int l = jobs2[k].first, i = jobs2[k].second;
+        guarded([&] { complete_flow_component(two[l][i], levels[l], (size_t)i, p); });
+        if (left2[l].fetch_sub(1, std::memory_order_acq_rel) == 1) finish_level(l);
+        stamp(end2);
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      Queued t{-1, -1, -1};
+      {
+        std::lock_guard<std::mutex> lk(qm);
+        if (!queue.empty()) {
+          t = queue.front();
+          queue.pop_front();
+        }
+      }
+      if (t.l >= 0 && t.k < 0) {
+        std::vector<Marker>& out = found[t.l][t.i];
+        guarded([&] {
+          Pcg32 rng(271828u);
+          loop_two_iteration(out, levels[t.l], loopTwo[t.l], (size_t)t.i, t.l,
+                             (float)std::pow(2.0, t.l), p, rng);
+        });
+        int refits = 0;
+        for (Marker& m : out) {
+          if (m.pyramid_level > 0) ++refits;
+          else m.rescaled_outer_ellipse_points = m.points.back();
+        }
+        if (refits > 0) {
+          pending.fetch_add(refits, std::memory_order_relaxed);
+          std::lock_guard<std::mutex> lk(qm);
+          for (int m = (int)out.size() - 1; m >= 0; --m)
+            if (out[m].pyramid_level > 0) queue.push_front(Queued{t.l, t.i, m});
+        }
+        stamp(end3);
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      if (t.l >= 0) {
+        guarded([&] { rescale_marker(found[t.l][t.i][t.k], levels[0]); });
+        pending.fetch_sub(1, std::memory_order_release);
+        continue;
+      }
+      if (pending.load(std::memory_order_acquire) == 0) return;
       if (++spins % 64 == 0) std::this_thread::yield();
     }
   };
-  Clock::time_point t_one_end = t0;
-  {
-    std::vector<std::pair<int, int>> order;  // (level, seed rank), level by level
-    for (int l = 0; l < nl; ++l)
-      for (size_t i = 0; i < n_seeds[l]; ++i) order.emplace_back(l, (int)i);
-    const int n = (int)order.size();
-    // a linking's result is allocated by the thread that links the seed
-    std::vector<std::unique_ptr<SeedLink>> links(n);
-    std::unique_ptr<std::atomic<int>[]> state(new std::atomic<int>[n]);  // kFree .. kLinked
-    for (int k = 0; k < n; ++k) state[k].store(kFree, std::memory_order_relaxed);
-    // the commits' processed_in flags as the linkers see them, and the
-    // points some linking has walked onto or would flag if its seed is
-    // accepted (a hint only; plain bytes accessed atomically)
-    std::vector<std::vector<uint8_t>> flagged(nl), tentative(nl);
-    for (int l = 0; l < nl; ++l) {
-      flagged[l].assign(levels[l].pts.size(), 0);
-      tentative[l].assign(levels[l].pts.size(), 0);
-    }
-    std::atomic<int> next{0}, committed{0};
-    std::atomic<bool> done{false};
-    const int tasks = pool ? pool->threads() : 1;
-    const int lookahead = 64 * tasks;
-    auto link_one = [&](int k) {
-      const int l = order[k].first;
-      links[k].reset(new SeedLink());
-      try {
-        link_seed(levels[l].seeds[order[k].second], levels[l], p, *links[k], tentative[l].data());
-      } catch (...) {
-        links[k]->error = std::current_exception();
-      }
-      for (const int q : links[k]->marks) __atomic_store_n(&tentative[l][q], (uint8_t)1, __ATOMIC_RELAXED);
-      state[k].store(kLinked, std::memory_order_release);
-    };
-    auto linker = [&] {
-      for (;;) {
-        if (done.load(std::memory_order_acquire)) break;
-        const int k = next.fetch_add(1, std::memory_order_relaxed);
-        if (k >= n) break;
-        bool over = false;
-        while (k - committed.load(std::memory_order_acquire) > lookahead) {
-          if ((over = done.load(std::memory_order_acquire))) break;
-          std::this_thread::yield();
-        }
-        if (over) break;
-        int expect = kFree;
-        if (!state[k].compare_exchange_strong(expect, kTaken)) continue;
-        // a seed an earlier linking covers is most likely skipped: leave it
-        // to the committer, which links it itself if it is not
-        const int l = order[k].first, seed = levels[l].seeds[order[k].second];
-        if (__atomic_load_n(&flagged[l][seed], __ATOMIC_RELAXED) ||
-            __atomic_load_n(&tentative[l][seed], __ATOMIC_RELAXED)) {
-          state[k].store(kFree, std::memory_order_release);
-          continue;
-        }
-        link_one(k);
-      }
-      flow_worker();
-    };
-    auto committer = [&] {
-      struct Done {
-        std::atomic<bool>& d;
-        std::atomic<bool>& q;
-        ~Done() {
-          d.store(true, std::memory_order_release);
-          q.store(true, std::memory_order_release);
-        }
-      } guard{done, all_queued};
-      int queued = 0;  // levels whose loop two is queued
-      for (int k = 0; k < n; ++k) {
-        const int l = order[k].first;
-        while (queued < l) queue_level(queued++);
-        EdgeSet& es = levels[l];
-        const int seed = es.seeds[order[k].second];
-        if (!es.test_processed_in(seed)) {
-          // link it here unless a linker holds it; a linker that passes it
-          // over hands it back free
-          for (;;) {
-            int expect = kFree;
-            if (state[k].compare_exchange_strong(expect, kTaken)) {
-              link_one(k);
-              break;
-            }
-            if (expect == kLinked) break;
-            std::this_thread::yield();
-          }
-          std::atomic_thread_fence(std::memory_order_acquire);
-          SeedLink& link = *links[k];
-          if (link.error) std::rethrow_exception(link.error);
-          construct_flow_component(seed, es, link, loopOne[l]);
-          for (const int q : link.marks) __atomic_store_n(&flagged[l][q], (uint8_t)1, __ATOMIC_RELAXED);
-        }
-        committed.store(k + 1, std::memory_order_release);
-      }
-      done.store(true, std::memory_order_release);
-      while (queued < nl) queue_level(queued++);
-      t_one_end = Clock::now();
-    };
-    parallel_for(pool, tasks, [&](int t) {
-      if (t == 0) {
-        committer();
-        flow_worker();
-      } else {
-        linker();
-      }
-    });
-  }
-  if (times) times->loop_one += std::chrono::duration<double, std::milli>(t_one_end - t0).count();
-  const Clock::time_point t_flow = t_one_end;
+  const auto t_flow = Clock::now();
+  parallel_for(pool, pool ? pool->threads() : 1, work);
   if (error) std::rethrow_exception(error);
   for (int l = nl - 1; l >= 0; --l)
     for (std::vector<Marker>& f : found[l])
