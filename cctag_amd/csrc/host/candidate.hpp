@@ -156,8 +156,9 @@ struct Marker {
 struct HostTimes {
   double load = 0, loop_one = 0, loop_two = 0, labels = 0, loop_three = 0, rescale = 0,
          ident = 0, finish = 0;
-  // inside `ident`: host preparation and result reading, and the two
-  // backend calls (transfers and kernels included)
+  // inside `ident`: host preparation and result reading, and the backend
+  // calls (transfers and kernels included; selection and refinement in one
+  // call count as refinement)
   double ident_prep = 0, ident_select = 0, ident_refine = 0, ident_post = 0;
 };
 

@@ -120,6 +120,25 @@ bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2
   return true;
 }
 
+void IdentBackend::select_and_refine(const std::vector<CutJob>& cuts,
+                                     std::vector<RefineJob>& jobs, const std::vector<int>& sel_of,
+                                     const HostParams& p, std::vector<CutSelection>& sel,
+                                     std::vector<RefineResult>& out) {
+  select_cuts(cuts, p, sel);
+  std::vector<RefineJob> run;
+  std::vector<size_t> at;
+  for (size_t r = 0; r < jobs.size(); ++r) {
+    jobs[r].stops = sel[sel_of[r]].stops;
+    if (jobs[r].stops.empty()) continue;
+    run.push_back(jobs[r]);
+    at.push_back(r);
+  }
+  std::vector<RefineResult> res;
+  refine(run, p, res);
+  out.assign(jobs.size(), RefineResult());
+  for (size_t k = 0; k < at.size(); ++k) out[at[k]] = std::move(res[k]);
+}
+
 namespace {
 // Working state of one frame between the backend calls.
 struct FrameWork {
@@ -128,7 +147,6 @@ struct FrameWork {
   std::vector<int> detected;
   std::vector<size_t> job_marker;   // select job -> marker
   std::vector<std::vector<Pt2>> stops;
-  std::vector<size_t> rmarker;      // refine job -> marker
 };
 }  // namespace
 
@@ -146,7 +164,7 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
   std::vector<std::pair<int, int>> all;  // (frame, marker)
   for (size_t f = 0; f < frames.size(); ++f) {
     IdentFrame& fr = frames[f];
-    fw.push_back(FrameWork{fr.markers, fr.image, {}, {}, {}, {}});
+    fw.push_back(FrameWork{fr.markers, fr.image, {}, {}, {}});
     const size_t n = fr.markers->size();
     fw.back().detected.assign(n, -1);
     fw.back().stops.assign(n, {});
@@ -174,6 +192,7 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     have[q] = 1;
   });
   std::vector<CutJob> jobs;
+  std::vector<std::pair<int, int>> job_at;  // select job -> (frame, marker)
   for (size_t q = 0; q < all.size(); ++q) {
     FrameWork& F = fw[all[q].first];
     if (!have[q]) {
@@ -182,68 +201,121 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     }
     jobs.push_back(std::move(slot[q]));
     F.job_marker.push_back(all[q].second);
+    job_at.push_back(all[q]);
+  }
+  // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
+  // prepared for every marker with cuts to select, so that the backend can
+  // run both steps at once.  conditionerFromEllipse (conditioner.hpp:22-46)
+  // caches (a+b)/2 of the first ellipse it sees: per frame, that of the
+  // first marker refined -- guessed as the frame's first marker with cuts,
+  // and the frame refined again when its selection turns out empty.
+  const IdentConstants kc = ident_constants(p, bank);
+  std::vector<int> guess(fw.size(), -1);
+  for (size_t f = 0; f < fw.size(); ++f)
+    if (!fw[f].job_marker.empty()) guess[f] = (int)fw[f].job_marker[0];
+  auto mean_of = [&](size_t f, int i) {
+    const Ellipse& e = (*fw[f].markers)[i].rescaled_outer_ellipse;
+    return (e.a() + e.b()) / 2.f;
+  };
+  const std::vector<Pt2> no_stops;
+  std::vector<RefineJob> rslot(jobs.size());
+  std::vector<uint8_t> made(jobs.size(), 0);  // 1: job made, 2: its conditioned ellipse failed
+  parallel_for(pool, (int)jobs.size(), [&](int q) {
+    const int f = job_at[q].first, i = job_at[q].second;
+    const Marker& m = (*fw[f].markers)[i];
+    rslot[q].image = fw[f].image;
+    made[q] = make_refine_job(m.rescaled_outer_ellipse, m.center_img, no_stops, p, kc,
+                              mean_of(f, guess[f]), rslot[q]) ? 1 : 2;
+  });
+  std::vector<RefineJob> rjobs;
+  std::vector<int> sel_of, rjob_of(jobs.size(), -1);
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    if (made[q] != 1) continue;
+    rjob_of[q] = (int)rjobs.size();
+    rjobs.push_back(std::move(rslot[q]));
+    sel_of.push_back((int)q);
   }
   lap(&HostTimes::ident_prep);
   std::vector<CutSelection> sel;
-  backend.select_cuts(jobs, p, sel);
-  lap(&HostTimes::ident_select);
-  size_t j = 0;
-  for (FrameWork& F : fw)
-    for (size_t i : F.job_marker) {
-      CutSelection& c = sel[j++];
-      if (c.n_collected == 0) F.detected[i] = status::no_collected_cuts;
-      else if (c.stops.empty()) F.detected[i] = status::no_selected_cuts;
-      else {
-        F.detected[i] = status::id_reliable;
-        F.stops[i] = std::move(c.stops);
-      }
+  std::vector<RefineResult> res;
+  backend.select_and_refine(jobs, rjobs, sel_of, p, sel, res);
+  lap(&HostTimes::ident_refine);
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    FrameWork& F = fw[job_at[q].first];
+    const int i = job_at[q].second;
+    CutSelection& c = sel[q];
+    if (c.n_collected == 0) F.detected[i] = status::no_collected_cuts;
+    else if (c.stops.empty()) F.detected[i] = status::no_selected_cuts;
+    else {
+      F.detected[i] = status::id_reliable;
+      F.stops[i] = std::move(c.stops);
     }
-  // --- identify_step_2 (Identification.cpp:1318-1480) -----------------------
-  // conditionerFromEllipse (conditioner.hpp:22-46) caches (a+b)/2 of the
-  // first ellipse it sees: per frame, the first marker refined.
-  const IdentConstants kc = ident_constants(p, bank);
-  std::vector<float> mean_ab(fw.size(), 0.f);
-  for (size_t f = 0; f < fw.size(); ++f)
+  }
+  // frames whose first refined marker is not the guessed one: their jobs
+  // again with that marker's (a+b)/2
+  std::vector<int> first(fw.size(), -1);
+  std::vector<uint8_t> redo(fw.size(), 0);
+  for (size_t f = 0; f < fw.size(); ++f) {
     for (size_t i = 0; i < fw[f].markers->size(); ++i)
       if (fw[f].detected[i] == status::id_reliable) {
-        const Ellipse& e = (*fw[f].markers)[i].rescaled_outer_ellipse;
-        mean_ab[f] = (e.a() + e.b()) / 2.f;
+        first[f] = (int)i;
         break;
       }
-  std::vector<RefineJob> rslot(all.size());
-  std::fill(have.begin(), have.end(), 0);
-  parallel_for(pool, (int)all.size(), [&](int q) {
-    const int f = all[q].first, i = all[q].second;
-    FrameWork& F = fw[f];
-    if (F.detected[i] != status::id_reliable) return;
-    const Marker& m = (*F.markers)[i];
-    rslot[q].image = F.image;
-    have[q] = make_refine_job(m.rescaled_outer_ellipse, m.center_img, F.stops[i], p, kc,
-                              mean_ab[f], rslot[q]) ? 1 : 2;
-  });
-  std::vector<RefineJob> rjobs;
-  for (size_t q = 0; q < all.size(); ++q) {
-    FrameWork& F = fw[all[q].first];
-    if (have[q] == 2) F.detected[all[q].second] = status::opti_has_diverged;
-    if (have[q] != 1) continue;
-    rjobs.push_back(std::move(rslot[q]));
-    F.rmarker.push_back(all[q].second);
+    redo[f] = first[f] >= 0 && first[f] != guess[f];
   }
-  lap(&HostTimes::ident_prep);
-  std::vector<RefineResult> res;
-  backend.refine(rjobs, p, res);
+  std::vector<RefineJob> rjobs2;
+  std::vector<RefineResult> res2;
+  std::vector<int> rjob2_of(jobs.size(), -1);
+  {
+    std::vector<int> qs;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const int f = job_at[q].first;
+      if (redo[f] && fw[f].detected[job_at[q].second] == status::id_reliable) qs.push_back((int)q);
+    }
+    if (!qs.empty()) {
+      std::vector<RefineJob> slot2(qs.size());
+      parallel_for(pool, (int)qs.size(), [&](int k) {
+        const int q = qs[k], f = job_at[q].first, i = job_at[q].second;
+        const Marker& m = (*fw[f].markers)[i];
+        slot2[k].image = fw[f].image;
+        made[q] = make_refine_job(m.rescaled_outer_ellipse, m.center_img, fw[f].stops[i], p, kc,
+                                  mean_of(f, first[f]), slot2[k]) ? 1 : 2;
+      });
+      for (size_t k = 0; k < qs.size(); ++k)
+        if (made[qs[k]] == 1) {
+          rjob2_of[qs[k]] = (int)rjobs2.size();
+          rjobs2.push_back(std::move(slot2[k]));
+        }
+      backend.refine(rjobs2, p, res2);
+    }
+  }
   lap(&HostTimes::ident_refine);
+  // the refined markers in frame and marker order, with their job and result
+  struct Refined {
+    int f;
+    size_t i;
+    const RefineJob* job;
+    const RefineResult* res;
+  };
+  std::vector<Refined> rj_at;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    const int f = job_at[q].first, i = job_at[q].second;
+    if (fw[f].detected[i] != status::id_reliable) continue;
+    if (made[q] == 2) {
+      fw[f].detected[i] = status::opti_has_diverged;
+      continue;
+    }
+    if (redo[f]) rj_at.push_back(Refined{f, (size_t)i, &rjobs2[rjob2_of[q]], &res2[rjob2_of[q]]});
+    else rj_at.push_back(Refined{f, (size_t)i, &rjobs[rjob_of[q]], &res[rjob_of[q]]});
+  }
   const int nb = (int)bank.size();
-  std::vector<std::pair<int, size_t>> rj_at;  // refine job -> (frame, marker)
-  for (size_t f = 0; f < fw.size(); ++f)
-    for (size_t i : fw[f].rmarker) rj_at.emplace_back((int)f, i);
   parallel_for(pool, (int)rj_at.size(), [&](int jj) {
-    FrameWork& F = fw[rj_at[jj].first];
-    const size_t i = rj_at[jj].second;
+    FrameWork& F = fw[rj_at[jj].f];
+    const size_t i = rj_at[jj].i;
     std::vector<Marker>& markers = *F.markers;
     std::vector<int>& detected = F.detected;
-    const RefineJob& rj = rjobs[jj];
-    const RefineResult& r = res[jj];
+    const RefineJob& rj = *rj_at[jj].job;
+    const RefineResult& r = *rj_at[jj].res;
     Marker& m = markers[i];
     if (!r.ok) {
       m.quality = 1.f / FLT_MAX;

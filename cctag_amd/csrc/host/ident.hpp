@@ -89,6 +89,14 @@ class IdentBackend {
   // refineConicFamilyGlob's search + final signals + orazio distances.
   virtual void refine(const std::vector<RefineJob>& jobs, const HostParams& p,
                       std::vector<RefineResult>& out) = 0;
+  // Both steps at once: select_cuts over `cuts`, then refine over `jobs`,
+  // job r taking the stops selected for cut job sel_of[r] (written into its
+  // `stops`); a job whose selection is empty is not refined and its result
+  // is not ok.  The results are those of the two calls; this default makes
+  // them one after the other.
+  virtual void select_and_refine(const std::vector<CutJob>& cuts, std::vector<RefineJob>& jobs,
+                                 const std::vector<int>& sel_of, const HostParams& p,
+                                 std::vector<CutSelection>& sel, std::vector<RefineResult>& out);
 };
 
 // Bank rows (CCTagMarkersBank.cpp:22-52), innermost ring first.
@@ -119,9 +127,10 @@ void identify_markers(std::vector<Marker>& markers, const HostParams& p,
                       const std::vector<std::vector<float>>& bank, IdentBackend& backend,
                       TaskPool* pool = nullptr, HostTimes* times = nullptr);
 
-// The same for several frames at once: one select_cuts and one refine call
-// of the backend for all their markers; each frame's markers come out as
-// identify_markers would leave them (the conditioner cache is per frame).
+// The same for several frames at once: one select_and_refine call of the
+// backend for all their markers (and one refine call for the frames whose
+// first marker with cuts gets none selected); each frame's markers come out
+// as identify_markers would leave them (the conditioner cache is per frame).
 struct IdentFrame {
   std::vector<Marker>* markers = nullptr;
   int image = 0;                     // backend image index of the frame

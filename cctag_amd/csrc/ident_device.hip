@@ -71,14 +71,12 @@ const DeviceImage& GpuIdentBackend::image(int i) const {
   return images_[i];
 }
 
-void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
-                                  std::vector<CutSelection>& out) {
-  out.assign(jobs.size(), CutSelection());
+// The selection jobs and their outer points into mapped host memory (the
+// caller has called ensure for them).
+void GpuIdentBackend::pack_select(const std::vector<CutJob>& jobs) {
   const int n = (int)jobs.size();
-  if (n == 0) return;
   int npts = 0;
   for (const CutJob& j : jobs) npts += (int)j.outer_points.size();
-  ensure(n, npts);
   select_jobs_ = n;
   select_samples_ = (long long)npts * (k_.n_samples + k_.n_refine_samples);
   SelectJobDev* hj = h_sel_jobs_.as<SelectJobDev>();
@@ -93,19 +91,14 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
                          (int)j.outer_points.size(), off};
     for (const Pt2& q : j.outer_points) hp[off++] = make_float4(q.x, q.y, q.dx, q.dy);
   }
-  hipStream_t s = stream_;
-  hipEventRecord(ev_[0], s);
-  if (launch_select_cuts(h_sel_jobs_.dev<SelectJobDev>(), n, h_pts_.dev<float4>(), k_.start_sig,
-                         k_.n_samples, k_.n_refine_samples, p.num_cuts_in_ident_step,
-                         h_sel_out_.dev<SelectOutDev>(), s) != 0)
-    throw std::runtime_error("k_select_cuts launch failed");
-  hipEventRecord(ev_[1], s);
+}
+
+void GpuIdentBackend::unpack_select(int n, std::vector<CutSelection>& out) const {
   const SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
-  hip_check(stream_wait(s), "sync select");
-  hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
-  select_us_ *= 1000.f;
+  out.assign(n, CutSelection());
   for (int i = 0; i < n; ++i) {
     out[i].n_collected = ho[i].n_collected;
+    out[i].stops.reserve(ho[i].n_selected);
     for (int k = 0; k < ho[i].n_selected; ++k) {
       const float4 v = ho[i].stops[k];
       out[i].stops.emplace_back(v.x, v.y, v.z, v.w);
@@ -113,9 +106,34 @@ void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostPar
   }
 }
 
+void GpuIdentBackend::select_cuts(const std::vector<CutJob>& jobs, const HostParams& p,
+                                  std::vector<CutSelection>& out) {
+  const int n = (int)jobs.size();
+  if (n == 0) {
+    out.clear();
+    return;
+  }
+  int npts = 0;
+  for (const CutJob& j : jobs) npts += (int)j.outer_points.size();
+  ensure(n, npts);
+  pack_select(jobs);
+  hipStream_t s = stream_;
+  hipEventRecord(ev_[0], s);
+  if (launch_select_cuts(h_sel_jobs_.dev<SelectJobDev>(), n, h_pts_.dev<float4>(), k_.start_sig,
+                         k_.n_samples, k_.n_refine_samples, p.num_cuts_in_ident_step,
+                         h_sel_out_.dev<SelectOutDev>(), s) != 0)
+    throw std::runtime_error("k_select_cuts launch failed");
+  hipEventRecord(ev_[1], s);
+  hip_check(stream_wait(s), "sync select");
+  hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
+  select_us_ *= 1000.f;
+  unpack_select(n, out);
+}
+
 // Fills the pinned job array for `jobs` (distances packed job after job);
 // returns the number of distance floats.
-int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs) {
+int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs,
+                                      const std::vector<int>* sel_of) {
   // the pinned array may still be the source of a queued copy
   if (jobs_in_flight_) hip_check(hipEventSynchronize(jobs_copied_), "job copy");
   jobs_in_flight_ = false;
@@ -141,12 +159,20 @@ int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs) {
     d.begin_sig = r.begin_sig;
     d.end_sig = r.end_sig;
     d.n_iter = r.n_iter;
-    d.n_cuts = (int)r.stops.size();
     d.c0x = r.center0.x;
     d.c0y = r.center0.y;
-    d.dist_offset = n_dist;
-    n_dist += d.n_cuts * nb;
-    for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
+    if (sel_of) {
+      d.sel_index = (*sel_of)[i];
+      d.n_cuts = 0;
+      d.dist_offset = (int)i * kMaxCuts * nb;
+      n_dist = ((int)i + 1) * kMaxCuts * nb;
+    } else {
+      d.sel_index = -1;
+      d.n_cuts = (int)r.stops.size();
+      d.dist_offset = n_dist;
+      n_dist += d.n_cuts * nb;
+      for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
+    }
   }
   return n_dist;
 }
@@ -218,23 +244,51 @@ void GpuIdentBackend::refine_gradient(const std::vector<RefineJob>& jobs, const 
     throw std::runtime_error("gradient refinement launch failed");
 }
 
+void GpuIdentBackend::unpack_refine(const std::vector<RefineJob>& jobs,
+                                    std::vector<RefineResult>& out) const {
+  const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
+  const float* hd = h_ref_dist_.as<float>();
+  const RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
+  const int nb = k_.bank_rows;
+  out.assign(jobs.size(), RefineResult());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    const RefineOutDev& o = ho[i];
+    RefineResult& r = out[i];
+    const int nc = (int)jobs[i].stops.size();
+    r.ok = o.ok != 0 && nc > 0;
+    if (!r.ok) continue;
+    r.center = Pt2(o.cx, o.cy);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) r.H.m[a][b] = o.H[a * 3 + b];
+    r.min_res = o.min_res;
+    r.bar_min = o.bar_min;
+    r.bar_max = o.bar_max;
+    r.oob.assign(o.oob, o.oob + nc);
+    r.id_dist.assign((size_t)nc * nb, 0.f);
+    const float* src = hd + hj[i].dist_offset;
+    for (int k = 0; k < nc; ++k)
+      if (!r.oob[k]) std::copy(src + (size_t)k * nb, src + (size_t)(k + 1) * nb, &r.id_dist[(size_t)k * nb]);
+  }
+}
+
 void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParams& p,
                              std::vector<RefineResult>& out) {
-  out.assign(jobs.size(), RefineResult());
   const int n = (int)jobs.size();
-  if (n == 0) return;
+  if (n == 0) {
+    out.clear();
+    return;
+  }
   ensure(n, 0);
   refine_jobs_ = n;
   refine_samples_ = 0;
   const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
   for (const RefineJob& r : jobs)
     refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
-  RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
-  const int nb = k_.bank_rows;
   pack_refine_jobs(jobs);
   hipStream_t s = stream_;
-  hip_check(hipMemcpyAsync(d_ref_jobs_, hj, sizeof(RefineJobDev) * n, hipMemcpyHostToDevice, s), "H2D");
-
+  hip_check(hipMemcpyAsync(d_ref_jobs_, h_ref_jobs_.as<RefineJobDev>(), sizeof(RefineJobDev) * n,
+                           hipMemcpyHostToDevice, s),
+            "H2D");
   hip_check(hipEventRecord(jobs_copied_, s), "event");
   jobs_in_flight_ = true;
   hipEventRecord(ev_[2], s);
@@ -249,29 +303,66 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
       throw std::runtime_error("k_refine launch failed");
   }
   hipEventRecord(ev_[3], s);
-  const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
-  const float* hd = h_ref_dist_.as<float>();
   hip_check(stream_wait(s), "sync refine");
   hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
   refine_us_ *= 1000.f;
-  for (int i = 0; i < n; ++i) {
-    const RefineOutDev& o = ho[i];
-    RefineResult& r = out[i];
-    r.ok = o.ok != 0;
-    if (!r.ok) continue;
-    r.center = Pt2(o.cx, o.cy);
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) r.H.m[a][b] = o.H[a * 3 + b];
-    r.min_res = o.min_res;
-    r.bar_min = o.bar_min;
-    r.bar_max = o.bar_max;
-    const int nc = (int)jobs[i].stops.size();
-    r.oob.assign(o.oob, o.oob + nc);
-    r.id_dist.assign((size_t)nc * nb, 0.f);
-    const float* src = hd + hj[i].dist_offset;
-    for (int k = 0; k < nc; ++k)
-      if (!r.oob[k]) std::copy(src + (size_t)k * nb, src + (size_t)(k + 1) * nb, &r.id_dist[(size_t)k * nb]);
+  unpack_refine(jobs, out);
+}
+
+void GpuIdentBackend::select_and_refine(const std::vector<CutJob>& cuts,
+                                        std::vector<RefineJob>& jobs, const std::vector<int>& sel_of,
+                                        const HostParams& p, std::vector<CutSelection>& sel,
+                                        std::vector<RefineResult>& out) {
+  const int nc = (int)cuts.size(), n = (int)jobs.size();
+  if (grad_steps_ > 0 || nc == 0 || n == 0) {
+    IdentBackend::select_and_refine(cuts, jobs, sel_of, p, sel, out);
+    return;
   }
+  if ((int)sel_of.size() != n) throw std::runtime_error("select_and_refine: one selection per job");
+  for (const int q : sel_of)
+    if (q < 0 || q >= nc) throw std::runtime_error("select_and_refine: selection index out of range");
+  int npts = 0;
+  for (const CutJob& j : cuts) npts += (int)j.outer_points.size();
+  ensure(std::max(nc, n), npts);
+  pack_select(cuts);
+  pack_refine_jobs(jobs, &sel_of);
+  hipStream_t s = stream_;
+  hip_check(hipMemcpyAsync(d_ref_jobs_, h_ref_jobs_.as<RefineJobDev>(), sizeof(RefineJobDev) * n,
+                           hipMemcpyHostToDevice, s),
+            "H2D");
+  hip_check(hipEventRecord(jobs_copied_, s), "event");
+  jobs_in_flight_ = true;
+  hipEventRecord(ev_[0], s);
+  if (launch_select_cuts(h_sel_jobs_.dev<SelectJobDev>(), nc, h_pts_.dev<float4>(), k_.start_sig,
+                         k_.n_samples, k_.n_refine_samples, p.num_cuts_in_ident_step,
+                         h_sel_out_.dev<SelectOutDev>(), s) != 0)
+    throw std::runtime_error("k_select_cuts launch failed");
+  hipEventRecord(ev_[1], s);
+  if (launch_take_stops(d_ref_jobs_, n, h_sel_out_.dev<SelectOutDev>(), s) != 0)
+    throw std::runtime_error("k_take_stops launch failed");
+  hipEventRecord(ev_[2], s);
+  int max_iter = 0;
+  for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);
+  if (launch_refine(d_ref_jobs_, n, max_iter, k_.n_samples, p.imaged_center_n_grid_sample,
+                    d_inv_ratio_, k_.bank_rows, k_.bank_cols, d_work_,
+                    h_ref_out_.dev<RefineOutDev>(), h_ref_dist_.dev<float>(), s) != 0)
+    throw std::runtime_error("k_refine launch failed");
+  hipEventRecord(ev_[3], s);
+  hip_check(stream_wait(s), "sync select and refine");
+  hipEventElapsedTime(&select_us_, ev_[0], ev_[1]);
+  select_us_ *= 1000.f;
+  hipEventElapsedTime(&refine_us_, ev_[2], ev_[3]);
+  refine_us_ *= 1000.f;
+  unpack_select(nc, sel);
+  refine_jobs_ = n;
+  refine_samples_ = 0;
+  const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
+  for (int r = 0; r < n; ++r) {
+    jobs[r].stops = sel[sel_of[r]].stops;
+    if (!jobs[r].stops.empty())
+      refine_samples_ += (long long)jobs[r].n_iter * g2 * (long long)jobs[r].stops.size() * k_.n_samples;
+  }
+  unpack_refine(jobs, out);
 }
 
 }  // namespace cctagx

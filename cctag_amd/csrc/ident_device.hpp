@@ -39,6 +39,7 @@ struct RefineJobDev {
   int n_iter, n_cuts;
   float c0x, c0y;
   int dist_offset;  // first float of its distances in the packed output
+  int sel_index;    // >= 0: the selection whose stops the job takes on the device
   float2 stops[kMaxCuts];
 };
 
@@ -53,6 +54,9 @@ struct RefineOutDev {
 
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
                        int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s);
+// Jobs with sel_index >= 0 take the stops of that selection (n_cuts = the
+// selected count; no iteration when it is zero).
+int launch_take_stops(RefineJobDev* jobs, int n_jobs, const SelectOutDev* sel, hipStream_t s);
 // Grid-search state of one marker between iteration launches.
 static_assert(kMaxCuts <= 32, "one bit per cut in the flag words");
 struct RefineStateDev {
@@ -113,6 +117,12 @@ class GpuIdentBackend : public IdentBackend {
                    std::vector<CutSelection>& out) override;
   void refine(const std::vector<RefineJob>& jobs, const HostParams& p,
               std::vector<RefineResult>& out) override;
+  // Both steps in one submission (the refinement jobs take their stops from
+  // the selection on the device) and one wait; the gradient refinement
+  // takes the two calls.
+  void select_and_refine(const std::vector<CutJob>& cuts, std::vector<RefineJob>& jobs,
+                         const std::vector<int>& sel_of, const HostParams& p,
+                         std::vector<CutSelection>& sel, std::vector<RefineResult>& out) override;
   // 0: the reference grid search (the default); steps > 0: its first
   // iteration, then `steps` descent steps on the differentiable score from
   // the centre it picked, and the read-out there.
@@ -137,7 +147,12 @@ class GpuIdentBackend : public IdentBackend {
 
  private:
   void ensure(int n_markers, int n_points);
-  int pack_refine_jobs(const std::vector<RefineJob>& jobs);
+  // sel_of: null, or per job the selection it takes its stops from on the
+  // device (its distances then at a fixed stride of kMaxCuts rows)
+  int pack_refine_jobs(const std::vector<RefineJob>& jobs, const std::vector<int>* sel_of = nullptr);
+  void pack_select(const std::vector<CutJob>& jobs);
+  void unpack_select(int n, std::vector<CutSelection>& out) const;
+  void unpack_refine(const std::vector<RefineJob>& jobs, std::vector<RefineResult>& out) const;
   hipStream_t stream_ = nullptr;
   std::vector<DeviceImage> images_;
   const DeviceImage& image(int i) const;

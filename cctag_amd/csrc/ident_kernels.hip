@@ -966,6 +966,23 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
   if (tid < nc) O.oob[tid] = (uint8_t)((sticky >> tid) & 1u);
 }
 
+// A refinement job takes the stops of the selection it names (one block of
+// 64 threads per job): its cut count is the selected count, and a job whose
+// selection is empty gets no iteration.
+__global__ __launch_bounds__(64) void k_take_stops(RefineJobDev* __restrict__ jobs,
+                                                   const SelectOutDev* __restrict__ sel) {
+  RefineJobDev& J = jobs[blockIdx.x];
+  const int s = J.sel_index;
+  if (s < 0) return;
+  const SelectOutDev& o = sel[s];
+  const int ns = o.n_collected == 0 ? 0 : min(o.n_selected, kMaxCuts);
+  if (threadIdx.x < ns) J.stops[threadIdx.x] = make_float2(o.stops[threadIdx.x].x, o.stops[threadIdx.x].y);
+  if (threadIdx.x == 0) {
+    J.n_cuts = ns;
+    if (ns == 0) J.n_iter = 0;
+  }
+}
+
 // The centre the grid search moves to after its first iteration (the job's
 // start when that iteration found no readable grid point).
 __global__ __launch_bounds__(256) void k_refine_coarse_out(const RefineJobDev* __restrict__ jobs,
@@ -1000,6 +1017,12 @@ int launch_refine_readout(const RefineJobDev* jobs, int n_jobs, int n_samples, c
   if (n_jobs <= 0) return 0;
   k_refine_final<<<n_jobs, kFinalBlock, 0, s>>>(jobs, n_samples, kMaxGrid, inv_ratio, bank_rows,
                                                  bank_cols, work, H, centers, cost, out, dist);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_take_stops(RefineJobDev* jobs, int n_jobs, const SelectOutDev* sel, hipStream_t s) {
+  if (n_jobs <= 0) return 0;
+  k_take_stops<<<n_jobs, 64, 0, s>>>(jobs, sel);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -185,7 +185,9 @@ typedef struct cctagx_detect_stats {
   float load_us, loop_one_us, loop_two_us, labels_us, loop_three_us, rescale_us, ident_us,
       finish_us;
   /* inside ident_us: host preparation and result reading, and the two
-   * identification steps (transfers and kernels) */
+   * identification steps (transfers and kernels).  The cut selection and
+   * the refinement are one backend call, counted in ident_refine_us;
+   * ident_select_us is zero */
   float ident_prep_us, ident_select_us, ident_refine_us, ident_post_us;
 } cctagx_detect_stats;
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);

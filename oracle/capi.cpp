@@ -158,6 +158,43 @@ int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w
   }
 }
 
+// The identification of given markers on an 8-bit frame (identify_markers
+// with this backend, serially): marker k has the outer ellipse ell[5k ..
+// 5k+4] (cx, cy, a, b, angle), npts[k] outer points pts[4j .. 4j+3]
+// (x, y, gx, gy; the markers' points one after the other) and the starting
+// centre c0[2k], c0[2k+1].  Writes each marker's status, ID and imaged
+// centre.  Returns 0, or -1 on an exception.
+int oracle_identify_given(const cctagx_params* params, const uint8_t* img, int w, int h, int n,
+                          const float* ell, const int* npts, const float* pts, const float* c0,
+                          int* status, int* id, float* center) {
+  try {
+    const auto bank = cctagx::marker_bank(params->n_crowns);
+    oracle::CpuIdentBackend backend(img, w, h, bank);
+    std::vector<cctagx::Marker> markers(n);
+    size_t at = 0;
+    for (int k = 0; k < n; ++k) {
+      cctagx::Marker& m = markers[k];
+      m.rescaled_outer_ellipse =
+          cctagx::Ellipse(ell[5 * k], ell[5 * k + 1], ell[5 * k + 2], ell[5 * k + 3], ell[5 * k + 4]);
+      m.outer_ellipse = m.rescaled_outer_ellipse;
+      for (int j = 0; j < npts[k]; ++j, ++at)
+        m.rescaled_outer_ellipse_points.emplace_back(pts[4 * at], pts[4 * at + 1], pts[4 * at + 2],
+                                                     pts[4 * at + 3]);
+      m.center_img = cctagx::Pt2(c0[2 * k], c0[2 * k + 1]);
+    }
+    cctagx::identify_markers(markers, cctagx::host_params(*params), bank, backend, nullptr);
+    for (int k = 0; k < n; ++k) {
+      status[k] = markers[k].status;
+      id[k] = markers[k].id;
+      center[2 * k] = markers[k].center_img.x;
+      center[2 * k + 1] = markers[k].center_img.y;
+    }
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,
                   cctagx_marker* out, int max_out) {
   return oracle_detect_threads(params, img, w, h, out, max_out, 1);

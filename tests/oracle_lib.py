@@ -49,6 +49,8 @@ def load():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
     o.oracle_linking_mismatches.argtypes = [vp, ctypes.c_int]
+    o.oracle_identify_given.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
     _o = o
     return o
 
@@ -171,3 +173,28 @@ def linking_mismatches(cases: np.ndarray) -> int:
     expressions."""
     c = np.ascontiguousarray(cases, dtype=np.float32)
     return load().oracle_linking_mismatches(c.ctypes.data, c.shape[0])
+
+
+def identify_given(img: np.ndarray, markers, params=None):
+    """The oracle's identification of given markers: each a dict with
+    "ellipse" (cx, cy, a, b, angle), "points" (n x 4: x, y, gx, gy) and
+    "center0".  Returns (status, id, centres n x 2)."""
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    if params is None:
+        params = _lib.default_params()
+    n = len(markers)
+    ell = np.ascontiguousarray([m["ellipse"] for m in markers], dtype=np.float32)
+    npts = np.ascontiguousarray([len(m["points"]) for m in markers], dtype=np.int32)
+    pts = np.ascontiguousarray(np.concatenate([np.asarray(m["points"], np.float32) for m in markers]))
+    c0 = np.ascontiguousarray([m["center0"] for m in markers], dtype=np.float32)
+    status = np.zeros(n, np.int32)
+    ids = np.zeros(n, np.int32)
+    centers = np.zeros((n, 2), np.float32)
+    rc = o.oracle_identify_given(ctypes.byref(params), img.ctypes.data, w, h, n, ell.ctypes.data,
+                                 npts.ctypes.data, pts.ctypes.data, c0.ctypes.data,
+                                 status.ctypes.data, ids.ctypes.data, centers.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle identification failed")
+    return status, ids, centers
