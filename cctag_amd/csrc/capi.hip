@@ -64,6 +64,10 @@ struct cctagx_handle {
   HostTimes host_times;  // last detection: host stages
   FrameWorkspace workspace;  // host edge sets reused across detections
   bool have_frame = false;
+  // the front end's launch sequence, captured at the first detection without
+  // per-kernel timing and replayed (the frame size and buffers are fixed)
+  hipGraph_t fe_graph = nullptr;
+  hipGraphExec_t fe_exec = nullptr;
 };
 
 // Copies what the candidate stage reads of every level into pinned memory.
@@ -175,7 +179,26 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   if (!h->have_frame) return CCTAGX_ERR_STATE;
   *n_out = 0;
   h->timer.reset();
-  if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
+  if (h->timer.per_kernel) {
+    if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
+  } else {
+    if (!h->fe_exec) {
+      CCTAGX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = h->fe.run(h->stream, nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (rc != 0 || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        set_error(std::string("front-end graph capture failed: ") + hipGetErrorString(e));
+        return CCTAGX_ERR_HIP;
+      }
+      h->fe_graph = g;
+      CCTAGX_HIP(hipGraphInstantiate(&h->fe_exec, h->fe_graph, nullptr, nullptr, 0));
+    }
+    h->timer.mark("start", h->stream);
+    CCTAGX_HIP(hipGraphLaunch(h->fe_exec, h->stream));
+    h->timer.mark("front_end", h->stream);
+  }
   h->have_front_end = true;
   h->level_loaded.assign(h->level_loaded.size(), false);
   const auto ts = std::chrono::steady_clock::now();
@@ -268,10 +291,10 @@ int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
 
 int cctagx_destroy(cctagx_handle* h) {
   if (!h) return CCTAGX_ERR_ARG;
-  if (h->stream) {
-    hipStreamSynchronize(h->stream);
-    hipStreamDestroy(h->stream);
-  }
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->fe_exec) hipGraphExecDestroy(h->fe_exec);
+  if (h->fe_graph) hipGraphDestroy(h->fe_graph);
+  if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return CCTAGX_OK;
 }
