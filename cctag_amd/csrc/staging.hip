@@ -93,18 +93,18 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
   }
   const uint64_t total = std::max<uint64_t>(at, 16);
   last_bytes_ = at;
+  // the gather writes straight into the mapped host buffer: no device
+  // staging buffer and no separate copy
   if (h_segs_.ensure(sizeof(CopySeg) * std::max<size_t>(segs.size(), 1)) ||
       d_segs_.ensure(sizeof(CopySeg) * std::max<size_t>(segs.size(), 1)) ||
-      d_packed_.ensure(total + total / 4) || h_packed_.ensure(total + total / 4))
+      h_packed_.ensure(total + total / 4, hipHostMallocCoherent))
     return -2;
   if (!segs.empty()) {
     std::memcpy(h_segs_.as<CopySeg>(), segs.data(), sizeof(CopySeg) * segs.size());
     CCTAGX_HIP(hipMemcpyAsync(d_segs_.as<CopySeg>(), h_segs_.as<CopySeg>(),
                               sizeof(CopySeg) * segs.size(), hipMemcpyHostToDevice, s));
-    if (launch_pack(d_segs_.as<CopySeg>(), (int)segs.size(), d_packed_.as<uint8_t>(), s))
+    if (launch_pack(d_segs_.as<CopySeg>(), (int)segs.size(), h_packed_.dev<uint8_t>(), s))
       return -2;
-    CCTAGX_HIP(hipMemcpyAsync(h_packed_.as<uint8_t>(), d_packed_.as<uint8_t>(), at,
-                              hipMemcpyDeviceToHost, s));
   }
   CCTAGX_HIP(stream_wait(s));
 

@@ -1,8 +1,7 @@
 // Device-to-host staging of what the host candidate stage reads of every
 // plane: the counterpart of TagPipe::convertToHost (cuda/frame_export.cu:
 // 23-165).  One counts read-back, then one gather kernel packs every
-// plane's arrays (exact sizes) into a single device buffer and one copy
-// brings it to pinned host memory.
+// plane's arrays (exact sizes) straight into one mapped pinned host buffer.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,7 +36,7 @@ class LevelStager {
 
  private:
   PinnedBuffer h_counts_, h_segs_, h_packed_;
-  DeviceBuffer d_segs_, d_packed_;
+  DeviceBuffer d_segs_;
   std::vector<PlaneCounts> counts_;
   size_t last_bytes_ = 0;
 };
