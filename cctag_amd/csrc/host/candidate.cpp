@@ -1347,7 +1347,8 @@ static bool intersect_line_two_ellipses(int y, const Ellipse& qIn, const Ellipse
   return true;
 }
 
-// The re-fit of one marker of a level above 0 (own rand_5_k stream).
+// Multiresolution.cpp:319-401: the level-0 re-fit of one marker found above
+// level 0 (own rand_5_k stream seeded 271828).
 static void rescale_marker(Marker& m, const EdgeSet& level0) {
   Pcg32 rng(271828u);
   const float scale = m.scale;
@@ -1376,16 +1377,6 @@ static void rescale_marker(Marker& m, const EdgeSet& level0) {
     m.rescaled_outer_ellipse_points = pts;
   } catch (...) {
   }
-}
-
-void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
-                     TaskPool* pool) {
-  (void)p;
-  parallel_for(pool, (int)markers.size(), [&](int im) {
-    Marker& m = markers[im];
-    if (m.pyramid_level > 0) rescale_marker(m, level0);
-    else m.rescaled_outer_ellipse_points = m.points.back();
-  });
 }
 
 std::vector<int> update_markers(const std::vector<Marker>& markers,

@@ -169,12 +169,6 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
                        const HostParams& p, TaskPool* pool, std::vector<Marker>& markers,
                        HostTimes* times = nullptr);
 
-// Multiresolution.cpp:319-401: re-fit markers of levels > 0 on level-0 edges,
-// one task per marker (own rand_5_k stream seeded 271828); what
-// detect_candidates does for the markers it finds.
-void rescale_markers(std::vector<Marker>& markers, const EdgeSet& level0, const HostParams& p,
-                     TaskPool* pool);
-
 // The edge linking's decisions on gradient angles (Vote.cpp:290-340), as
 // functions of the gradients: the first neighbour direction to try,
 // round(((angle + pi/4) / 2pi) * 8) - 1 modulo 8, and the convexity test
