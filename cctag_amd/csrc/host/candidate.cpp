@@ -1233,8 +1233,12 @@ void detect_candidates(std::vector<EdgeSet>& levels, const std::vector<int>& lev
     if (p.search_for_another_segment)
       for (int i = 0; i < n; ++i) {
         const Candidate& c = loopTwo[l][i];
-        const float quality = (float)c.outer_points.size() / (float)rasterize_ellipse_perimeter(c.outer);
-        long_task[i] = quality > 0.25 && quality < 0.7;
+        try {  // an order hint only: a failure leaves the task at the back
+          const float quality =
+              (float)c.outer_points.size() / (float)rasterize_ellipse_perimeter(c.outer);
+          long_task[i] = quality > 0.25 && quality < 0.7;
+        } catch (...) {
+        }
       }
     pending.fetch_add(n, std::memory_order_relaxed);
     std::lock_guard<std::mutex> lk(qm);
