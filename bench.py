@@ -302,7 +302,7 @@ def main():
                     help="frames of the uneven-size batch (0 skips the batched comparison)")
     ap.add_argument("--refine-markers", type=int, default=4096,
                     help="markers of the refinement comparison (0 skips it)")
-    ap.add_argument("--gradient-frames", type=int, default=16,
+    ap.add_argument("--gradient-frames", type=int, default=32,
                     help="1920x1080 frames of the plain / batched / batched+gradient "
                          "comparison (0 skips it)")
     ap.add_argument("--gradient-steps", type=int, default=30,
