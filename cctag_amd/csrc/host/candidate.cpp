@@ -496,27 +496,28 @@ static void outlier_removal(const EdgeSet& es, const std::vector<int>& children,
     Q.m[1][1] = t[2];
     Q.m[1][2] = Q.m[2][1] = t[4];
     Q.m[2][2] = 1.f;
-    try {
-      Ellipse q(Q);
-      const float ratio = q.a() / q.b();
-      if ((ratio < 0.04f) || (ratio > 25)) {
-        ++counter;
-        continue;
-      }
-      distance_points_ellipse(dist, pts, q);
-      if (weightedType != kNoWeight)
-        for (size_t i = 0; i < dist.size(); ++i) dist[i] = dist[i] * weights[i];
-      // most fits lose to the best so far: a counting pass rejects them
-      // with the result the median comparison would give
-      const float S = median_below(dist, Sm) != 0 ? median_ref(dist) : Sm;
-      if (S < Sm) {
-        counter = 0;
-        qm = Ellipse(Q);
-        Sm = S;
-      } else {
-        ++counter;
-      }
-    } catch (...) {
+    // Ellipse(Q) is built only for the ratio test and the throw on
+    // imaginary semi-axes (a dropped, uncounted trial); the invariants
+    // decide both without it except next to a boundary
+    int cls = conic_trial_class_fast(t, 0.04f, 25.f);
+    if (cls < 0) cls = conic_trial_class_exact(Q, 0.04f, 25.f);
+    if (cls == 0) continue;
+    if (cls == 1) {
+      ++counter;
+      continue;
+    }
+    distance_points_conic(dist, pts, Q);
+    if (weightedType != kNoWeight)
+      for (size_t i = 0; i < dist.size(); ++i) dist[i] = dist[i] * weights[i];
+    // most fits lose to the best so far: a counting pass rejects them
+    // with the result the median comparison would give
+    const float S = median_below(dist, Sm) != 0 ? median_ref(dist) : Sm;
+    if (S < Sm) {
+      counter = 0;
+      qm = Ellipse(Q);
+      Sm = S;
+    } else {
+      ++counter;
     }
   }
   std::vector<float> vDistFinal;

@@ -168,7 +168,10 @@ float distance_point_ellipse(float x, float y, const Ellipse& q) {
 
 void distance_points_ellipse(std::vector<float>& dist, const std::vector<Pt2>& pts,
                              const Ellipse& q) {
-  const Mat3& Q = q.matrix();
+  distance_points_conic(dist, pts, q.matrix());
+}
+
+void distance_points_conic(std::vector<float>& dist, const std::vector<Pt2>& pts, const Mat3& Q) {
   dist.resize(pts.size());
   for (size_t i = 0; i < pts.size(); ++i) {
     const float x = pts[i].x, y = pts[i].y, w = 1.f;
@@ -184,6 +187,54 @@ void distance_points_ellipse(std::vector<float>& dist, const std::vector<Pt2>& p
     d = d + Q.m[2][2];
     d = d * d;
     dist[i] = d / denom;
+  }
+}
+
+// Ellipse::compute_parameters rotates the conic by half the angle of
+// (p0 - p2, p1), which makes Auu the larger and Avv the smaller eigenvalue
+// of the quadratic part M = [t0 t1; t1 t2]; the centre value is
+// w = 1 - u^T M^-1 u with u = (t3, t4), the semi-axes are real unless w has
+// the eigenvalues' sign, and (a/b)^2 = Avv/Auu.  Products of two floats are
+// exact in double, so these invariants carry only double rounding; the float
+// derivation is trusted to within rho, 1000 float ulps scaled by M's
+// conditioning s / |Avv|, on Auu, Avv, the ratio and the terms of w (its
+// errors, from the rounded rotation angle, the rotated coefficients and
+// their squares, stay below about 80 ulps times that conditioning), and
+// outcomes closer than that to a boundary are left to it.
+int conic_trial_class_fast(const float t[5], float lo, float hi) {
+  const double a = t[0], b = t[1], c = t[2], u = t[3], v = t[4];
+  const double det = a * c - b * b;
+  if (!(det > 0.0)) return -1;
+  const double half = 0.5 * (a + c);
+  const double rad = std::sqrt(0.25 * (a - c) * (a - c) + b * b);
+  const double big = half >= 0.0 ? half + rad : half - rad;  // larger magnitude
+  const double small = det / big;
+  const double s = std::fabs(a) + std::fabs(c) + 2.0 * std::fabs(b);
+  const double lmin = std::fabs(small);
+  if (!(s < 1e3 * lmin)) return -1;
+  const double rho = 1000.0 * 0x1p-24 * (1.0 + s / lmin);
+  // w = 1 - q: the float path's terms Au^2/(4 Auu) and Av^2/(4 Avv) share
+  // the sign of q and sum to it, each off by at most rho of q
+  const double q = (c * u * u - 2.0 * b * u * v + a * v * v) / det;
+  const double w = 1.0 - q;
+  if (!(std::fabs(w) > rho * (1.0 + std::fabs(q)))) return -1;
+  if ((w > 0.0) == (half > 0.0)) return 0;
+  // (a/b)^2: Avv/Auu, at most 1 for a positive definite M, at least 1 else
+  const double r = half > 0.0 ? small / big : big / small;
+  const double lo2 = (double)lo * (double)lo, hi2 = (double)hi * (double)hi;
+  const double m = 2.0 * rho;
+  if (r * (1.0 + m) < lo2 || r * (1.0 - m) > hi2) return 1;
+  if (r * (1.0 - m) > lo2 && r * (1.0 + m) < hi2) return 2;
+  return -1;
+}
+
+int conic_trial_class_exact(const Mat3& Q, float lo, float hi) {
+  try {
+    const Ellipse q(Q);
+    const float ratio = q.a() / q.b();
+    return (ratio < lo || ratio > hi) ? 1 : 2;
+  } catch (const std::invalid_argument&) {
+    return 0;
   }
 }
 

@@ -71,6 +71,18 @@ void compute_hull(const Ellipse& e, float delta, Ellipse& qin, Ellipse& qout);
 float distance_point_ellipse(float x, float y, const Ellipse& q);
 // Distance.cpp:323-417: the batched form (AVX2 operation order).
 void distance_points_ellipse(std::vector<float>& dist, const std::vector<Pt2>& pts, const Ellipse& q);
+// The same distances to the conic x^T Q x = 0 (the ellipse's matrix).
+void distance_points_conic(std::vector<float>& dist, const std::vector<Pt2>& pts, const Mat3& Q);
+
+// Outcome of a RANSAC trial's classification (Vote.cpp:486-503) for the
+// conic t0 x^2 + 2 t1 xy + t2 y^2 + 2 t3 x + 2 t4 y + 1 with t0 t2 - t1^2 > 0:
+// 0 when Ellipse(Q) throws (imaginary semi-axes), 1 when the semi-axis
+// ratio a/b lies outside [lo, hi], 2 otherwise.  conic_trial_class_fast
+// decides it from the conic's invariants in double and returns -1 where
+// the float derivation's rounding could decide otherwise;
+// conic_trial_class_exact runs the float derivation itself.
+int conic_trial_class_fast(const float t[5], float lo, float hi);
+int conic_trial_class_exact(const Mat3& Q, float lo, float hi);
 
 // EllipseFromPoints.cpp:157-197.
 int intersect_ellipse_with_line(const Ellipse& e, float y, bool horizontal, float out[2]);

@@ -226,6 +226,31 @@ int oracle_linking_mismatches(const float* g, int n) {
   return bad;
 }
 
+// The RANSAC trials' classification of the shared candidate stage
+// (cctagx::conic_trial_class_fast) against the float derivation of
+// Ellipse(Q) and its ratio test (Vote.cpp:486-503) for n conics
+// t[5i .. 5i+4] with t0 t2 - t1^2 > 0: the number of conics the fast test
+// decides differently; *undecided receives the number it leaves to the
+// float derivation.
+int oracle_conic_class_mismatches(const float* t, int n, float lo, float hi, int* undecided) {
+  int bad = 0, open = 0;
+  for (int i = 0; i < n; ++i) {
+    const float* c = t + 5 * i;
+    cctagx::Mat3 Q = cctagx::Mat3::zero();
+    Q.m[0][0] = c[0];
+    Q.m[0][1] = Q.m[1][0] = c[1];
+    Q.m[0][2] = Q.m[2][0] = c[3];
+    Q.m[1][1] = c[2];
+    Q.m[1][2] = Q.m[2][1] = c[4];
+    Q.m[2][2] = 1.f;
+    const int fast = cctagx::conic_trial_class_fast(c, lo, hi);
+    if (fast < 0) ++open;
+    else if (fast != cctagx::conic_trial_class_exact(Q, lo, hi)) ++bad;
+  }
+  if (undecided) *undecided = open;
+  return bad;
+}
+
 // Wall time of the full CPU detection (front end + candidates + identification)
 // on `threads` threads.
 double oracle_time_detect(const cctagx_params* params, const uint8_t* img, int w, int h, int iters,

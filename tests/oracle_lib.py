@@ -49,6 +49,7 @@ def load():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_detect.restype = ctypes.c_double
     o.oracle_linking_mismatches.argtypes = [vp, ctypes.c_int]
+    o.oracle_conic_class_mismatches.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp]
     o.oracle_identify_given.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
     _o = o
@@ -198,3 +199,13 @@ def identify_given(img: np.ndarray, markers, params=None):
     if rc != 0:
         raise RuntimeError("oracle identification failed")
     return status, ids, centers
+
+
+def conic_class_mismatches(conics: np.ndarray, lo: float = 0.04, hi: float = 25.0):
+    """(mismatches, undecided) of the candidate stage's fast RANSAC-trial
+    classification against the float derivation, for conics (n x 5 float32)."""
+    c = np.ascontiguousarray(conics, dtype=np.float32)
+    open_ = ctypes.c_int(0)
+    bad = load().oracle_conic_class_mismatches(c.ctypes.data, c.shape[0], lo, hi,
+                                               ctypes.byref(open_))
+    return bad, open_.value
