@@ -93,6 +93,46 @@ class Marker(ctypes.Structure):
                 "homography": list(self.homography)}
 
 
+class IdentMarker(ctypes.Structure):
+    """cctagx_ident_marker."""
+    _fields_ = [
+        ("ellipse", c_f32 * 5), ("center", c_f32 * 2), ("scale", c_f32), ("n_points", c_i32),
+        ("points", P(c_f32)),
+    ]
+
+
+class IdentResult(ctypes.Structure):
+    """cctagx_ident_result."""
+    _fields_ = [
+        ("status", c_i32), ("id", c_i32), ("x", c_f32), ("y", c_f32), ("quality", c_f32),
+        ("homography", c_f32 * 9), ("n_cuts", c_i32), ("stops", (c_f32 * 2) * 32),
+    ]
+
+    def as_dict(self) -> dict:
+        n = self.n_cuts
+        return {"status": self.status, "id": self.id, "x": self.x, "y": self.y,
+                "quality": self.quality, "homography": list(self.homography),
+                "stops": np.array([[self.stops[k][0], self.stops[k][1]] for k in range(n)],
+                                  np.float32).reshape(n, 2)}
+
+
+def ident_markers(markers):
+    """A cctagx_ident_marker array for marker dicts with "ellipse" (cx, cy,
+    a, b, angle), "center0" (x, y), "points" (n x 4: x, y, gx, gy) and an
+    optional "scale"; returns (array, the point buffers it refers to)."""
+    arr = (IdentMarker * max(len(markers), 1))()
+    keep = []
+    for i, m in enumerate(markers):
+        pts = np.ascontiguousarray(np.asarray(m["points"], np.float32).reshape(-1, 4))
+        keep.append(pts)
+        arr[i].ellipse[:] = [float(v) for v in m["ellipse"]]
+        arr[i].center[:] = [float(v) for v in m["center0"]]
+        arr[i].scale = float(m.get("scale", 1.0))
+        arr[i].n_points = len(pts)
+        arr[i].points = pts.ctypes.data_as(P(c_f32))
+    return arr, keep
+
+
 class DetectStats(ctypes.Structure):
     """cctagx_detect_stats."""
     _fields_ = [
@@ -186,15 +226,18 @@ def load() -> ctypes.CDLL:
     lib.cctagx_params_default.restype = None
     lib.cctagx_create.argtypes = [P(Params), ctypes.c_int, ctypes.c_int, P(vp)]
     lib.cctagx_destroy.argtypes = [vp]
-    lib.cctagx_front_end.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.cctagx_front_end.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+    lib.cctagx_set_bank.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.cctagx_identify.argtypes = [vp, P(IdentMarker), ctypes.c_int, P(IdentResult)]
+    lib.cctagx_last_markers.argtypes = [vp, P(Marker), ctypes.c_int, P(ctypes.c_int)]
     lib.cctagx_get_level.argtypes = [vp, ctypes.c_int, P(LevelView)]
     lib.cctagx_set_stage_timing.argtypes = [vp, ctypes.c_int]
     lib.cctagx_stage_count.argtypes = [vp]
     lib.cctagx_stage_time.argtypes = [vp, ctypes.c_int, P(ctypes.c_char_p), P(c_f32)]
     lib.cctagx_time_front_end.argtypes = [vp, ctypes.c_int, P(c_f32)]
-    lib.cctagx_detect.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_uint64, P(Marker),
-                                  ctypes.c_int, P(ctypes.c_int)]
-    lib.cctagx_load.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.cctagx_detect.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                  ctypes.c_uint64, P(Marker), ctypes.c_int, P(ctypes.c_int)]
+    lib.cctagx_load.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
     lib.cctagx_detect_loaded.argtypes = [vp, ctypes.c_uint64, P(Marker), ctypes.c_int,
                                          P(ctypes.c_int)]
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]
@@ -205,7 +248,8 @@ def load() -> ctypes.CDLL:
     lib.cctagx_get_host_threads.argtypes = [vp]
     lib.cctagx_batch_create.argtypes = [P(Params), ctypes.c_int, P(c_i32), P(c_i32), P(vp)]
     lib.cctagx_batch_destroy.argtypes = [vp]
-    lib.cctagx_batch_load.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    lib.cctagx_batch_load.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_size_t]
     lib.cctagx_batch_set_partition.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.cctagx_batch_detect.argtypes = [vp, P(Marker), ctypes.c_int, P(c_i32), P(ctypes.c_int)]
     lib.cctagx_batch_get_stats.argtypes = [vp, P(BatchStats)]

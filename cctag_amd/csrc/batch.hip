@@ -198,10 +198,15 @@ int cctagx_batch_destroy(cctagx_batch* b) {
   return CCTAGX_OK;
 }
 
-int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, size_t stride) {
+int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, int width, int height,
+                      size_t stride) {
   if (!b || !gray || frame < 0 || frame >= b->n_frames ||
       stride < (size_t)b->sizes[frame].first)
     return CCTAGX_ERR_ARG;
+  if (width != b->sizes[frame].first || height != b->sizes[frame].second) {
+    set_error("frame size differs from the one given at cctagx_batch_create");
+    return CCTAGX_ERR_ARG;
+  }
   if (int rc = b->fe.upload(frame, gray, stride, b->stream)) return rc;
   b->loaded[frame] = true;
   return CCTAGX_OK;

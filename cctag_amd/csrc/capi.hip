@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "device.hpp"
@@ -157,19 +158,116 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   return CCTAGX_OK;
 }
 
-int cctagx_load(cctagx_handle* h, const uint8_t* gray, size_t stride) {
-  if (!h || !gray || stride < (size_t)h->width) return CCTAGX_ERR_ARG;
-  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
-  h->have_frame = true;
+// A frame handed to the handle: non-null, the handle's size, a stride that
+// covers a row.
+static int check_frame(const cctagx_handle* h, const uint8_t* gray, int width, int height,
+                       size_t stride) {
+  if (!h || !gray) return CCTAGX_ERR_ARG;
+  if (width != h->width || height != h->height) {
+    set_error("frame is " + std::to_string(width) + "x" + std::to_string(height) +
+              ", the handle was created for " + std::to_string(h->width) + "x" +
+              std::to_string(h->height));
+    return CCTAGX_ERR_ARG;
+  }
+  if (stride < (size_t)h->width) {
+    set_error("stride shorter than a row");
+    return CCTAGX_ERR_ARG;
+  }
   return CCTAGX_OK;
 }
 
-int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
-                  cctagx_marker* out, int max_out, int* n_out) {
-  if (!h || !gray || stride < (size_t)h->width || (max_out > 0 && !out) || !n_out)
-    return CCTAGX_ERR_ARG;
-  if (int rc = cctagx_load(h, gray, stride)) return rc;
+int cctagx_load(cctagx_handle* h, const uint8_t* gray, int width, int height, size_t stride) {
+  if (int rc = check_frame(h, gray, width, height, stride)) return rc;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  h->have_frame = true;
+  h->have_front_end = false;
+  return CCTAGX_OK;
+}
+
+int cctagx_detect(cctagx_handle* h, const uint8_t* gray, int width, int height, size_t stride,
+                  uint64_t frame, cctagx_marker* out, int max_out, int* n_out) {
+  if ((max_out > 0 && !out) || !n_out) return CCTAGX_ERR_ARG;
+  if (int rc = cctagx_load(h, gray, width, height, stride)) return rc;
   return cctagx_detect_loaded(h, frame, out, max_out, n_out);
+}
+
+int cctagx_set_bank(cctagx_handle* h, const float* ratios, int rows, int cols) {
+  if (!h || !ratios || rows < 1 || cols < 1) return CCTAGX_ERR_ARG;
+  if (rows > kMaxBankRows || rows > kMaxBank || cols > kMaxRatios) {
+    set_error("the identification kernels take at most 128 bank rows of at most 8 ratios");
+    return CCTAGX_ERR_UNSUPPORTED;
+  }
+  std::vector<std::vector<float>> bank(rows, std::vector<float>(cols));
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) {
+      const float v = ratios[(size_t)i * cols + j];
+      if (!(std::isfinite(v) && v > 0.f)) {
+        set_error("bank ratios must be finite and positive");
+        return CCTAGX_ERR_ARG;
+      }
+      bank[i][j] = v;
+    }
+  try {
+    h->ident.set_bank(bank, h->host);
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_HIP;
+  }
+  h->bank = std::move(bank);
+  return CCTAGX_OK;
+}
+
+int cctagx_identify(cctagx_handle* h, const cctagx_ident_marker* in, int n,
+                    cctagx_ident_result* out) {
+  if (!h || n < 0 || (n > 0 && (!in || !out))) return CCTAGX_ERR_ARG;
+  if (!h->have_front_end) return CCTAGX_ERR_STATE;
+  std::vector<Marker> markers(n);
+  try {
+    for (int k = 0; k < n; ++k) {
+      const cctagx_ident_marker& a = in[k];
+      if (a.n_points < 0 || (a.n_points > 0 && !a.points)) return CCTAGX_ERR_ARG;
+      Marker& m = markers[k];
+      m.rescaled_outer_ellipse =
+          Ellipse(a.ellipse[0], a.ellipse[1], a.ellipse[2], a.ellipse[3], a.ellipse[4]);
+      m.outer_ellipse = m.rescaled_outer_ellipse;
+      m.center_img = Pt2(a.center[0], a.center[1]);
+      m.scale = a.scale;
+      m.rescaled_outer_ellipse_points.reserve(a.n_points);
+      for (int j = 0; j < a.n_points; ++j) {
+        const float* q = a.points + 4 * (size_t)j;
+        m.rescaled_outer_ellipse_points.emplace_back(q[0], q[1], q[2], q[3]);
+      }
+    }
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_ARG;
+  }
+  try {
+    const PlaneDesc& d0 = h->fe.planes().planes[0];
+    h->ident.set_image(h->fe.buffers().quad + d0.pix, d0.w, d0.h);
+    h->ident.clear_stats();
+    identify_markers(markers, h->host, h->bank, h->ident, h->pool.get(), nullptr);
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return CCTAGX_ERR_HIP;
+  }
+  for (int k = 0; k < n; ++k) {
+    const Marker& m = markers[k];
+    cctagx_ident_result& o = out[k];
+    o.status = m.status;
+    o.id = m.id;
+    o.x = m.center_img.x;
+    o.y = m.center_img.y;
+    o.quality = m.quality;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) o.homography[i * 3 + j] = m.homography.m[i][j];
+    o.n_cuts = (int32_t)std::min<size_t>(m.cut_stops.size(), 32);
+    for (int c = 0; c < 32; ++c) {
+      o.stops[c][0] = c < o.n_cuts ? m.cut_stops[c].x : 0.f;
+      o.stops[c][1] = c < o.n_cuts ? m.cut_stops[c].y : 0.f;
+    }
+  }
+  return CCTAGX_OK;
 }
 
 int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
@@ -217,6 +315,14 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
     return CCTAGX_ERR_HIP;
   }
   h->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const int n = (int)h->markers.size();
+  for (int i = 0; i < n && i < max_out; ++i) to_c_marker(h->markers[i], out[i]);
+  *n_out = n;
+  return CCTAGX_OK;
+}
+
+int cctagx_last_markers(cctagx_handle* h, cctagx_marker* out, int max_out, int* n_out) {
+  if (!h || (max_out > 0 && !out) || !n_out) return CCTAGX_ERR_ARG;
   const int n = (int)h->markers.size();
   for (int i = 0; i < n && i < max_out; ++i) to_c_marker(h->markers[i], out[i]);
   *n_out = n;
@@ -299,8 +405,9 @@ int cctagx_destroy(cctagx_handle* h) {
   return CCTAGX_OK;
 }
 
-int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, size_t stride) {
-  if (!h || !gray || stride < (size_t)h->width) return CCTAGX_ERR_ARG;
+int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, int width, int height,
+                     size_t stride) {
+  if (int rc = check_frame(h, gray, width, height, stride)) return rc;
   if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
   h->have_frame = true;
   h->timer.reset();

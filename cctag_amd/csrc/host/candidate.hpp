@@ -131,6 +131,7 @@ struct Marker {
   std::vector<Pt2> rescaled_outer_ellipse_points;
   Mat3 homography = Mat3::zero();
   std::vector<float> radius_ratios;
+  std::vector<Pt2> cut_stops;           // stops of the cuts identify_step_1 selected
 
   // CCTag.hpp:68-93 constructor.
   static Marker make(const Ellipse& outer, std::vector<std::vector<Pt2>> pts, int level,

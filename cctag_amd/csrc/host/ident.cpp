@@ -249,6 +249,7 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     else {
       F.detected[i] = status::id_reliable;
       F.stops[i] = std::move(c.stops);
+      (*F.markers)[i].cut_stops = F.stops[i];
     }
   }
   // frames whose first refined marker is not the guessed one: their jobs
@@ -337,7 +338,7 @@ void identify_frames(std::vector<IdentFrame>& frames, const HostParams& p,
     std::vector<std::list<float>> vScore(nb);
     std::vector<float> v(nb);
     for (size_t k = 0; k < ncut; ++k) {
-      if (r.oob[k]) return;
+      if (r.oob[k]) continue;  // Identification.cpp:78: only in-bounds cuts vote
       // the largest exp(-d) keyed in a std::map<float, int>: the last row
       // among equal keys; a scan gives the same when no key is NaN
       bool nan = false;

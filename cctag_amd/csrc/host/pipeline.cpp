@@ -93,18 +93,4 @@ void finish_markers(std::vector<Marker>& markers, std::vector<Marker>& out) {
   std::stable_sort(out.begin(), out.end(), [](const Marker& a, const Marker& b) { return a.id < b.id; });
 }
 
-void detect_markers(const std::vector<LevelEdges>& levels, const HostParams& p,
-                    const std::vector<std::vector<float>>& bank, IdentBackend* ident,
-                    std::vector<Marker>& out, TaskPool* pool, HostTimes* times,
-                    FrameWorkspace* ws) {
-  std::vector<Marker> markers;
-  frame_candidates(levels, p, pool, markers, times, ws);
-  auto t = Clock::now();
-  if (p.do_identification && ident) identify_markers(markers, p, bank, *ident, pool, times);
-  if (times) times->ident += ms_since(t);
-  t = Clock::now();
-  finish_markers(markers, out);
-  if (times) times->finish += ms_since(t);
-}
-
 }  // namespace cctagx

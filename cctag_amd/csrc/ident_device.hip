@@ -32,6 +32,18 @@ int GpuIdentBackend::init(const std::vector<std::vector<float>>& bank, const Hos
   return 0;
 }
 
+void GpuIdentBackend::set_bank(const std::vector<std::vector<float>>& bank, const HostParams& p) {
+  const IdentConstants k = ident_constants(p, bank);
+  if (k.bank_rows > kMaxBankRows || k.bank_rows > kMaxBank || k.bank_cols > kMaxRatios)
+    throw std::runtime_error("marker bank larger than the identification kernels accept");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  k_ = k;
+  if (d_inv_ratio_)
+    hip_check(hipMemcpy(d_inv_ratio_, &k_.inv_ratio[0][0], sizeof(float) * kMaxBank * kMaxRatios,
+                        hipMemcpyHostToDevice),
+              "hipMemcpy bank");
+}
+
 void GpuIdentBackend::ensure(int n_markers, int n_points) {
   if (n_markers <= cap_markers_ && n_points <= cap_points_) return;
   const int nm = std::max(n_markers, cap_markers_), np = std::max(n_points, cap_points_);

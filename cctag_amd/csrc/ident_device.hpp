@@ -110,6 +110,8 @@ struct DeviceImage {
 class GpuIdentBackend : public IdentBackend {
  public:
   int init(const std::vector<std::vector<float>>& bank, const HostParams& p, hipStream_t s);
+  // Replaces the marker bank the read-out scores (after init).
+  void set_bank(const std::vector<std::vector<float>>& bank, const HostParams& p);
   // Images that CutJob::image / RefineJob::image index.
   void set_images(const std::vector<DeviceImage>& images) { images_ = images; }
   void set_image(const uint32_t* dev_quad, int w, int h) { images_.assign(1, DeviceImage{dev_quad, w, h}); }

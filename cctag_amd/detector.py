@@ -40,34 +40,45 @@ class Detector:
         except Exception:
             pass
 
-    def _gray(self, img: np.ndarray) -> np.ndarray:
+    @staticmethod
+    def _gray(img: np.ndarray) -> np.ndarray:
         img = np.ascontiguousarray(img)
         if img.dtype != np.uint8 or img.ndim != 2:
             raise ValueError("expected a 2-D uint8 gray frame")  # Detection.cpp:815-817
-        if img.shape != (self.height, self.width):
-            raise ValueError("frame size %s differs from the handle's %s"
-                             % (img.shape, (self.height, self.width)))
         return img
 
     def front_end(self, img: np.ndarray) -> None:
         img = self._gray(img)
-        _lib.check(self._lib.cctagx_front_end(self._h, img.ctypes.data, img.strides[0]),
+        h, w = img.shape
+        _lib.check(self._lib.cctagx_front_end(self._h, img.ctypes.data, w, h, img.strides[0]),
                    "cctagx_front_end")
+
+    def _markers(self, buf, n: int, max_markers: int) -> list:
+        if n > max_markers:  # every marker, as the reference returns them all
+            buf = (_lib.Marker * n)()
+            got = ctypes.c_int()
+            _lib.check(self._lib.cctagx_last_markers(self._h, buf, n, ctypes.byref(got)),
+                       "cctagx_last_markers")
+        return [buf[i].as_dict() for i in range(n)]
 
     def detect(self, img: np.ndarray, frame: int = 0, max_markers: int = 1024) -> list:
         """cctagDetection (ICCTag.hpp:124-131): every marker found, sorted by
-        id; only status == 1 markers are identified."""
+        id; only status == 1 markers are identified.  A frame of another size
+        than the handle's is rejected (the reference stops the process,
+        Detection.cpp:753-757)."""
         img = self._gray(img)
+        h, w = img.shape
         buf = (_lib.Marker * max_markers)()
         n = ctypes.c_int()
-        _lib.check(self._lib.cctagx_detect(self._h, img.ctypes.data, img.strides[0], frame, buf,
-                                           max_markers, ctypes.byref(n)), "cctagx_detect")
-        return [buf[i].as_dict() for i in range(min(n.value, max_markers))]
+        _lib.check(self._lib.cctagx_detect(self._h, img.ctypes.data, w, h, img.strides[0], frame,
+                                           buf, max_markers, ctypes.byref(n)), "cctagx_detect")
+        return self._markers(buf, n.value, max_markers)
 
     def load(self, img: np.ndarray) -> None:
         """TagPipe::load (cuda/tag.cu:154-159): the frame stays resident."""
         img = self._gray(img)
-        _lib.check(self._lib.cctagx_load(self._h, img.ctypes.data, img.strides[0]),
+        h, w = img.shape
+        _lib.check(self._lib.cctagx_load(self._h, img.ctypes.data, w, h, img.strides[0]),
                    "cctagx_load")
 
     def detect_loaded(self, frame: int = 0, max_markers: int = 1024) -> list:
@@ -76,7 +87,27 @@ class Detector:
         n = ctypes.c_int()
         _lib.check(self._lib.cctagx_detect_loaded(self._h, frame, buf, max_markers,
                                                   ctypes.byref(n)), "cctagx_detect_loaded")
-        return [buf[i].as_dict() for i in range(min(n.value, max_markers))]
+        return self._markers(buf, n.value, max_markers)
+
+    def set_bank(self, ratios) -> None:
+        """The marker bank identification reads (rows of radius ratios,
+        innermost ring first), as cctagDetection's pBank supplies it."""
+        r = np.ascontiguousarray(np.asarray(ratios, np.float32))
+        if r.ndim != 2:
+            raise ValueError("a bank is a 2-D table of radius ratios")
+        _lib.check(self._lib.cctagx_set_bank(self._h, r.ctypes.data, r.shape[0], r.shape[1]),
+                   "cctagx_set_bank")
+
+    def identify(self, markers) -> list:
+        """The identification of given markers on the frame of the last front
+        end (cctagx_identify): dicts with "ellipse", "center0", "points" and
+        an optional "scale"; returns per marker status, id, centre, quality,
+        homography and the selected cut stops."""
+        arr, keep = _lib.ident_markers(markers)
+        out = (_lib.IdentResult * max(len(markers), 1))()
+        _lib.check(self._lib.cctagx_identify(self._h, arr, len(markers), out), "cctagx_identify")
+        del keep
+        return [out[i].as_dict() for i in range(len(markers))]
 
     def detect_times(self) -> dict:
         host, sel, ref = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
@@ -171,8 +202,8 @@ class BatchDetector:
         w, h = self.sizes[frame]
         if img.dtype != np.uint8 or img.shape != (h, w):
             raise ValueError("frame %d must be a %dx%d uint8 image" % (frame, w, h))
-        _lib.check(self._lib.cctagx_batch_load(self._h, frame, img.ctypes.data, img.strides[0]),
-                   "cctagx_batch_load")
+        _lib.check(self._lib.cctagx_batch_load(self._h, frame, img.ctypes.data, w, h,
+                                               img.strides[0]), "cctagx_batch_load")
 
     def set_refinement(self, gradient_steps: int = 0) -> None:
         """As Detector.set_refinement, for every frame of the batch."""

@@ -268,6 +268,32 @@ def make_frame(cfg: FrameConfig, seed: int, n_markers: int | None = None):
     return np.clip(np.rint(img), 0, 255).astype(np.uint8), gts
 
 
+def border_frame(seed: int = 7000, w: int = 640, h: int = 480):
+    """Markers cut by the frame border: four markers (outer radius 40-60 px,
+    tilt <= 30 deg) whose centres lie 2-4 px inside the left, right, top and
+    bottom borders, and two whole markers inside; blur 1 px, noise 4 DN.
+    Returns (uint8 image, list[MarkerGT])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    img = 128.0 + 20.0 * _perlin(h, w, 256, rng)
+    ratios = bank_ratios(3)
+    focal = 2.0 * max(w, h)
+    d = lambda: rng.uniform(2.0, 4.0)
+    centres = [(d(), rng.uniform(0.3, 0.7) * h), (w - d(), rng.uniform(0.3, 0.7) * h),
+               (rng.uniform(0.3, 0.7) * w, d()), (rng.uniform(0.3, 0.7) * w, h - d()),
+               (0.3 * w, 0.5 * h), (0.7 * w, 0.5 * h)]
+    gts = []
+    for cx, cy in centres:
+        mid = int(rng.integers(0, len(ratios)))
+        r = rng.uniform(40.0, 60.0)
+        H = marker_homography(cx, cy, r, math.radians(rng.uniform(0, 30)),
+                              rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), focal)
+        _render_marker(img, H, ratios[mid])
+        gts.append(MarkerGT(mid, (cx - 0.5, cy - 0.5), r, H))
+    img = ndimage.gaussian_filter(img, 1.0, mode="nearest")
+    img = img + rng.normal(0.0, 4.0, size=img.shape)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8), gts
+
+
 def config_frame(config: int, k: int = 0):
     """Frame k of configuration 1, 2 or 3."""
     return make_frame(CONFIGS[config], 1000 * config + k)

@@ -66,16 +66,30 @@ typedef struct cctagx_handle cctagx_handle;
 
 /* Replaces TagPipe::initialize(w, h) (cuda/tag.cu:76-127) and the per-pipe
  * allocation of initCuda (Detection.cpp:732-760).  Allocates every device and
- * pinned host buffer for frames of width x height once. */
+ * pinned host buffer for frames of width x height once.  The handle's
+ * resolution is fixed: every entry point that takes a frame checks its
+ * width and height against it and returns CCTAGX_ERR_ARG on a mismatch,
+ * where the reference stops the process (Detection.cpp:753-757). */
 int cctagx_create(const cctagx_params* params, int width, int height, cctagx_handle** out);
 int cctagx_destroy(cctagx_handle* h);
+
+/* The marker bank identification reads: rows x cols radius ratios
+ * (R_outer / r_j, innermost ring first, row-major), the table of
+ * CCTagMarkersBank (CCTagMarkersBank.cpp:22-78) that cctagDetection's
+ * `pBank` / `cctagBankFilename` arguments supply (ICCTag.cpp:60-67,86-93).
+ * A handle starts with the built-in bank of params->n_crowns
+ * (CCTagMarkersBank.cpp:120-282).  1 <= rows <= 128, 1 <= cols <= 8, every
+ * ratio finite and > 0; a bank whose rows differ in length is not
+ * supported (CCTAGX_ERR_UNSUPPORTED). */
+int cctagx_set_bank(cctagx_handle* h, const float* ratios, int rows, int cols);
 
 /* ---- front end (pyramid .. vote) ---------------------------------------
  * Replaces TagPipe::load + TagPipe::tagframe (cuda/tag.cu:154-216): upload a
  * row-major 8-bit gray frame (stride in bytes) and run every pyramid level
  * through derivatives, Canny, thinning, edge compaction, gradient descent,
  * vote and seed selection on the GPU. */
-int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, size_t stride);
+int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, int width, int height,
+                     size_t stride);
 
 /* Host view of one level after the front end; what TagPipe::convertToHost
  * (cuda/frame_export.cu:23-165) and TagPipe::getPlane/getDx/getDy/getEdges
@@ -111,7 +125,7 @@ int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* out);
  * One marker as cctag::ICCTag exposes it (ICCTag.hpp:28-82): imaged centre
  * x(), y(), id(), getStatus() (1 = identified; negative codes of
  * CCTag.hpp:416-426), rescaledOuterEllipse(), plus quality, pyramid level,
- * scale and the image->marker homography. */
+ * scale and the homography. */
 typedef struct cctagx_marker {
   float x, y;
   int32_t id;
@@ -120,7 +134,7 @@ typedef struct cctagx_marker {
   int32_t pyramid_level;
   float scale;
   float ellipse[5];      /* rescaled outer ellipse: cx, cy, a, b, angle */
-  float homography[9];   /* row-major */
+  float homography[9];   /* row-major, marker plane -> image (CCTag::homography) */
 } cctagx_marker;
 
 /* Replaces cctag::cctagDetection(markers, pipeId, frame, graySrc, params,
@@ -128,17 +142,49 @@ typedef struct cctagx_marker {
  * identification on the GPU, candidate stage on the host.  Writes up to
  * max_out markers sorted by id and their total count in *n_out (which may
  * exceed max_out).  `frame` is informational, as in the reference. */
-int cctagx_detect(cctagx_handle* h, const uint8_t* gray, size_t stride, uint64_t frame,
-                  cctagx_marker* out, int max_out, int* n_out);
+int cctagx_detect(cctagx_handle* h, const uint8_t* gray, int width, int height, size_t stride,
+                  uint64_t frame, cctagx_marker* out, int max_out, int* n_out);
+
+/* The markers of the last cctagx_detect / cctagx_detect_loaded (up to
+ * max_out; *n_out = their count), for a caller whose buffer was too small. */
+int cctagx_last_markers(cctagx_handle* h, cctagx_marker* out, int max_out, int* n_out);
 
 /* The two halves of cctagx_detect.  cctagx_load is TagPipe::load
  * (cuda/tag.cu:154-159): the frame is copied into device memory and stays
  * resident.  cctagx_detect_loaded runs the detection on the resident frame
  * (TagPipe::tagframe onwards), so a caller can time or repeat detection
  * without the host-to-device copy. */
-int cctagx_load(cctagx_handle* h, const uint8_t* gray, size_t stride);
+int cctagx_load(cctagx_handle* h, const uint8_t* gray, int width, int height, size_t stride);
 int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
                          int* n_out);
+
+/* ---- identification of given markers ------------------------------------
+ * The identification loop of cctagDetection (Detection.cpp:873-965:
+ * identify_step_1, then TagPipe::uploadCuts / imageCenterOptLoop /
+ * imageCenterRetrieve and identify_step_2, Identification.cpp:1172-1480)
+ * for markers the caller supplies, on the frame of the last
+ * cctagx_front_end, cctagx_detect or cctagx_detect_loaded (the frame must
+ * have been through the front end: CCTAGX_ERR_STATE otherwise).  Markers are
+ * identified in list order, as the reference does; the conditioning scale of
+ * the grid search is that of the first marker refined in this call. */
+typedef struct cctagx_ident_marker {
+  float ellipse[5];     /* rescaledOuterEllipse: cx, cy, a, b, angle (level-0 pixels) */
+  float center[2];      /* initial imaged centre (CCTag::centerImg) */
+  float scale;          /* CCTag::scale(): 2^pyramid level of the detection */
+  int32_t n_points;     /* rescaledOuterEllipsePoints */
+  const float* points;  /* n_points x 4: x, y, gx, gy */
+} cctagx_ident_marker;
+typedef struct cctagx_ident_result {
+  int32_t status;       /* 1 identified, else CCTag.hpp:416-426 */
+  int32_t id;           /* -1 when no ID was read */
+  float x, y;           /* imaged centre after the grid search */
+  float quality;        /* 1 / normalised residual (CCTag::setQuality) */
+  float homography[9];  /* row-major, marker plane -> image */
+  int32_t n_cuts;       /* cuts identify_step_1 selected (<= 32) */
+  float stops[32][2];   /* their stops: refined outer-edge points */
+} cctagx_ident_result;
+int cctagx_identify(cctagx_handle* h, const cctagx_ident_marker* in, int n,
+                    cctagx_ident_result* out);
 
 /* Host threads of the candidate stage, the counterpart of the reference's
  * tbb::parallel_for loops (Detection.cpp:616,644,680).  A handle starts with
@@ -204,8 +250,10 @@ typedef struct cctagx_batch cctagx_batch;
 int cctagx_batch_create(const cctagx_params* params, int n_frames, const int32_t* widths,
                         const int32_t* heights, cctagx_batch** out);
 int cctagx_batch_destroy(cctagx_batch* b);
-/* Copies frame `frame` (row-major 8-bit, stride in bytes) to the device. */
-int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, size_t stride);
+/* Copies frame `frame` (row-major 8-bit, stride in bytes) to the device;
+ * width and height must be those given at cctagx_batch_create. */
+int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, int width, int height,
+                      size_t stride);
 /* Work partition of the batched front end: filter tile variant (0, 1, 2 =
  * 64x16, 64x32, 64x64 tiles) and blocks per plane of the edge-indexed
  * kernels.  -1 / 0 leave the choice to the autotuner, which times the

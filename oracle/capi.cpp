@@ -6,7 +6,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <algorithm>
 #include <exception>
+#include <stdexcept>
 #include <memory>
 #include <vector>
 
@@ -53,6 +55,35 @@ int oracle_level_view(void* handle, int level, cctagx_level_view* v) {
 
 void oracle_free(void* handle) { delete static_cast<OracleRun*>(handle); }
 
+// Edge points, descent links, votes and seeds (Canny.cpp:15-34, Vote.cpp:
+// 58-247) of one level given its thinned edge map and derivatives.
+int oracle_edges_and_vote(const uint8_t* edges, const int16_t* dx, const int16_t* dy, int w, int h,
+                          int dist_search, int thr_grad, float ratio_voting, int n_crowns,
+                          int min_votes, void** out) {
+  try {
+    auto* r = new OracleRun();
+    r->p.dist_search = dist_search;
+    r->p.thr_grad_vote = thr_grad;
+    r->p.ratio_voting = ratio_voting;
+    r->p.n_crowns = n_crowns;
+    r->p.min_votes = min_votes;
+    r->levels.resize(1);
+    oracle::LevelResult& L = r->levels[0];
+    L.w = L.W0 = w;
+    L.h = L.H0 = h;
+    const size_t N = (size_t)w * h;
+    L.img.assign(N, 0);
+    L.edges.assign(edges, edges + N);
+    L.dx.assign(dx, dx + N);
+    L.dy.assign(dy, dy + N);
+    oracle::edges_and_vote(L, r->p);
+    *out = r;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 void oracle_derivatives(const uint8_t* img, int w, int h, int16_t* dx, int16_t* dy) {
   oracle::derivatives(img, w, h, dx, dy);
 }
@@ -93,8 +124,8 @@ double oracle_time_front_end(const uint8_t* img, int w, int h, int n_levels, int
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Full CPU detection: the oracle front end and identification around the
-// shared host candidate stage.
+// Full CPU detection: the oracle front end and the oracle identification
+// around the shared host candidate stage and de-duplication.
 #include "../cctag_amd/csrc/host/cabi.hpp"
 #include "../cctag_amd/csrc/host/pipeline.hpp"
 #include "oracle_ident.hpp"
@@ -130,69 +161,184 @@ oracle::Params front_params(const cctagx_params& c) {
   p.n_levels = c.number_of_processed_multires_layers;
   return p;
 }
+
+oracle::IdentParams ident_params(const cctagx_params& c) {
+  oracle::IdentParams p;
+  p.n_crowns = c.n_crowns;
+  p.n_samples_outer_ellipse = c.n_samples_outer_ellipse;
+  p.num_cuts_in_ident_step = c.num_cuts_in_ident_step;
+  p.num_samples_outer_edge_points_refinement = c.num_samples_outer_edge_points_refinement;
+  p.sample_cut_length = c.sample_cut_length;
+  p.imaged_center_n_grid_sample = c.imaged_center_n_grid_sample;
+  p.imaged_center_neighbour_size = c.imaged_center_neighbour_size;
+  p.min_ident_proba = c.min_ident_proba;
+  return p;
+}
+
+// The banks the oracle identifies with, set by the tests from the
+// reference's tables (oracle_set_bank); indexed by ring count.
+std::vector<std::vector<float>> g_bank[5];
+
+const std::vector<std::vector<float>>& bank_of(int n_crowns) {
+  if (n_crowns < 0 || n_crowns > 4 || g_bank[n_crowns].empty())
+    throw std::runtime_error("oracle: no bank set for this ring count");
+  return g_bank[n_crowns];
+}
+
+// The oracle identification of the candidate stage's markers, written back
+// into them.
+void identify_markers(std::vector<cctagx::Marker>& markers, const uint8_t* img, int w, int h,
+                      const cctagx_params& params, const std::vector<std::vector<float>>& bank) {
+  std::vector<oracle::IdentMarker> im(markers.size());
+  for (size_t i = 0; i < markers.size(); ++i) {
+    const cctagx::Marker& m = markers[i];
+    oracle::IdentMarker& o = im[i];
+    const cctagx::Ellipse& e = m.rescaled_outer_ellipse;
+    o.cx = e.cx(); o.cy = e.cy(); o.a = e.a(); o.b = e.b(); o.angle = e.angle();
+    for (const cctagx::Pt2& q : m.rescaled_outer_ellipse_points)
+      o.points.push_back(oracle::DPt{q.x, q.y, q.dx, q.dy});
+    o.center_x = m.center_img.x;
+    o.center_y = m.center_img.y;
+    o.scale = m.scale;
+  }
+  oracle::identify(img, w, h, ident_params(params), bank, im);
+  for (size_t i = 0; i < markers.size(); ++i) {
+    cctagx::Marker& m = markers[i];
+    const oracle::IdentMarker& o = im[i];
+    m.status = o.status;
+    m.id = o.id;
+    m.center_img = cctagx::Pt2(o.center_x, o.center_y);
+    if (o.status == 1 || o.status == -4 || o.status == -5 || o.status == -3) m.quality = o.quality;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) m.homography.m[r][c] = o.H[r * 3 + c];
+    m.cut_stops.clear();
+    for (const oracle::DPt& q : o.stops) m.cut_stops.emplace_back(q.x, q.y, q.dx, q.dy);
+  }
+}
 }  // namespace
 
 extern "C" {
 
+// The bank of n_crowns rings (rows x cols radius ratios, row-major) the
+// oracle identifies with.
+int oracle_set_bank(int n_crowns, const float* ratios, int rows, int cols) {
+  if (n_crowns < 0 || n_crowns > 4 || !ratios || rows < 1 || cols < 1) return -1;
+  g_bank[n_crowns].assign(rows, std::vector<float>(cols));
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) g_bank[n_crowns][i][j] = ratios[(size_t)i * cols + j];
+  return 0;
+}
+
 // cctagDetection on the CPU; returns the marker count or -1.  `threads` > 1
-// runs the shared host candidate stage and the per-candidate identification
-// on a pool of that many threads (the reference's tbb::parallel_for loops),
-// 1 serially; the front end is serial either way.
+// runs the shared host candidate stage on a pool of that many threads (the
+// reference's tbb::parallel_for loops), 1 serially; the front end and the
+// identification are serial either way.
 int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w, int h,
                           cctagx_marker* out, int max_out, int threads) {
   try {
+    const auto& bank = bank_of(params->n_crowns);
     std::vector<oracle::LevelResult> levels;
     oracle::front_end(img, w, h, front_params(*params), levels);
-    const auto bank = cctagx::marker_bank(params->n_crowns);
-    oracle::CpuIdentBackend backend(levels[0].img.data(), w, h, bank);
-    std::vector<cctagx::Marker> markers;
     std::unique_ptr<cctagx::TaskPool> pool;
     if (threads > 1) pool = std::make_unique<cctagx::TaskPool>(threads);
-    backend.set_pool(pool.get());
-    cctagx::detect_markers(level_edges(levels), cctagx::host_params(*params), bank, &backend,
-                           markers, pool.get());
-    for (size_t i = 0; i < markers.size() && (int)i < max_out; ++i) cctagx::to_c_marker(markers[i], out[i]);
-    return (int)markers.size();
+    std::vector<cctagx::Marker> markers, kept;
+    const cctagx::HostParams hp = cctagx::host_params(*params);
+    cctagx::frame_candidates(level_edges(levels), hp, pool.get(), markers);
+    if (hp.do_identification) identify_markers(markers, levels[0].img.data(), w, h, *params, bank);
+    cctagx::finish_markers(markers, kept);
+    for (size_t i = 0; i < kept.size() && (int)i < max_out; ++i) cctagx::to_c_marker(kept[i], out[i]);
+    return (int)kept.size();
   } catch (const std::exception&) {
     return -1;
   }
 }
 
-// The identification of given markers on an 8-bit frame (identify_markers
-// with this backend, serially): marker k has the outer ellipse ell[5k ..
-// 5k+4] (cx, cy, a, b, angle), npts[k] outer points pts[4j .. 4j+3]
-// (x, y, gx, gy; the markers' points one after the other) and the starting
-// centre c0[2k], c0[2k+1].  Writes each marker's status, ID and imaged
-// centre.  Returns 0, or -1 on an exception.
-int oracle_identify_given(const cctagx_params* params, const uint8_t* img, int w, int h, int n,
-                          const float* ell, const int* npts, const float* pts, const float* c0,
-                          int* status, int* id, float* center) {
+// The oracle identification of given markers on an 8-bit frame (w x h,
+// stride w), the counterpart of cctagx_identify.  Returns 0, or -1 on an
+// exception.
+int oracle_identify_given(const cctagx_params* params, const uint8_t* img, int w, int h,
+                          const cctagx_ident_marker* in, int n, cctagx_ident_result* out) {
   try {
-    const auto bank = cctagx::marker_bank(params->n_crowns);
-    oracle::CpuIdentBackend backend(img, w, h, bank);
-    std::vector<cctagx::Marker> markers(n);
-    size_t at = 0;
+    const auto& bank = bank_of(params->n_crowns);
+    std::vector<oracle::IdentMarker> im(n);
     for (int k = 0; k < n; ++k) {
-      cctagx::Marker& m = markers[k];
-      m.rescaled_outer_ellipse =
-          cctagx::Ellipse(ell[5 * k], ell[5 * k + 1], ell[5 * k + 2], ell[5 * k + 3], ell[5 * k + 4]);
-      m.outer_ellipse = m.rescaled_outer_ellipse;
-      for (int j = 0; j < npts[k]; ++j, ++at)
-        m.rescaled_outer_ellipse_points.emplace_back(pts[4 * at], pts[4 * at + 1], pts[4 * at + 2],
-                                                     pts[4 * at + 3]);
-      m.center_img = cctagx::Pt2(c0[2 * k], c0[2 * k + 1]);
+      const cctagx_ident_marker& a = in[k];
+      oracle::IdentMarker& o = im[k];
+      o.cx = a.ellipse[0]; o.cy = a.ellipse[1]; o.a = a.ellipse[2]; o.b = a.ellipse[3];
+      o.angle = a.ellipse[4];
+      for (int j = 0; j < a.n_points; ++j)
+        o.points.push_back(oracle::DPt{a.points[4 * j], a.points[4 * j + 1], a.points[4 * j + 2],
+                                       a.points[4 * j + 3]});
+      o.center_x = a.center[0];
+      o.center_y = a.center[1];
+      o.scale = a.scale;
     }
-    cctagx::identify_markers(markers, cctagx::host_params(*params), bank, backend, nullptr);
+    oracle::identify(img, w, h, ident_params(*params), bank, im);
     for (int k = 0; k < n; ++k) {
-      status[k] = markers[k].status;
-      id[k] = markers[k].id;
-      center[2 * k] = markers[k].center_img.x;
-      center[2 * k + 1] = markers[k].center_img.y;
+      const oracle::IdentMarker& o = im[k];
+      cctagx_ident_result& r = out[k];
+      r.status = o.status;
+      r.id = o.id;
+      r.x = o.center_x;
+      r.y = o.center_y;
+      r.quality = o.quality;
+      for (int i = 0; i < 9; ++i) r.homography[i] = o.H[i];
+      r.n_cuts = (int)std::min<size_t>(o.stops.size(), 32);
+      for (int c = 0; c < 32; ++c) {
+        r.stops[c][0] = c < r.n_cuts ? o.stops[c].x : 0.f;
+        r.stops[c][1] = c < r.n_cuts ? o.stops[c].y : 0.f;
+      }
     }
     return 0;
   } catch (const std::exception&) {
     return -1;
   }
+}
+
+// The oracle's closed forms, for the tests that pin them: the homography of
+// an ellipse and an imaged centre, the conditioner and the grid search's
+// iteration count, the sorted and sub-sampled outer points (indices into
+// the input written to idx; returns their count).
+void oracle_homography_from_center(const float* ell, float x, float y, float* H) {
+  oracle::homography_from_center(ell[0], ell[1], ell[2], ell[3], ell[4], x, y, H);
+}
+void oracle_conditioner(float cx, float cy, float a, float b, float* T) {
+  oracle::conditioner(cx, cy, a, b, T);
+}
+int oracle_sample_cut(const uint8_t* img, int w, int h, const float* H, const float* stop,
+                      float begin, float end, int n, int homography, float* sig) {
+  const oracle::DPt p{stop[0], stop[1], 0.f, 0.f};
+  return oracle::sample_cut(img, w, h, H, p, begin, end, n, homography != 0, sig) ? 1 : 0;
+}
+int oracle_cut_distances(int n_crowns, const float* sig, int n, float begin, float end,
+                         float* out) {
+  try {
+    std::vector<float> d;
+    oracle::cut_distances(std::vector<float>(sig, sig + n), begin, end, bank_of(n_crowns), d);
+    std::copy(d.begin(), d.end(), out);
+    return (int)d.size();
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+float oracle_start_signal(int n_crowns) { return oracle::start_signal(n_crowns); }
+int oracle_refine_stop(const uint8_t* img, int w, int h, float* stop, float scale, int n_ref) {
+  oracle::DPt p{stop[0], stop[1], stop[2], stop[3]};
+  if (!oracle::refine_stop(img, w, h, p, scale, n_ref)) return 0;
+  stop[0] = p.x;
+  stop[1] = p.y;
+  return 1;
+}
+int oracle_grid_iterations(float a, float b, float neighbour, int grid_n) {
+  return oracle::grid_iterations(a, b, neighbour, grid_n);
+}
+int oracle_sorted_outer_points(float cx, float cy, const float* pts, int n, int requested,
+                               int* idx) {
+  std::vector<oracle::DPt> in(n), outp;
+  for (int i = 0; i < n; ++i) in[i] = oracle::DPt{pts[2 * i], pts[2 * i + 1], (float)i, 0.f};
+  oracle::sorted_outer_points(cx, cy, in, outp, (size_t)requested);
+  for (size_t k = 0; k < outp.size(); ++k) idx[k] = (int)outp[k].dx;
+  return (int)outp.size();
 }
 
 int oracle_detect(const cctagx_params* params, const uint8_t* img, int w, int h,

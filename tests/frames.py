@@ -2,10 +2,13 @@
 from __future__ import annotations
 
 import functools
+import math
 
 import numpy as np
+from scipy import ndimage
 
 from cctag_amd import synth
+from cctag_amd.bank import bank_ratios
 
 
 @functools.lru_cache(maxsize=None)
@@ -51,3 +54,78 @@ def small_batch():
     for img in out:
         img.setflags(write=False)
     return out
+
+
+@functools.lru_cache(maxsize=None)
+def border_frame(seed: int = 7000):
+    img, gts = synth.border_frame(seed)
+    img.setflags(write=False)
+    return img, gts
+
+
+@functools.lru_cache(maxsize=None)
+def batch_frame(k: int):
+    img, gts = synth.batch_frame(k)
+    img.setflags(write=False)
+    return img, gts
+
+
+# ---------------------------------------------------------------------------
+# identification inputs built from ground truth
+
+def outer_points(e, n=150, mag=100.0):
+    """n points on the ellipse (cx, cy, a, b, angle) with outward normals
+    (x, y, gx, gy)."""
+    cx, cy, a, b, ang = e
+    t = 2 * np.pi * np.arange(n) / n
+    ca, sa = np.cos(ang), np.sin(ang)
+    x = cx + a * np.cos(t) * ca - b * np.sin(t) * sa
+    y = cy + a * np.cos(t) * sa + b * np.sin(t) * ca
+    u, v = (x - cx) * ca + (y - cy) * sa, -(x - cx) * sa + (y - cy) * ca
+    gu, gv = u / a ** 2, v / b ** 2
+    gx, gy = gu * ca - gv * sa, gu * sa + gv * ca
+    nrm = np.hypot(gx, gy)
+    return np.column_stack([x, y, mag * gx / nrm, mag * gy / nrm])
+
+
+def ident_input(H_synth):
+    """The identification input of a synthetic marker: its outer ellipse,
+    150 points on it and the ellipse centre as the starting imaged centre."""
+    m = synth.refine_marker(H_synth)
+    e = m["ellipse"]
+    return {"ellipse": e, "points": outer_points(e), "center0": m["center0"],
+            "center_gt": m["center_gt"]}
+
+
+def clean_marker(seed, w=320, h=240, radius=(40, 80), tilt=40.0, blur=0.8):
+    """One marker rendered without noise: (image, its ID, ident_input)."""
+    rng = np.random.default_rng(seed)
+    mid = int(rng.integers(0, 32))
+    img = np.full((h, w), 128.0)
+    H = synth.marker_homography(w / 2 + rng.uniform(-3, 3), h / 2 + rng.uniform(-3, 3),
+                                rng.uniform(*radius), math.radians(rng.uniform(0, tilt)),
+                                rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi),
+                                2.0 * max(w, h))
+    synth._render_marker(img, H, bank_ratios(3)[mid])
+    img = ndimage.gaussian_filter(img, blur, mode="nearest")
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8), mid, ident_input(H)
+
+
+def faded_sector(img, e, lo=0.2, width=np.pi / 2, contrast=0.63):
+    """img with the sector [lo, lo + width) of angles about the ellipse
+    centre shown at `contrast` of its contrast around 128."""
+    f = img.astype(np.float64)
+    yy, xx = np.mgrid[0:img.shape[0], 0:img.shape[1]]
+    ang = np.arctan2(yy - e[1], xx - e[0])
+    sector = (ang > lo) & (ang < lo + width)
+    f[sector] = 128 + (f[sector] - 128) * contrast
+    return np.clip(np.rint(f), 0, 255).astype(np.uint8)
+
+
+def failing_marker():
+    """A marker input whose cuts are all collected but whose refinement cuts
+    all leave the image, so no cut is selected (status -2)."""
+    ys = np.linspace(280.0, 320.0, 12)
+    return {"ellipse": (6.0, 300.0, 40.0, 40.0, 0.0),
+            "points": np.column_stack([np.full(12, 3.0), ys, np.full(12, -100.0), np.zeros(12)]),
+            "center0": (6.0, 300.0)}

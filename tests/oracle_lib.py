@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 
 from cctag_amd import _lib
+from tests import golden_tables
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -51,7 +52,33 @@ def load():
     o.oracle_linking_mismatches.argtypes = [vp, ctypes.c_int]
     o.oracle_conic_class_mismatches.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp]
     o.oracle_identify_given.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
-                                        ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
+                                        ctypes.POINTER(_lib.IdentMarker), ctypes.c_int,
+                                        ctypes.POINTER(_lib.IdentResult)]
+    o.oracle_set_bank.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
+    o.oracle_homography_from_center.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
+    o.oracle_homography_from_center.restype = None
+    o.oracle_conditioner.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, vp]
+    o.oracle_sample_cut.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_int, ctypes.c_int, vp]
+    o.oracle_conditioner.restype = None
+    o.oracle_grid_iterations.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    o.oracle_sorted_outer_points.argtypes = [ctypes.c_float, ctypes.c_float, vp, ctypes.c_int,
+                                             ctypes.c_int, vp]
+    o.oracle_edges_and_vote.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(vp)]
+    o.oracle_cut_distances.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_float,
+                                       ctypes.c_float, vp]
+    o.oracle_start_signal.argtypes = [ctypes.c_int]
+    o.oracle_start_signal.restype = ctypes.c_float
+    o.oracle_refine_stop.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_float, ctypes.c_int]
+    # the oracle identifies with the reference's own bank tables
+    # (CCTagMarkersBank.cpp:120-282, read as data into the golden tables)
+    for n, key in ((3, "bank3"), (4, "bank4")):
+        b = np.ascontiguousarray(golden_tables.tables()[key], dtype=np.float32)
+        if o.oracle_set_bank(n, b.ctypes.data, b.shape[0], b.shape[1]) != 0:
+            raise RuntimeError("oracle_set_bank failed")
     _o = o
     return o
 
@@ -176,29 +203,115 @@ def linking_mismatches(cases: np.ndarray) -> int:
     return load().oracle_linking_mismatches(c.ctypes.data, c.shape[0])
 
 
-def identify_given(img: np.ndarray, markers, params=None):
-    """The oracle's identification of given markers: each a dict with
-    "ellipse" (cx, cy, a, b, angle), "points" (n x 4: x, y, gx, gy) and
-    "center0".  Returns (status, id, centres n x 2)."""
+def identify_given(img: np.ndarray, markers, params=None) -> list:
+    """The oracle's identification of given markers (cctagx_identify's
+    counterpart): each a dict with "ellipse" (cx, cy, a, b, angle), "points"
+    (n x 4: x, y, gx, gy), "center0" and an optional "scale".  Returns per
+    marker a dict: status, id, x, y, quality, homography, stops."""
     o = load()
     img = np.ascontiguousarray(img, dtype=np.uint8)
     h, w = img.shape
     if params is None:
         params = _lib.default_params()
-    n = len(markers)
-    ell = np.ascontiguousarray([m["ellipse"] for m in markers], dtype=np.float32)
-    npts = np.ascontiguousarray([len(m["points"]) for m in markers], dtype=np.int32)
-    pts = np.ascontiguousarray(np.concatenate([np.asarray(m["points"], np.float32) for m in markers]))
-    c0 = np.ascontiguousarray([m["center0"] for m in markers], dtype=np.float32)
-    status = np.zeros(n, np.int32)
-    ids = np.zeros(n, np.int32)
-    centers = np.zeros((n, 2), np.float32)
-    rc = o.oracle_identify_given(ctypes.byref(params), img.ctypes.data, w, h, n, ell.ctypes.data,
-                                 npts.ctypes.data, pts.ctypes.data, c0.ctypes.data,
-                                 status.ctypes.data, ids.ctypes.data, centers.ctypes.data)
+    arr, keep = _lib.ident_markers(markers)
+    out = (_lib.IdentResult * max(len(markers), 1))()
+    rc = o.oracle_identify_given(ctypes.byref(params), img.ctypes.data, w, h, arr, len(markers), out)
+    del keep
     if rc != 0:
         raise RuntimeError("oracle identification failed")
-    return status, ids, centers
+    return [out[i].as_dict() for i in range(len(markers))]
+
+
+def edges_and_vote(edges: np.ndarray, dx: np.ndarray, dy: np.ndarray, dist_search: int = 30,
+                   thr_grad: int = 2500, ratio_voting: float = 4.0, n_crowns: int = 3,
+                   min_votes: int = 3) -> dict:
+    """Edge points, links, votes and seeds of one level from a given thinned
+    edge map {0, 255} and derivative maps (the level-view dict)."""
+    o = load()
+    e = np.ascontiguousarray(edges, np.uint8)
+    gx = np.ascontiguousarray(dx, np.int16)
+    gy = np.ascontiguousarray(dy, np.int16)
+    h, w = e.shape
+    handle = ctypes.c_void_p()
+    if o.oracle_edges_and_vote(e.ctypes.data, gx.ctypes.data, gy.ctypes.data, w, h, dist_search,
+                               thr_grad, ratio_voting, n_crowns, min_votes,
+                               ctypes.byref(handle)) != 0:
+        raise RuntimeError("oracle edges_and_vote failed")
+    try:
+        v = _lib.LevelView()
+        o.oracle_level_view(handle, 0, ctypes.byref(v))
+        return _lib.level_view_to_dict(v)
+    finally:
+        o.oracle_free(handle)
+
+
+def cut_distances(sig, begin: float = 0.25, end: float = 1.0, n_crowns: int = 3) -> np.ndarray:
+    """orazioDistanceRobust's distance of one cut signal to every bank row."""
+    s = np.ascontiguousarray(sig, np.float32)
+    out = np.zeros(256, np.float32)
+    n = load().oracle_cut_distances(n_crowns, s.ctypes.data, len(s), begin, end, out.ctypes.data)
+    if n < 0:
+        raise RuntimeError("oracle cut distances failed")
+    return out[:n]
+
+
+def start_signal(n_crowns: int) -> float:
+    return load().oracle_start_signal(n_crowns)
+
+
+def refine_stop(img: np.ndarray, stop, scale: float = 1.0, n_ref: int = 20):
+    """outerEdgeRefinement of one stop (x, y, gx, gy) -> refined (x, y), or
+    None when its cut leaves the image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    st = np.ascontiguousarray(stop, np.float32).copy()
+    ok = load().oracle_refine_stop(img.ctypes.data, w, h, st.ctypes.data, scale, n_ref)
+    return (float(st[0]), float(st[1])) if ok else None
+
+
+def homography_from_center(ellipse, x: float, y: float) -> np.ndarray:
+    """The oracle's computeHomographyFromEllipseAndImagedCenter (3 x 3)."""
+    e = np.ascontiguousarray(ellipse, np.float32)
+    H = np.zeros(9, np.float32)
+    load().oracle_homography_from_center(e.ctypes.data, x, y, H.ctypes.data)
+    return H.reshape(3, 3)
+
+
+def conditioner(cx: float, cy: float, a: float, b: float) -> np.ndarray:
+    """conditionerFromEllipse after an ellipse with semi-axes a, b set its scale."""
+    T = np.zeros(9, np.float32)
+    load().oracle_conditioner(cx, cy, a, b, T.ctypes.data)
+    return T.reshape(3, 3)
+
+
+def sample_cut(img: np.ndarray, stop, begin: float, end: float, n: int, H=None, start=None):
+    """One cut's samples and out-of-bounds flag: through the homography H
+    (extractSignalUsingHomography) or, with `start`, straight from start to
+    stop (cutInterpolated)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    if H is not None:
+        m = np.ascontiguousarray(np.asarray(H, np.float32).reshape(9))
+    else:
+        m = np.zeros(9, np.float32)
+        m[:2] = start
+    st = np.ascontiguousarray(stop, np.float32)
+    sig = np.zeros(n, np.float32)
+    oob = load().oracle_sample_cut(img.ctypes.data, w, h, m.ctypes.data, st.ctypes.data, begin, end,
+                                   n, int(H is not None), sig.ctypes.data)
+    return sig, bool(oob)
+
+
+def grid_iterations(a: float, b: float, neighbour: float = 0.2, grid_n: int = 5) -> int:
+    return load().oracle_grid_iterations(a, b, neighbour, grid_n)
+
+
+def sorted_outer_points(cx: float, cy: float, pts, requested: int) -> np.ndarray:
+    """Indices of the points getSortedOuterPoints keeps, in its order."""
+    p = np.ascontiguousarray(np.asarray(pts, np.float32).reshape(-1, 2))
+    idx = np.zeros(max(len(p), 1), np.int32)
+    n = load().oracle_sorted_outer_points(cx, cy, p.ctypes.data, len(p), requested, idx.ctypes.data)
+    return idx[:n]
 
 
 def conic_class_mismatches(conics: np.ndarray, lo: float = 0.04, hi: float = 25.0):

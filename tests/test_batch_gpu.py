@@ -4,7 +4,8 @@ work partition the autotuner can choose, and graph replays repeat."""
 import pytest
 
 from cctag_amd.detector import BatchDetector, Detector
-from tests import frames
+from tests import frames, oracle_lib
+from tests.test_detect_gpu import match
 
 
 def plain(imgs):
@@ -65,3 +66,24 @@ def test_batch_gradient_refinement_matches_plain_gradient_refinement():
             b.load(f, im)
         got = b.detect()
     assert got == ref
+
+
+# frames of the 64-frame uneven-size batch: 640x480 (2, 3), 1280x720 (6),
+# 1920x1080 (7, 8) and 3840x2160 (0, 5)
+UNEVEN = [2, 7, 0, 6, 8, 3, 5]
+
+
+@pytest.mark.gpu
+def test_uneven_batch_matches_oracle_per_frame():
+    """Frames of the uneven-size batch, up to 3840x2160, detected in one
+    batch: each frame's markers against the oracle's for that frame alone."""
+    imgs = [frames.batch_frame(k)[0] for k in UNEVEN]
+    print("uneven batch", [im.shape for im in imgs], flush=True)
+    with BatchDetector([(im.shape[1], im.shape[0]) for im in imgs]) as b:
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+    for f, (k, im) in enumerate(zip(UNEVEN, imgs)):
+        print("oracle on batch frame", k, im.shape, flush=True)
+        match(got[f], oracle_lib.detect(im, threads=8))
+    assert sum(m["status"] == 1 for ms in got for m in ms) >= 20

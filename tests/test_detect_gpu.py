@@ -36,17 +36,21 @@ def gt_errors(markers, gts):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("config", [1, 2])
-def test_detect_matches_oracle(config):
-    img, gts = frames.config_frame(config)
-    print("detect parity config", config, img.shape, flush=True)
-    ref = oracle_lib.detect(img)
+@pytest.mark.parametrize("config,k", [(1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 1), (2, 2),
+                                      (2, 3), (3, 0), (3, 1)])
+def test_detect_matches_oracle(config, k):
+    """Frames k of the 1280x720, 1920x1080 and 3840x2160 configurations
+    (the last with 200 distractors and motion blur)."""
+    img, gts = frames.config_frame(config, k)
+    print("detect parity config", config, "frame", k, img.shape, flush=True)
+    ref = oracle_lib.detect(img, threads=8)
     h, w = img.shape
     with Detector(w, h) as det:
         got = det.detect(img)
     match(got, ref)
     errs, ids_ok = gt_errors(got, gts)
-    assert ids_ok == len(errs) and len(errs) >= len(gts) - 1
+    print("identified", len(errs), "of", len(gts), "max centre error", max(errs), flush=True)
+    assert ids_ok == len(errs) and len(errs) >= len(gts) - (1 if config < 3 else 3)
     assert max(errs) < 0.5
 
 

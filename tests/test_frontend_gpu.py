@@ -39,10 +39,10 @@ def run_parity(img, n_levels=4):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("config", [1, 2])
-def test_front_end_parity_configs(config):
-    img, _ = frames.config_frame(config)
-    print("front-end parity config", config, img.shape, flush=True)
+@pytest.mark.parametrize("config,k", [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1)])
+def test_front_end_parity_configs(config, k):
+    img, _ = frames.config_frame(config, k)
+    print("front-end parity config", config, "frame", k, img.shape, flush=True)
     ref = run_parity(img)
     assert ref[0]["n_edges"] > 1000 and len(ref[0]["seeds"]) > 0
 

@@ -69,7 +69,13 @@ def test_argument_errors_without_device():
     for n in ("cctagx_destroy", "cctagx_stage_count"):
         assert getattr(lib, n)(None) == -1
     n = ctypes.c_int()
-    assert lib.cctagx_detect(None, None, 0, 0, None, 0, ctypes.byref(n)) == -1
+    assert lib.cctagx_detect(None, None, 0, 0, 0, 0, None, 0, ctypes.byref(n)) == -1
+    assert lib.cctagx_load(None, None, 0, 0, 0) == -1
+    assert lib.cctagx_front_end(None, None, 0, 0, 0) == -1
+    assert lib.cctagx_last_markers(None, None, 0, ctypes.byref(n)) == -1
+    bank = np.ones((32, 5), np.float32)
+    assert lib.cctagx_set_bank(None, bank.ctypes.data, 32, 5) == -1
+    assert lib.cctagx_identify(None, None, 0, None) == -1
 
 
 def test_create_without_device_fails_cleanly():
