@@ -51,7 +51,8 @@ def test_detect_matches_oracle(config, k):
     errs, ids_ok = gt_errors(got, gts)
     print("identified", len(errs), "of", len(gts), "max centre error", max(errs), flush=True)
     assert ids_ok == len(errs) and len(errs) >= len(gts) - (1 if config < 3 else 3)
-    assert max(errs) < 0.5
+    # the 3840x2160 frame has up to 7 px of motion blur and 70 degrees of tilt
+    assert max(errs) < (0.5 if config < 3 else 1.5)
 
 
 @pytest.mark.gpu

@@ -15,7 +15,14 @@ from tests import frames, golden_tables, oracle_lib
 
 CENTER_TOL_PX = 0.05   # imaged centre, GPU vs oracle
 STOP_TOL_PX = 1e-5     # selected cut stops (identify_step_1)
-RESIDUAL_RTOL = 1e-4   # normalised residual = 1 / quality
+RESIDUAL_RTOL = 1e-4   # normalised residual = 1 / quality, at the same grid point
+# The grid search compares residuals that the reference sums in float and
+# the kernels in double (the two sums differ by <= 1e-5 relative); where
+# two grid points of an iteration tie to that precision the two searches
+# can end on neighbouring points of the last 5x5 grid (<= 0.02 px apart),
+# whose residuals differ by up to a few 1e-4.
+RESIDUAL_RTOL_NEIGHBOUR = 2e-3
+SAME_POINT_PX = 1e-3
 
 
 def compare(got, ref, where):
@@ -32,7 +39,8 @@ def compare(got, ref, where):
             d = math.hypot(g["x"] - r["x"], g["y"] - r["y"])
             assert d <= CENTER_TOL_PX, (w, "centre", d)
             rel = abs(1 / g["quality"] - 1 / r["quality"]) * r["quality"]
-            assert rel <= RESIDUAL_RTOL, (w, "residual", rel)
+            tol = RESIDUAL_RTOL if d < SAME_POINT_PX else RESIDUAL_RTOL_NEIGHBOUR
+            assert rel <= tol, (w, "residual", rel, "centres", d, "px apart")
 
 
 def identify_both(img, markers, where):
