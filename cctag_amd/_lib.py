@@ -263,6 +263,9 @@ def load() -> ctypes.CDLL:
     lib.cctagx_refiner_refine.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     lib.cctagx_refiner_grid.argtypes = [vp, vp, vp]
     lib.cctagx_refiner_times.argtypes = [vp, P(c_f32), P(c_f32)]
+    lib.cctagx_refine_grid_static.argtypes = [P(Params), vp, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_size_t, P(RefineMarker), ctypes.c_int, vp,
+                                              vp]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int

@@ -325,6 +325,24 @@ int cctagx_refiner_grid(cctagx_refiner* r, float* centers, int32_t* ok);
 /* Device time of the last descend (or refine) and grid calls, microseconds. */
 int cctagx_refiner_times(cctagx_refiner* r, float* descend_us, float* grid_us);
 
+/* No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  cctagx_refiner_set_image + cctagx_refiner_set_markers
+ * + cctagx_refiner_grid in one synchronous call that allocates nothing: the
+ * frame (host memory, at most CCTAGX_STATIC_MAX_PIXELS pixels), its quad
+ * image, up to CCTAGX_STATIC_MAX_MARKERS markers' jobs and the grid search's
+ * state go into one fixed device array of the library (24.3 MB, held by every
+ * GPU the library runs on until the process ends); larger inputs return
+ * CCTAGX_ERR_CAPACITY.  Runs on the current device's null stream and writes
+ * the same centres (n x 2) and ok flags (n) as cctagx_refiner_grid.  All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * race and must be serialized by the caller. */
+#define CCTAGX_STATIC_MAX_PIXELS (1920 * 1088)
+#define CCTAGX_STATIC_MAX_MARKERS 64
+int cctagx_refine_grid_static(const cctagx_params* params, const uint8_t* gray, int width,
+                              int height, size_t stride, const cctagx_refine_marker* markers,
+                              int n, float* centers, int32_t* ok);
+
 /* ---- timing ------------------------------------------------------------
  * Stage times of the last front end in microseconds, measured with HIP
  * events on the library stream; names are static strings.  By default the
