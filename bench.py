@@ -3,16 +3,17 @@
 
 Workload: one seeded synthetic 1920x1080 uint8 frame with 16 three-ring
 markers (outer radius 25-200 px, tilt <= 60 deg, 20 % partially occluded,
-blur 0.5-2 px, noise 6 DN), default parameters, 3 crowns.  The frame is
-loaded into device memory once (cctagx_load); a step is one full
-cctagx_detect_loaded: front end on the GPU, candidate stage on the host,
-identification on the GPU, markers back on the host.
+blur 0.5-2 px, noise 6 DN), default parameters, 3 crowns.  A step is one
+drop-in call, cctagx_detect: the frame from host memory to the device, front
+end on the GPU, candidate stage on the host, identification on the GPU,
+markers back on the host.  The same detection on the frame already resident
+in device memory (cctagx_detect_loaded) is reported beside it.
 
 Prints one JSON line with the throughput, the roofline position of the
 kernel that takes the most device time per frame (durations from HIP events
-recorded around that kernel on the library stream) and the CPU baseline: the
-oracle's single-threaded restatement of the reference CPU path timed on a
-bounded sample of the same frame.
+recorded around that kernel on the library stream) and the CPU baseline:
+the oracle's restatement of the reference CPU path timed on a bounded
+sample of the same frame.
 """
 from __future__ import annotations
 
@@ -32,12 +33,35 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3    # MI355X fp32
 
-# Floating-point operations per bilinear cut sample of the identification
-# kernels: homography (6 mul + 6 add + 2 div), bilinear (4 sub, 8 mul,
-# 3 add, 1 div), step (2 add), per-sample cost terms (3 fp64 per cut).
-FLOP_PER_REFINE_SAMPLE = 14 + 16 + 2 + 3
-# Cut walk of select: bilinear 16, step 2, running mean/variance 10.
+# Cut walk of select: bilinear 16, step 2, running mean/variance 10.  (The
+# grid search's algorithmic FLOPs come from the library: per marker,
+# iteration and grid point 25 per cut sample and 3 per sample of every cut
+# pair, 3.1 MFLOP for 22 cuts of 100 samples.)
 FLOP_PER_SELECT_SAMPLE = 16 + 2 + 10
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def counter_traffic(path: str) -> dict:
+    """Per-kernel HBM bytes per launch from a committed rocprofv3 counter
+    summary (lines `kernel_bytes: <kernel> <fetch> <write>`), or {}."""
+    out = {}
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            if line.startswith("kernel_bytes:"):
+                _, k, f, w = line.split()
+                out[k] = (int(f), int(w))
+    except OSError:
+        pass
+    return out
 
 
 def level_counts(det, n_levels):
@@ -339,17 +363,17 @@ def main():
     h, w = img.shape
     det = Detector(w, h)
     host_threads = det.host_threads()
-    det.load(img)
     markers = []
     for _ in range(args.warmup):
-        markers = det.detect_loaded()
+        markers = det.detect(img)
 
+    # the timed steps: the drop-in call with the frame in host memory
     K = args.steps
     step_markers = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(K):
-        step_markers.append(det.detect_loaded())
+        step_markers.append(det.detect(img))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -358,6 +382,16 @@ def main():
     markers = step_markers[-1]
     assert all(m == markers for m in step_markers), "detection differs between steps"
 
+    # the same detection on the frame resident in device memory
+    det.load(img)
+    det.detect_loaded()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        det.detect_loaded()
+    torch.cuda.synchronize()
+    fps_resident = K / (time.perf_counter() - t0)
+
     # per-kernel device times: a second pass of K steps, reading the HIP
     # event times the library records on its stream after each step
     kern = {}
@@ -365,7 +399,7 @@ def main():
                              "loop_two_us", "labels_us", "loop_three_us", "rescale_us",
                              "ident_us", "finish_us", "ident_prep_us", "ident_select_us",
                              "ident_refine_us", "ident_post_us")}
-    work = {"select_samples": 0, "refine_samples": 0}
+    work = {"select_samples": 0, "refine_samples": 0, "refine_flops": 0}
     det.set_stage_timing(True)
     for _ in range(K):
         det.detect_loaded()
@@ -381,16 +415,7 @@ def main():
     det.set_stage_timing(False)
 
     pipes = None if args.no_pipes else pipes_comparison(img, Detector, markers)
-    progress("concurrent pipes done")
-
-    # the drop-in call with the frame in host memory (upload included)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(K):
-        det.detect(img)
-    torch.cuda.synchronize()
-    fps_upload = K / (time.perf_counter() - t0)
-    progress("per-kernel times and upload-inclusive rate done")
+    progress("per-kernel times and concurrent pipes done")
 
     # accuracy against the synthetic ground truth
     errs, ids_ok = [], 0
@@ -406,9 +431,9 @@ def main():
     dom = max(avg, key=avg.get)
     lv = level_counts(det, 4)
     if dom == "refine":
-        flops = work["refine_samples"] / K * FLOP_PER_REFINE_SAMPLE
+        flops = work["refine_flops"] / K
         achieved = flops / (avg[dom] * 1e-6) / 1e12
-        roof = {"kernel": "k_refine", "bound": "mfma", "achieved": achieved,
+        roof = {"kernel": "k_refine", "bound": "fp32-valu", "achieved": achieved,
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                 "traffic": measured_traffic("profiles/refine_1080p16_pmc.md"),
                 "traffic_source": "profiles/refine_1080p16_pmc.md",
@@ -416,7 +441,7 @@ def main():
     elif dom == "select_cuts":
         flops = work["select_samples"] / K * FLOP_PER_SELECT_SAMPLE
         achieved = flops / (avg[dom] * 1e-6) / 1e12
-        roof = {"kernel": "k_select_cuts", "bound": "mfma", "achieved": achieved,
+        roof = {"kernel": "k_select_cuts", "bound": "fp32-valu", "achieved": achieved,
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                 "traffic": None, "avg_us": avg[dom], "algorithmic_per_launch": flops}
     else:
@@ -426,17 +451,22 @@ def main():
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "avg_us": avg[dom], "algorithmic_per_launch": b}
     kernels = {}
+    measured = counter_traffic("profiles/frontend_1080p16_pmc.md")
     for n, t in sorted(avg.items(), key=lambda kv: -kv[1]):
         e = {"avg_us": round(t, 2)}
         if n not in ("refine", "select_cuts"):
             e["GB/s"] = round(frontend_bytes(n, lv) / (t * 1e-6) / 1e9, 1) if t > 0 else None
+            if "k_" + n in measured and t > 0:
+                f, wr = measured["k_" + n]
+                e["counter_GB/s"] = round((f + wr) / (t * 1e-6) / 1e9, 1)
+                e["counter_bytes"] = f + wr
         kernels["k_" + n] = e
 
     cpu = None
     if not args.no_cpu_baseline:
         from tests import oracle_lib
-        # all host cores (up to 16), as the reference runs its candidate and
-        # identification loops under tbb::parallel_for; one thread beside it
+        # the host's CPU share of one GPU (16 cores), as the reference runs
+        # its candidate loops under tbb::parallel_for; one thread beside it
         threads = max(1, min(16, os.cpu_count() or 1))
         t_serial = oracle_lib.time_detect(img, 1, threads=1)
         t_one = oracle_lib.time_detect(img, 1, threads=threads)
@@ -444,9 +474,10 @@ def main():
         t_avg = oracle_lib.time_detect(img, iters, threads=threads)
         cpu = {"value": 1.0 / t_avg, "unit": "frames/s", "cores": threads, "kind": "port",
                "sample": "%d+1 full detections (front end, candidates, identification) of the "
-                         "same 1920x1080 frame on %d threads (front end serial, candidate "
-                         "stage and per-candidate identification parallel)" % (iters, threads),
-               "single_thread_value": 1.0 / t_serial}
+                         "same 1920x1080 frame on %d threads (front end and identification "
+                         "serial, candidate stage parallel)" % (iters, threads),
+               "single_thread_value": 1.0 / t_serial,
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
         progress("CPU baseline done: %.2f frames/s on %d threads" % (cpu["value"], threads))
 
     batched = batched_comparison(side.get("batch") or [], Detector, BatchDetector)
@@ -471,13 +502,13 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": "plain 1920x1080 uint8 frame, 16 three-ring markers, "
-                               "4-level pyramid, default parameters, frame resident in HBM",
+                               "4-level pyramid, default parameters, frame uploaded each step",
                    "width": w, "height": h, "markers": len(gts), "pipes": 1,
                    "host_threads": host_threads},
         "roofline": roof,
         "cpu_baseline": cpu,
         "vs_cpu": (fps / cpu["value"]) if cpu else None,
-        "with_upload_frames_per_s": fps_upload,
+        "resident_frames_per_s": fps_resident,
         "accuracy": {"identified": len(errs), "ids_correct": ids_ok, "gt_markers": len(gts),
                      "center_err_px_mean": (sum(errs) / len(errs)) if errs else None,
                      "center_err_px_max": max(errs) if errs else None},

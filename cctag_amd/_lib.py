@@ -140,6 +140,7 @@ class DetectStats(ctypes.Structure):
         ("select_us", c_f32), ("refine_us", c_f32),
         ("n_select", c_i32), ("n_refine", c_i32),
         ("select_samples", ctypes.c_int64), ("refine_samples", ctypes.c_int64),
+        ("refine_flops", ctypes.c_int64),
         ("load_us", c_f32), ("loop_one_us", c_f32), ("loop_two_us", c_f32), ("labels_us", c_f32),
         ("loop_three_us", c_f32), ("rescale_us", c_f32), ("ident_us", c_f32),
         ("finish_us", c_f32), ("ident_prep_us", c_f32), ("ident_select_us", c_f32),

@@ -379,6 +379,7 @@ int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
   st->n_refine = h->ident.last_refine_jobs();
   st->select_samples = h->ident.last_select_samples();
   st->refine_samples = h->ident.last_refine_samples();
+  st->refine_flops = h->ident.last_refine_flops();
   const HostTimes& t = h->host_times;
   st->load_us = (float)(t.load * 1e3);
   st->loop_one_us = (float)(t.loop_one * 1e3);

@@ -9,6 +9,12 @@
 
 namespace cctagx {
 
+long long refine_flops(long long n_iter, long long grid_points, long long cuts, long long samples) {
+  // per grid point: 25 FLOP per bilinear cut sample (homography, sampling,
+  // step) and 3 per sample of every cut pair's difference (sub, mul, add)
+  return n_iter * grid_points * (25 * cuts * samples + 3 * samples * cuts * (cuts - 1) / 2);
+}
+
 namespace {
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -293,9 +299,12 @@ void GpuIdentBackend::refine(const std::vector<RefineJob>& jobs, const HostParam
   ensure(n, 0);
   refine_jobs_ = n;
   refine_samples_ = 0;
+  refine_flops_ = 0;
   const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
-  for (const RefineJob& r : jobs)
+  for (const RefineJob& r : jobs) {
     refine_samples_ += (long long)r.n_iter * g2 * (long long)r.stops.size() * k_.n_samples;
+    refine_flops_ += refine_flops(r.n_iter, g2, (long long)r.stops.size(), k_.n_samples);
+  }
   pack_refine_jobs(jobs);
   hipStream_t s = stream_;
   hip_check(hipMemcpyAsync(d_ref_jobs_, h_ref_jobs_.as<RefineJobDev>(), sizeof(RefineJobDev) * n,
@@ -368,11 +377,15 @@ void GpuIdentBackend::select_and_refine(const std::vector<CutJob>& cuts,
   unpack_select(nc, sel);
   refine_jobs_ = n;
   refine_samples_ = 0;
+  refine_flops_ = 0;
   const long long g2 = (long long)p.imaged_center_n_grid_sample * p.imaged_center_n_grid_sample;
   for (int r = 0; r < n; ++r) {
     jobs[r].stops = sel[sel_of[r]].stops;
     if (!jobs[r].stops.empty())
+    {
       refine_samples_ += (long long)jobs[r].n_iter * g2 * (long long)jobs[r].stops.size() * k_.n_samples;
+      refine_flops_ += refine_flops(jobs[r].n_iter, g2, (long long)jobs[r].stops.size(), k_.n_samples);
+    }
   }
   unpack_refine(jobs, out);
 }

@@ -98,6 +98,12 @@ int launch_refine(const RefineJobDev* jobs, int n_jobs, int max_iter, int n_samp
                   const float* inv_ratio, int bank_rows, int bank_cols, const RefineWork& work,
                   RefineOutDev* out, float* dist, hipStream_t s);
 
+// Algorithmic FLOPs of the grid search on one marker: n_iter iterations of
+// grid_points homographies, each sampling `cuts` cuts of `samples` samples
+// and differencing every cut pair (3.1 MFLOP per iteration for 22 cuts of
+// 100 samples on a 5x5 grid).
+long long refine_flops(long long n_iter, long long grid_points, long long cuts, long long samples);
+
 // A device image the identification jobs sample, as quads: word (x, y)
 // holds pixels (x, y), (x+1, y), (x, y+1), (x+1, y+1) in bytes 0..3.
 struct DeviceImage {
@@ -141,10 +147,12 @@ class GpuIdentBackend : public IdentBackend {
   int last_refine_jobs() const { return refine_jobs_; }
   long long last_select_samples() const { return select_samples_; }
   long long last_refine_samples() const { return refine_samples_; }
+  // Algorithmic FLOPs of the last grid search (refine_flops over its jobs).
+  long long last_refine_flops() const { return refine_flops_; }
   void clear_stats() {
     select_us_ = refine_us_ = 0;
     select_jobs_ = refine_jobs_ = 0;
-    select_samples_ = refine_samples_ = 0;
+    select_samples_ = refine_samples_ = refine_flops_ = 0;
   }
 
  private:
@@ -174,7 +182,7 @@ class GpuIdentBackend : public IdentBackend {
   DeviceBuffer d_grad_jobs_, d_coarse_, d_gcenters_, d_gcost_, d_gH_;
   float select_us_ = 0, refine_us_ = 0;
   int select_jobs_ = 0, refine_jobs_ = 0;
-  long long select_samples_ = 0, refine_samples_ = 0;
+  long long select_samples_ = 0, refine_samples_ = 0, refine_flops_ = 0;
 };
 
 }  // namespace cctagx

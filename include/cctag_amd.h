@@ -224,6 +224,9 @@ typedef struct cctagx_detect_stats {
   int32_t n_refine;
   int64_t select_samples;
   int64_t refine_samples;
+  /* algorithmic FLOPs of the grid search: per marker, iteration and grid
+   * point 25 per cut sample and 3 per sample of every cut pair */
+  int64_t refine_flops;
   /* host wall time of the stages inside host_us: edge sets, seed linking
    * (loop one), flow components (loop two), segment labels, candidate
    * checks (loop three), level-0 re-fit, identification (including its two
