@@ -267,6 +267,14 @@ def load() -> ctypes.CDLL:
     lib.cctagx_refine_grid_static.argtypes = [P(Params), vp, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_size_t, P(RefineMarker), ctypes.c_int, vp,
                                               vp]
+    lib.cctagx_params_parse_xml.argtypes = [ctypes.c_char_p, P(Params)]
+    lib.cctagx_params_read_xml.argtypes = [ctypes.c_char_p, P(Params)]
+    lib.cctagx_params_write_xml.argtypes = [P(Params), ctypes.c_char_p, ctypes.c_size_t,
+                                            P(ctypes.c_size_t)]
+    lib.cctagx_bank_parse.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int,
+                                      P(ctypes.c_int), P(ctypes.c_int)]
+    lib.cctagx_bank_read.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int,
+                                     P(ctypes.c_int), P(ctypes.c_int)]
     lib.cctagx_version.restype = ctypes.c_char_p
     lib.cctagx_last_error.restype = ctypes.c_char_p
     lib.cctagx_device_count.restype = ctypes.c_int
@@ -284,3 +292,43 @@ def default_params(n_crowns: int = 3) -> Params:
     p = Params()
     load().cctagx_params_default(ctypes.byref(p), n_crowns)
     return p
+
+
+def params_from_xml(text: str | None = None, path: str | None = None) -> Params:
+    """cctag::Parameters from its XML archive (the `parameterFile` of
+    cctagDetection): the archive text or a file path."""
+    p = Params()
+    lib = load()
+    if path is not None:
+        check(lib.cctagx_params_read_xml(path.encode(), ctypes.byref(p)), "cctagx_params_read_xml")
+    else:
+        check(lib.cctagx_params_parse_xml(text.encode(), ctypes.byref(p)), "cctagx_params_parse_xml")
+    return p
+
+
+def params_to_xml(p: Params) -> str:
+    """The XML archive text of p (Parameters::serialize's members)."""
+    lib = load()
+    n = ctypes.c_size_t()
+    check(lib.cctagx_params_write_xml(ctypes.byref(p), None, 0, ctypes.byref(n)),
+          "cctagx_params_write_xml")
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(lib.cctagx_params_write_xml(ctypes.byref(p), buf, n.value + 1, ctypes.byref(n)),
+          "cctagx_params_write_xml")
+    return buf.value.decode()
+
+
+def bank_from_text(text: str | None = None, path: str | None = None) -> np.ndarray:
+    """A marker bank (rows of radius ratios) from its text format
+    (CCTagMarkersBank::read): the text or a file path."""
+    lib = load()
+    rows, cols = ctypes.c_int(), ctypes.c_int()
+    if path is not None:
+        fn, arg, what = lib.cctagx_bank_read, path.encode(), "cctagx_bank_read"
+    else:
+        fn, arg, what = lib.cctagx_bank_parse, text.encode(), "cctagx_bank_parse"
+    check(fn(arg, None, 0, 0, ctypes.byref(rows), ctypes.byref(cols)), what)
+    out = np.zeros((rows.value, cols.value), np.float32)
+    check(fn(arg, out.ctypes.data, rows.value, cols.value, ctypes.byref(rows), ctypes.byref(cols)),
+          what)
+    return out

@@ -62,6 +62,32 @@ typedef struct cctagx_params {
 /* cctag::Parameters::Parameters(nCrowns), Params.cpp:38-76. */
 void cctagx_params_default(cctagx_params* p, int n_crowns);
 
+/* ---- file formats (host only, no device needed) --------------------------
+ * The Parameters XML archive that cctagDetection's `parameterFile` names
+ * (boost::archive::xml_iarchive of Parameters::serialize, Params.hpp:210-248,
+ * root element "CCTagsParams", ICCTag.cpp:47-58): every serialized member
+ * must be present, as the archive requires; members without a field here
+ * (_cutsSelectionTrials, _useLMDif, _writeOutput, _maxEdges, _useCuda,
+ * _pinned*) are read and ignored.  A missing file returns CCTAGX_ERR_ARG
+ * where the reference prints to stderr and returns (ICCTag.cpp:51-54). */
+int cctagx_params_parse_xml(const char* text, cctagx_params* out);
+int cctagx_params_read_xml(const char* path, cctagx_params* out);
+/* The archive text of *p (the members without a field get the reference's
+ * defaults); *len = its length, written to buf (NUL-terminated) when cap
+ * allows, else CCTAGX_ERR_CAPACITY (buf == NULL: length only). */
+int cctagx_params_write_xml(const cctagx_params* p, char* buf, size_t cap, size_t* len);
+/* A marker bank text (CCTagMarkersBank::read, CCTagMarkersBank.cpp:60-78):
+ * one row per non-empty line, numbers written as "p/q" or decimals; a line
+ * stops at the first text that is neither.  *rows, *cols = the bank's size;
+ * the ratios are written row-major when ratios != NULL and the bank fits
+ * max_rows x max_cols (else CCTAGX_ERR_CAPACITY).  Rows of different lengths:
+ * CCTAGX_ERR_UNSUPPORTED.  An unreadable file: CCTAGX_ERR_ARG (the reference
+ * throws std::ios_base::failure). */
+int cctagx_bank_parse(const char* text, float* ratios, int max_rows, int max_cols, int* rows,
+                      int* cols);
+int cctagx_bank_read(const char* path, float* ratios, int max_rows, int max_cols, int* rows,
+                     int* cols);
+
 typedef struct cctagx_handle cctagx_handle;
 
 /* Replaces TagPipe::initialize(w, h) (cuda/tag.cu:76-127) and the per-pipe
