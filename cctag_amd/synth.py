@@ -54,6 +54,7 @@ class FrameConfig:
     distractor_radius: tuple = (5, 300)
     motion_blur: float = 0.0
     perlin_amp: float = 20.0
+    n_crowns: int = 3
 
 
 CONFIGS = {
@@ -215,7 +216,7 @@ def make_frame(cfg: FrameConfig, seed: int, n_markers: int | None = None):
     rng = np.random.Generator(np.random.PCG64(seed))
     w, h = cfg.width, cfg.height
     img = 128.0 + cfg.perlin_amp * _perlin(h, w, 256, rng)
-    ratios = bank_ratios(3)
+    ratios = bank_ratios(cfg.n_crowns)
     if n_markers is None:
         n_markers = int(rng.integers(cfg.n_markers[0], cfg.n_markers[1] + 1))
     nd = int(rng.integers(cfg.distractors[0], cfg.distractors[1] + 1))
@@ -292,6 +293,15 @@ def border_frame(seed: int = 7000, w: int = 640, h: int = 480):
     img = ndimage.gaussian_filter(img, 1.0, mode="nearest")
     img = img + rng.normal(0.0, 4.0, size=img.shape)
     return np.clip(np.rint(img), 0, 255).astype(np.uint8), gts
+
+
+FOUR_CROWNS = FrameConfig(1280, 720, (6, 6), (40, 120), 45.0, (1.0, 1.0), 4.0, n_crowns=4)
+
+
+def four_crown_frame(k: int = 0):
+    """A 1280x720 frame with 6 four-ring markers (IDs from the 128-row
+    bank), otherwise as the 1280x720 configuration; seed 6000 + k."""
+    return make_frame(FOUR_CROWNS, 6000 + k)
 
 
 def config_frame(config: int, k: int = 0):

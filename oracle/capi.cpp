@@ -219,6 +219,20 @@ void identify_markers(std::vector<cctagx::Marker>& markers, const uint8_t* img, 
 
 extern "C" {
 
+// The front end with the given parameters (a handle for oracle_level_view).
+int oracle_front_end_params(const cctagx_params* params, const uint8_t* img, int w, int h,
+                            void** out) {
+  try {
+    auto* r = new OracleRun();
+    r->p = front_params(*params);
+    oracle::front_end(img, w, h, r->p, r->levels);
+    *out = r;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 // The bank of n_crowns rings (rows x cols radius ratios, row-major) the
 // oracle identifies with.
 int oracle_set_bank(int n_crowns, const float* ratios, int rows, int cols) {

@@ -65,6 +65,8 @@ def load():
     o.oracle_grid_iterations.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     o.oracle_sorted_outer_points.argtypes = [ctypes.c_float, ctypes.c_float, vp, ctypes.c_int,
                                              ctypes.c_int, vp]
+    o.oracle_front_end_params.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int,
+                                          ctypes.c_int, ctypes.POINTER(vp)]
     o.oracle_edges_and_vote.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(vp)]
@@ -95,6 +97,26 @@ def front_end(img: np.ndarray, n_levels: int = 4) -> list:
     try:
         out = []
         for l in range(n_levels):
+            v = _lib.LevelView()
+            o.oracle_level_view(handle, l, ctypes.byref(v))
+            out.append(_lib.level_view_to_dict(v))
+        return out
+    finally:
+        o.oracle_free(handle)
+
+
+def front_end_params(img: np.ndarray, params) -> list:
+    """All levels of the oracle front end with the given parameters."""
+    o = load()
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape
+    handle = ctypes.c_void_p()
+    if o.oracle_front_end_params(ctypes.byref(params), img.ctypes.data, w, h,
+                                 ctypes.byref(handle)) != 0:
+        raise RuntimeError("oracle front end failed")
+    try:
+        out = []
+        for l in range(params.number_of_processed_multires_layers):
             v = _lib.LevelView()
             o.oracle_level_view(handle, l, ctypes.byref(v))
             out.append(_lib.level_view_to_dict(v))

@@ -125,3 +125,20 @@ def test_gradient_refinement_mode(config):
     print("centre error vs ground truth: grid", np.mean(e_grid), "gradient", np.mean(e_grad), flush=True)
     assert ok_grad == ok_grid == len(e_grid)
     assert np.mean(e_grad) <= np.mean(e_grid) + 0.02
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_detect_four_crowns_matches_oracle(k):
+    """1280x720 frames of six four-ring markers (128-row bank), nCrowns 4."""
+    from cctag_amd import _lib, synth
+    img, gts = synth.four_crown_frame(k)
+    print("detect parity four crowns frame", k, flush=True)
+    prm = _lib.default_params(4)
+    ref = oracle_lib.detect(img, params=prm, threads=8)
+    h, w = img.shape
+    with Detector(w, h, params=_lib.default_params(4)) as det:
+        got = det.detect(img)
+    match(got, ref)
+    errs, ids_ok = gt_errors(got, gts)
+    assert ids_ok == len(errs) and len(errs) >= len(gts) - 1

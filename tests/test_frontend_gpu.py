@@ -86,3 +86,18 @@ def test_front_end_repeatable():
         det.front_end(img)
         for l in range(4):
             compare_levels(det.level(l), a[l], "rerun level %d" % l)
+
+
+@pytest.mark.gpu
+def test_front_end_parity_four_crowns():
+    """The vote walks four crowns (nCrowns 4) on a four-ring marker frame."""
+    from cctag_amd import _lib, synth
+    img, _ = synth.four_crown_frame(0)
+    prm = _lib.default_params(4)
+    ref = oracle_lib.front_end_params(img, prm)
+    h, w = img.shape
+    with Detector(w, h, params=prm) as det:
+        det.front_end(img)
+        for l in range(4):
+            compare_levels(det.level(l), ref[l], "four crowns level %d" % l)
+    assert len(ref[0]["seeds"]) > 0

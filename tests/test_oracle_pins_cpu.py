@@ -327,3 +327,17 @@ def test_border_markers_get_a_valid_id_or_a_failure_status():
         for g in gts:
             near = [m for m in ok if math.hypot(m["x"] - g.center[0], m["y"] - g.center[1]) < 2.0]
             assert near and near[0]["id"] == g.id, (seed, g)
+
+
+def test_four_crown_markers_are_identified():
+    """Four-ring markers (the 128-row bank, CCTagMarkersBank.cpp:154-282, cuts
+    from 0.26 of the radius): the oracle finds every marker of two frames
+    with its true ID."""
+    from cctag_amd import _lib
+    for k in (0, 1):
+        img, gts = synth.four_crown_frame(k)
+        ms = oracle_lib.detect(img, params=_lib.default_params(4))
+        ok = [m for m in ms if m["status"] == 1]
+        for g in gts:
+            near = [m for m in ok if math.hypot(m["x"] - g.center[0], m["y"] - g.center[1]) < 1.0]
+            assert near and near[0]["id"] == g.id, (k, g.id)
