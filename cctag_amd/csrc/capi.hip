@@ -440,7 +440,6 @@ int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* v) {
     rc |= d2h(L.dx, b.dx + d.pix, N, s);
     rc |= d2h(L.dy, b.dy + d.pix, N, s);
     rc |= d2h(L.edges, b.edges + d.pix, N, s);
-    rc |= d2h(L.emap, b.emap + d.pix, N, s);
     rc |= d2h(L.ex, b.ex + d.pix, n, s);
     rc |= d2h(L.ey, b.ey + d.pix, n, s);
     rc |= d2h(L.egx, b.egx + d.pix, n, s);
@@ -456,6 +455,10 @@ int cctagx_get_level(cctagx_handle* h, int level, cctagx_level_view* v) {
     rc |= d2h(L.seeds, b.seeds + d.pix, (size_t)c.n_seeds, s);
     if (rc) return CCTAGX_ERR_HIP;
     CCTAGX_HIP(hipStreamSynchronize(s));
+    // the index map of the view: raster rank of every edge pixel, -1 elsewhere
+    L.emap.resize(N);
+    int32_t next = 0;
+    for (size_t i = 0; i < N; ++i) L.emap[i] = L.edges[i] == 255 ? next++ : -1;
     h->level_loaded[level] = true;
   }
   v->width = L.w; v->height = L.h; v->shape_w = L.W0; v->shape_h = L.H0;

@@ -80,8 +80,8 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t o_rs = arena_.reserve(N);
   const size_t o_edges = arena_.reserve(N);
   const size_t o_tmp = arena_.reserve(N);
-  const size_t o_emap = arena_.reserve(4 * N);
   const size_t o_ebits = arena_.reserve(N / 8 + 4);
+  const size_t o_erank = arena_.reserve(N / 8 + 4);
   const size_t o_tcount = arena_.reserve(4 * (size_t)std::max(ps.n_tiles_c, 1));
   const size_t o_ex = arena_.reserve(2 * N);
   const size_t o_ey = arena_.reserve(2 * N);
@@ -114,8 +114,8 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   b.root_strong = arena_.at<uint8_t>(o_rs);
   b.edges = arena_.at<uint8_t>(o_edges);
   b.tmp = arena_.at<uint8_t>(o_tmp);
-  b.emap = arena_.at<int32_t>(o_emap);
   b.ebits = arena_.at<uint32_t>(o_ebits);
+  b.erank = arena_.at<int32_t>(o_erank);
   b.tcount = arena_.at<int32_t>(o_tcount);
   b.ex = arena_.at<int16_t>(o_ex);
   b.ey = arena_.at<int16_t>(o_ey);

@@ -640,10 +640,13 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const PlaneDesc* __restri
   if (threadIdx.x == 0) counts[p].n_edges = carry;
 }
 
+// Writes the edge list and, per 32-pixel mask word, the index of its first
+// edge pixel (a point's index is then that plus the edges before it in the
+// word).
 __global__ __launch_bounds__(256) void k_compact_write(
     const PlaneDesc* __restrict__ P, int np, const uint8_t* __restrict__ edges,
     const int16_t* __restrict__ dx, const int16_t* __restrict__ dy,
-    const int32_t* __restrict__ tcount, int32_t* __restrict__ emap, int16_t* __restrict__ ex,
+    const int32_t* __restrict__ tcount, int32_t* __restrict__ erank, int16_t* __restrict__ ex,
     int16_t* __restrict__ ey, int16_t* __restrict__ egx, int16_t* __restrict__ egy,
     int32_t* __restrict__ vcount, int32_t* __restrict__ seedmark) {
   const int p = plane_of<F_TILE_C>(P, np, blockIdx.x);
@@ -660,12 +663,14 @@ __global__ __launch_bounds__(256) void k_compact_write(
   }
   int total;
   int idx = tcount[blockIdx.x] + block_exclusive_scan(cnt, &total);
+  // even lanes start a 32-pixel word
+  if ((threadIdx.x & 1) == 0 && b < n) erank[(d.pix >> 5) + (b >> 5)] = idx;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int li = b + k;
     if (li >= n) break;
-    const int g = d.pix + li;
     if (e[k] == 255) {
+      const int g = d.pix + li;
       const int y = li / d.w, x = li - y * d.w;
       const int s = d.pix + idx;
       ex[s] = (int16_t)x;
@@ -674,9 +679,7 @@ __global__ __launch_bounds__(256) void k_compact_write(
       egy[s] = dy[g];
       vcount[s] = 0;
       seedmark[s] = -1;
-      emap[g] = idx++;
-    } else {
-      emap[g] = -1;
+      ++idx;
     }
   }
 }
@@ -684,12 +687,26 @@ __global__ __launch_bounds__(256) void k_compact_write(
 // ---------------------------------------------------------------------------
 // a8: gradientDirectionDescent (Bresenham.cpp:18-209).  One lane per
 // (edge point, direction).  Bounds are the collection shape (the level-0
-// size); lookups beyond the level size find nothing.
+// size); lookups beyond the level size find nothing.  The walk tests the
+// plane's 1-bit edge mask (1/32 of a word per pixel: the whole pyramid's,
+// 344 KB at 1920x1080, stays in L2); at the edge it stops on, the point's
+// index is the index of the mask word's first edge (a second table of one
+// word per 32 pixels) plus the edges before it in the word.
 struct EdgeLookup {
-  const int32_t* emap;
-  int pix, w, h;
+  const uint32_t* bits;  // the plane's mask words (plane starts are 256-aligned)
+  const int32_t* rank;   // the plane's first index per mask word
+  int w, h;
+  __device__ __forceinline__ bool edge(int lin) const {
+    return (bits[lin >> 5] >> (lin & 31)) & 1u;
+  }
+  // index of the edge point at linear position lin (an edge)
+  __device__ __forceinline__ int index(int lin) const {
+    return rank[lin >> 5] + __builtin_popcount(bits[lin >> 5] & ((1u << (lin & 31)) - 1u));
+  }
   __device__ __forceinline__ int at(int x, int y) const {
-    return (x < w && y < h) ? emap[pix + y * w + x] : -1;
+    if (x >= w || y >= h) return -1;
+    const int lin = y * w + x;
+    return edge(lin) ? index(lin) : -1;
   }
 };
 
@@ -734,18 +751,22 @@ __device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float 
     return -1;
   }
   // Later steps move along fixed (dx, dy): the positions follow from
-  // (x, y, e) alone, so kDescentBatch steps are laid out first, their map
-  // reads issued together (a read outside the level reads its first pixel
-  // and is ignored), then checked in step order as one step at a time would.
+  // (x, y, e) alone, so kDescentBatch steps are laid out first and their
+  // mask reads issued together (a read outside the level reads its first
+  // word and is ignored).  Slot 2j is step j's point, slot 2j+1 the point
+  // beside it; the walk ends at the first slot that is an edge or that a
+  // one-step-at-a-time walk would not reach (past nmax or off the image),
+  // found with bit masks so that no read waits on another.
   const float a = ymajor ? fabsf(__fdiv_rn(dx, dy)) : fabsf(__fdiv_rn(dy, dx));
   sx = (dx > 0.0f) - (dx < 0.0f);
   sy = (dy > 0.0f) - (dy < 0.0f);
+  static_assert(2 * kDescentBatch < 32, "slot masks are 32-bit");
   while (n <= nmax) {
-    int r0[kDescentBatch], r1[kDescentBatch];
-    bool in0[kDescentBatch], in1[kDescentBatch], live[kDescentBatch];
+    int l0[kDescentBatch], l1[kDescentBatch];
+    uint32_t hit = 0, reach = ~0u << (2 * kDescentBatch);  // bits above the slots: reached
 #pragma unroll
     for (int j = 0; j < kDescentBatch; ++j) {
-      live[j] = n + j <= nmax;
+      const bool live = n + j <= nmax;
       e = __fadd_rn(e, a);
       if (ymajor) {
         y += sy;
@@ -761,22 +782,31 @@ __device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float 
         }
       }
       const int xs = ymajor ? x : x - sx, ys = ymajor ? y - sy : y;
-      in0[j] = x >= 0 && x < W && y >= 0 && y < H;
-      in1[j] = xs >= 0 && xs < W && ys >= 0 && ys < H;
-      const bool m0 = in0[j] && x < L.w && y < L.h, m1 = in1[j] && xs < L.w && ys < L.h;
-      const int v0 = L.emap[L.pix + (m0 ? y * L.w + x : 0)];
-      const int v1 = L.emap[L.pix + (m1 ? ys * L.w + xs : 0)];
-      r0[j] = m0 ? v0 : -1;
-      r1[j] = m1 ? v1 : -1;
+      const bool in0 = x >= 0 && x < W && y >= 0 && y < H;
+      const bool in1 = xs >= 0 && xs < W && ys >= 0 && ys < H;
+      const bool m0 = in0 && x < L.w && y < L.h, m1 = in1 && xs < L.w && ys < L.h;
+      l0[j] = m0 ? y * L.w + x : 0;
+      l1[j] = m1 ? ys * L.w + xs : 0;
+      const uint32_t w0 = L.bits[l0[j] >> 5], w1 = L.bits[l1[j] >> 5];
+      hit |= ((uint32_t)m0 & (w0 >> (l0[j] & 31))) << (2 * j);
+      hit |= ((uint32_t)m1 & (w1 >> (l1[j] & 31))) << (2 * j + 1);
+      reach |= (uint32_t)(live && in0) << (2 * j);
+      reach |= (uint32_t)(live && in0 && in1) << (2 * j + 1);
     }
+    // slots reached in order: the run of ones at the bottom of reach
+    const uint32_t upto = ~reach & (reach + 1u);  // lowest slot not reached (0 if all are)
+    const uint32_t found = hit & reach & (upto - 1u);
+    if (found) {
+      const uint32_t first = found & (0u - found);
+      int lin = 0;
 #pragma unroll
-    for (int j = 0; j < kDescentBatch; ++j) {
-      if (!live[j]) return -1;
-      if (!in0[j]) return -1;
-      if (r0[j] >= 0) return r0[j];
-      if (!in1[j]) return -1;
-      if (r1[j] >= 0) return r1[j];
+      for (int j = 0; j < kDescentBatch; ++j) {
+        lin = (first >> (2 * j)) & 1u ? l0[j] : lin;
+        lin = (first >> (2 * j + 1)) & 1u ? l1[j] : lin;
+      }
+      return L.index(lin);
     }
+    if (upto != 0u) return -1;
     n += kDescentBatch;
   }
   return -1;
@@ -784,7 +814,8 @@ __device__ int descent(const EdgeLookup& L, int W, int H, int px, int py, float 
 
 __global__ __launch_bounds__(256) void k_descent(const PlaneDesc* __restrict__ P,
                                                   const PlaneCounts* __restrict__ counts,
-                                                  const int32_t* __restrict__ emap,
+                                                  const uint32_t* __restrict__ ebits,
+                                                  const int32_t* __restrict__ erank,
                                                   const int16_t* __restrict__ ex,
                                                   const int16_t* __restrict__ ey,
                                                   const int16_t* __restrict__ egx,
@@ -793,7 +824,7 @@ __global__ __launch_bounds__(256) void k_descent(const PlaneDesc* __restrict__ P
                                                   int32_t* __restrict__ after) {
   const PlaneDesc d = P[blockIdx.y];
   const int n2 = 2 * counts[blockIdx.y].n_edges;
-  const EdgeLookup L{emap, d.pix, d.w, d.h};
+  const EdgeLookup L{ebits + (d.pix >> 5), erank + (d.pix >> 5), d.w, d.h};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += gridDim.x * blockDim.x) {
     const int i = k >> 1, s = d.pix + i;
     const int dir = (k & 1) ? 1 : -1;
@@ -1171,11 +1202,11 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   mark("compact_count");
   k_compact_scan<<<np, 1024, 0, s>>>(P, np, b.tcount, b.counts);
   mark("compact_scan");
-  k_compact_write<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.dx, b.dy, b.tcount, b.emap,
+  k_compact_write<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.dx, b.dy, b.tcount, b.erank,
                                                b.ex, b.ey, b.egx, b.egy, b.vcount, b.seedmark);
   mark("compact_write");
   const dim3 g(L.edge_blocks, np);
-  k_descent<<<g, 256, 0, s>>>(P, b.counts, b.emap, b.ex, b.ey, b.egx, b.egy,
+  k_descent<<<g, 256, 0, s>>>(P, b.counts, b.ebits, b.erank, b.ex, b.ey, b.egx, b.egy,
                               L.prm.dist_search, L.prm.thr_grad, b.before, b.after);
   mark("descent");
   k_vote<<<g, 256, 0, s>>>(P, b.counts, b.ex, b.ey, b.egx, b.egy, b.before, b.after,
