@@ -159,7 +159,7 @@ int cctagx_batch_create(const cctagx_params* params, int n_frames, const int32_t
   }
   PlaneSet ps;
   if (!plan_planes(b->sizes, b->n_levels, ps)) {
-    set_error("every frame size must halve exactly at every pyramid level");
+    set_error("a frame is too small for the pyramid: a level would be empty");
     return CCTAGX_ERR_UNSUPPORTED;
   }
   CCTAGX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));

@@ -136,7 +136,7 @@ int cctagx_create(const cctagx_params* params, int width, int height, cctagx_han
   h->height = height;
   PlaneSet ps;
   if (!plan_planes({{width, height}}, params->number_of_processed_multires_layers, ps)) {
-    set_error("frame size must halve exactly at every pyramid level");
+    set_error("frame too small for the pyramid: a level would be empty");
     return CCTAGX_ERR_UNSUPPORTED;
   }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)

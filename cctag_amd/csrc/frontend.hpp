@@ -27,6 +27,7 @@ struct PlaneDesc {
   int32_t tile_s;           // first block of the stencil grids (64x16 tiles)
   int32_t tile_c;           // first block of the compaction grid (4096-px chunks)
   int32_t tile_p;           // first block of the pyramid / quad grid (level-0 planes only)
+  int32_t tile_r;           // first block of its level's resampling grid (levels >= 1)
 };
 
 // Per-plane counters written by the GPU.
@@ -50,6 +51,7 @@ constexpr int kFilterTileHs[kFilterVariants] = {16, 32, 64};
 constexpr int kStencilTileW = 64, kStencilTileH = 16;
 constexpr int kChunk = 4096;        // compaction chunk, pixels
 constexpr int kPyrTile = 64;        // pyramid tile at level 0 (-> 32, 16, 8)
+constexpr int kResTileW = 64, kResTileH = 16;  // resampling tile (one level from its parent)
 
 struct FrontEndParams {
   int32_t low, high;        // integer Canny thresholds (floor of thr * 256)

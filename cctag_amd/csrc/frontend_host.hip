@@ -21,6 +21,8 @@ bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_leve
                  PlaneSet& out) {
   out = PlaneSet();
   out.n_levels = n_levels;
+  out.level_planes.assign(n_levels, {});
+  out.n_tiles_r.assign(n_levels, 0);
   int pix = 0, tp = 0, tf = 0, ts = 0, tc = 0;
   for (size_t f = 0; f < frame_sizes.size(); ++f) {
     const int W = frame_sizes[f].first, H = frame_sizes[f].second;
@@ -30,7 +32,8 @@ bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_leve
         const int pw = lw, ph = lh;
         lw = pw / 2;
         lh = ph / 2;
-        if (pw != 2 * lw || ph != 2 * lh || lw < 1 || lh < 1) return false;
+        if (lw < 1 || lh < 1) return false;
+        if (pw != 2 * lw || ph != 2 * lh) out.exact_halves = false;
       }
       PlaneDesc d{};
       d.w = lw;
@@ -53,6 +56,9 @@ bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_leve
       tc += cdiv(lw * lh, kChunk);
       d.tile_p = tp;
       if (l == 0) tp += cdiv(lw, kPyrTile) * cdiv(lh, kPyrTile);
+      d.tile_r = out.n_tiles_r[l];
+      if (l > 0) out.n_tiles_r[l] += cdiv(lw, kResTileW) * cdiv(lh, kResTileH);
+      out.level_planes[l].push_back((int32_t)out.planes.size());
       out.planes.push_back(d);
       pix += align256(lw * lh);
     }
@@ -101,6 +107,7 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t o_part = arena_.reserve(sizeof(int32_t) * NP * kScanBlocks);
   const size_t o_counts = arena_.reserve(sizeof(PlaneCounts) * NP);
   const size_t o_planes = arena_.reserve(sizeof(PlaneDesc) * NP);
+  const size_t o_lplanes = arena_.reserve(sizeof(int32_t) * NP);
   size_t o_fstart[kFilterVariants];
   for (int v = 0; v < kFilterVariants; ++v) o_fstart[v] = arena_.reserve(sizeof(int32_t) * NP);
   if (int rc = arena_.allocate()) return rc;
@@ -138,6 +145,21 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   CCTAGX_HIP(hipMemcpy(dp, ps.planes.data(), sizeof(PlaneDesc) * NP, hipMemcpyHostToDevice));
   launch_.dev_planes = dp;
   launch_.n_planes = (int)NP;
+  {
+    std::vector<int32_t> lp;
+    launch_.level_plane_start.clear();
+    launch_.level_plane_count.clear();
+    for (const auto& v : ps.level_planes) {
+      launch_.level_plane_start.push_back((int)lp.size());
+      launch_.level_plane_count.push_back((int)v.size());
+      lp.insert(lp.end(), v.begin(), v.end());
+    }
+    int32_t* dl = arena_.at<int32_t>(o_lplanes);
+    CCTAGX_HIP(hipMemcpy(dl, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice));
+    launch_.dev_level_planes = dl;
+    launch_.resample = !ps.exact_halves;
+    launch_.n_tiles_r = ps.n_tiles_r;
+  }
   launch_.n_levels = ps.n_levels;
   launch_.n_tiles_p = ps.n_tiles_p;
   launch_.n_tiles_f = ps.n_tiles_f;

@@ -41,7 +41,7 @@ __device__ __forceinline__ int plane_of_starts(const int32_t* a, int np, int blk
   }
   return lo;
 }
-static_assert(sizeof(PlaneDesc) == 12 * sizeof(int32_t), "PlaneDesc layout");
+static_assert(sizeof(PlaneDesc) == 13 * sizeof(int32_t), "PlaneDesc layout");
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -129,6 +129,82 @@ __global__ __launch_bounds__(256) void k_pyramid(const PlaneDesc* __restrict__ P
                     s2[2 * ly + 1][2 * lx + 1];
       img[d3.pix + y * d3.w + x] = area2(s, x, vec_end);
     }
+  }
+}
+
+// a1, general sizes: levels are floor-halved (ImagePyramid.cpp:23-33), so a
+// level whose parent has an odd side is not an exact half of it, and
+// cv::resize then takes OpenCV's generic fixed-point bilinear path (8U,
+// 11-bit weights).  Along an axis of source length sn resampled to dn, output
+// d reads source s0 = floor(f) and s0 + 1, f = (float)((d + 0.5) * scale -
+// 0.5), scale = 1 / (dn / sn) in double, with weights round((1 - f) * 2048)
+// and round(f * 2048), half to even; at the last source sample it reads that
+// sample alone.  Rows are combined exactly, the two rows as
+// ((b0 * (S0 >> 4)) >> 16) + ((b1 * (S1 >> 4)) >> 16), + 2, >> 2.
+struct LinearTap {
+  int s0, a0, a1;
+};
+__device__ __forceinline__ LinearTap linear_tap(int d, int sn, int dn) {
+  const double scale = __ddiv_rn(1.0, __ddiv_rn((double)dn, (double)sn));
+  float f = __double2float_rn(__dsub_rn(__dmul_rn(__dadd_rn((double)d, 0.5), scale), 0.5));
+  int s = (int)floorf(f);
+  f = __fsub_rn(f, (float)s);
+  if (s < 0) {
+    f = 0.0f;
+    s = 0;
+  }
+  if (s >= sn - 1) {
+    f = 0.0f;
+    s = sn - 1;
+  }
+  return LinearTap{s, (int)rintf(__fmul_rn(__fsub_rn(1.0f, f), 2048.0f)),
+                   (int)rintf(__fmul_rn(f, 2048.0f))};
+}
+
+// One level from its parent, for the planes of one level (an exact half
+// takes the 2x2 area rule of k_pyramid).  64x16 outputs per block: 64 lanes
+// across a row, each thread 4 rows.
+__global__ __launch_bounds__(256) void k_resample(const PlaneDesc* __restrict__ P,
+                                                   const int32_t* __restrict__ lplanes, int nlp,
+                                                   uint8_t* __restrict__ img) {
+  int lo = 0, hi = nlp - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P[lplanes[mid]].tile_r <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const PlaneDesc d = P[lplanes[lo]];
+  const PlaneDesc s = P[d.parent];
+  const int t = blockIdx.x - d.tile_r;
+  const int tx_n = (d.w + kResTileW - 1) / kResTileW;
+  const int x = (t % tx_n) * kResTileW + (threadIdx.x & 63);
+  const int y0 = (t / tx_n) * kResTileH + (threadIdx.x >> 6) * (kResTileH / 4);
+  if (x >= d.w) return;
+  const uint8_t* src = img + s.pix;
+  uint8_t* dst = img + d.pix;
+  if (s.w == 2 * d.w && s.h == 2 * d.h) {
+    const int vec_end = (d.w / 8) * 8;
+    for (int k = 0; k < kResTileH / 4; ++k) {
+      const int y = y0 + k;
+      if (y >= d.h) break;
+      const uint8_t* r0 = src + (2 * y) * s.w + 2 * x;
+      dst[y * d.w + x] = area2(r0[0] + r0[1] + r0[s.w] + r0[s.w + 1], x, vec_end);
+    }
+    return;
+  }
+  const LinearTap cx = linear_tap(x, s.w, d.w);
+  const int x1 = cx.s0 + 1 < s.w ? cx.s0 + 1 : cx.s0;
+  const int a1 = cx.s0 + 1 < s.w ? cx.a1 : 0;
+  const int a0 = cx.s0 + 1 < s.w ? cx.a0 : 2048;
+  for (int k = 0; k < kResTileH / 4; ++k) {
+    const int y = y0 + k;
+    if (y >= d.h) break;
+    const LinearTap cy = linear_tap(y, s.h, d.h);
+    const int r0 = min(cy.s0, s.h - 1), r1 = min(cy.s0 + 1, s.h - 1);
+    const int h0 = src[r0 * s.w + cx.s0] * a0 + src[r0 * s.w + x1] * a1;
+    const int h1 = src[r1 * s.w + cx.s0] * a0 + src[r1 * s.w + x1] * a1;
+    dst[y * d.w + x] =
+        (uint8_t)((((cy.a0 * (h0 >> 4)) >> 16) + ((cy.a1 * (h1 >> 4)) >> 16) + 2) >> 2);
   }
 }
 
@@ -1173,7 +1249,14 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
 
   mark("start", true);
   if (L.n_tiles_p > 0) {
-    k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img, b.quad);
+    if (!L.resample) {
+      k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, L.n_levels, b.img, b.quad);
+    } else {
+      k_pyramid<<<L.n_tiles_p, 256, 0, s>>>(P, np, 1, b.img, b.quad);
+      for (int l = 1; l < L.n_levels; ++l)
+        k_resample<<<L.n_tiles_r[l], 256, 0, s>>>(P, L.dev_level_planes + L.level_plane_start[l],
+                                                   L.level_plane_count[l], b.img);
+    }
     mark("pyramid");
   }
   {

@@ -41,6 +41,12 @@ struct FrontEndLaunch {
   int n_planes = 0;
   int n_levels = 0;
   int n_tiles_p = 0;   // pyramid blocks (level-0 64x64 tiles)
+  // Level-by-level resampling, used when some level is not an exact half of
+  // its parent: per level l >= 1, its planes (indices into dev_planes, at
+  // dev_level_planes + level_plane_start[l]) and their blocks.
+  bool resample = false;
+  const int32_t* dev_level_planes = nullptr;
+  std::vector<int> level_plane_start, level_plane_count, n_tiles_r;
   int n_tiles_f = 0;   // filter blocks of the default variant
   int filter_variant = 1;                     // index into kFilterTileHs
   const int32_t* dev_filter_starts[kFilterVariants] = {};  // per variant: first block of each plane
@@ -62,10 +68,15 @@ struct PlaneSet {
   int n_tiles_p = 0, n_tiles_f = 0, n_tiles_s = 0, n_tiles_c = 0;
   std::vector<int32_t> filter_starts[kFilterVariants];  // per variant, one entry per plane
   int n_tiles_fv[kFilterVariants] = {};
+  // true when every level of every frame is an exact half of its parent
+  // (the pyramid then runs as one fused 2x2 kernel)
+  bool exact_halves = true;
+  std::vector<std::vector<int32_t>> level_planes;  // per level: its planes in plane order
+  std::vector<int> n_tiles_r;                       // per level: resampling blocks
 };
 
-// Returns false when a level is not an exact half of its parent (the fast
-// 2x2 area path is the only resampling built).
+// Levels are floor(w/2) x floor(h/2) of their parent (ImagePyramid.cpp:23-33).
+// Returns false when a level would be empty.
 bool plan_planes(const std::vector<std::pair<int, int>>& frame_sizes, int n_levels,
                  PlaneSet& out);
 

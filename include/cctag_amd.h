@@ -92,8 +92,10 @@ typedef struct cctagx_handle cctagx_handle;
 
 /* Replaces TagPipe::initialize(w, h) (cuda/tag.cu:76-127) and the per-pipe
  * allocation of initCuda (Detection.cpp:732-760).  Allocates every device and
- * pinned host buffer for frames of width x height once.  The handle's
- * resolution is fixed: every entry point that takes a frame checks its
+ * pinned host buffer for frames of width x height once.  Pyramid level l is
+ * (width >> l) x (height >> l), each level resampled from the one above
+ * (ImagePyramid.cpp:23-33, Level.cpp:67); a size at which a level would be
+ * empty is CCTAGX_ERR_UNSUPPORTED.  The handle's resolution is fixed: every entry point that takes a frame checks its
  * width and height against it and returns CCTAGX_ERR_ARG on a mismatch,
  * where the reference stops the process (Detection.cpp:753-757). */
 int cctagx_create(const cctagx_params* params, int width, int height, cctagx_handle** out);

@@ -14,15 +14,23 @@
 namespace oracle {
 
 // ---------------------------------------------------------------------------
-// a1: pyramid.  Level.cpp:67 calls cv::resize(src, dst, Size(w/2, h/2)) with
-// the default INTER_LINEAR.  For an exact 2x factor OpenCV 4.x replaces it by
-// its fast INTER_AREA path: SIMD lanes (8 outputs per step on the SSE
-// baseline) compute (a+b+c+d+2)>>2, the scalar tail computes
-// cvRound((a+b+c+d) * 0.25f) (round half to even).  This is an assumption
-// about OpenCV internals that no reference test pins.
-void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
-  if (sw != 2 * dw || sh != 2 * dh)
-    throw std::invalid_argument("oracle::downsample: only exact 2x levels are supported");
+// a1: pyramid.  ImagePyramid.cpp:23-33 sizes level l at floor(w/2) x
+// floor(h/2) of level l-1, and Level.cpp:67 fills it with cv::resize(src,
+// dst, dst size), default INTER_LINEAR.  How OpenCV 4.x resolves that call
+// is an assumption about its internals that no reference test pins:
+//  * both factors exactly 2 (even parent): INTER_LINEAR is switched to the
+//    fast 2x2 INTER_AREA path: SIMD lanes (8 outputs per step on the SSE
+//    baseline) compute (a+b+c+d+2)>>2, the scalar tail computes
+//    cvRound((a+b+c+d) * 0.25f) (round half to even);
+//  * otherwise the generic fixed-point bilinear path (8U, 11 coefficient
+//    bits): per output column the source column sx = floor(fx) with
+//    fx = (float)((dx + 0.5) * scale_x - 0.5), scale_x = 1 / (dw / sw) in
+//    double, weights round((1 - f) * 2048) and round(f * 2048) (half to
+//    even); the horizontal pass is exact integer arithmetic, the vertical
+//    pass computes ((b0 * (S0 >> 4)) >> 16) + ((b1 * (S1 >> 4)) >> 16),
+//    plus 2, >> 2 for every column (the scalar loop of the 8U vertical
+//    resampler repeats the SIMD formula); source rows clamp to the image.
+static void downsample_area2(const uint8_t* src, int sw, uint8_t* dst, int dw, int dh) {
   const int vec_end = (dw / 8) * 8;
   for (int y = 0; y < dh; ++y) {
     const uint8_t* r0 = src + (size_t)(2 * y) * sw;
@@ -35,6 +43,56 @@ void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh
       dst[(size_t)y * dw + x] = (uint8_t)v;
     }
   }
+}
+
+// Source index and the two 11-bit weights of output coordinate d along an
+// axis of source length sn resampled to dn.
+void linear_tap(int d, int sn, int dn, int* s0, int* a0, int* a1) {
+  const double inv_scale = (double)dn / (double)sn;
+  const double scale = 1.0 / inv_scale;
+  float f = (float)(((double)d + 0.5) * scale - 0.5);
+  int s = (int)std::floor(f);
+  f -= (float)s;
+  if (s < 0) {
+    f = 0.0f;
+    s = 0;
+  }
+  if (s >= sn - 1) {  // the last source sample alone
+    f = 0.0f;
+    s = sn - 1;
+  }
+  *s0 = s;
+  *a0 = (int)std::nearbyint((1.0f - f) * 2048.0f);
+  *a1 = (int)std::nearbyint(f * 2048.0f);
+}
+
+static void downsample_linear(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
+  std::vector<int> xs(dw), xa0(dw), xa1(dw);
+  for (int x = 0; x < dw; ++x) linear_tap(x, sw, dw, &xs[x], &xa0[x], &xa1[x]);
+  std::vector<int> h0(dw), h1(dw);
+  auto hpass = [&](int row, std::vector<int>& out) {
+    const uint8_t* r = src + (size_t)row * sw;
+    for (int x = 0; x < dw; ++x) {
+      const int s = xs[x];
+      out[x] = s + 1 < sw ? r[s] * xa0[x] + r[s + 1] * xa1[x] : r[s] * 2048;
+    }
+  };
+  for (int y = 0; y < dh; ++y) {
+    int sy, b0, b1;
+    linear_tap(y, sh, dh, &sy, &b0, &b1);
+    hpass(std::min(sy, sh - 1), h0);
+    hpass(std::min(sy + 1, sh - 1), h1);
+    for (int x = 0; x < dw; ++x)
+      dst[(size_t)y * dw + x] =
+          (uint8_t)((((b0 * (h0[x] >> 4)) >> 16) + ((b1 * (h1[x] >> 4)) >> 16) + 2) >> 2);
+  }
+}
+
+void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
+  if (dw < 1 || dh < 1 || dw > sw || dh > sh)
+    throw std::invalid_argument("oracle::downsample: the target must be a non-empty shrink");
+  if (sw == 2 * dw && sh == 2 * dh) downsample_area2(src, sw, dst, dw, dh);
+  else downsample_linear(src, sw, sh, dst, dw, dh);
 }
 
 // ---------------------------------------------------------------------------
@@ -474,7 +532,8 @@ void edges_and_vote(LevelResult& L, const Params& p) {
     L.voters_list.insert(L.voters_list.end(), voters[i].begin(), voters[i].end());
 }
 
-// ImagePyramid.cpp:23-52, Level.cpp:193-217, Multiresolution.cpp:225-237.
+// ImagePyramid.cpp:23-52 (levels floor-halved), Level.cpp:193-217,
+// Multiresolution.cpp:225-237.
 void front_end(const uint8_t* img, int w, int h, const Params& p,
                std::vector<LevelResult>& levels) {
   levels.assign(p.n_levels, LevelResult());

@@ -52,7 +52,7 @@ struct LevelResult {
   }
 };
 
-// Level.cpp:67 -- cv::resize(src, dst, Size(w/2, h/2)) on 8U.
+// Level.cpp:67 -- cv::resize(src, dst, Size(dw, dh)) on 8U, dw <= sw, dh <= sh.
 void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh);
 // cvRecode.cpp:74-92 -- 9x9 derivative filters, BORDER_REPLICATE, int16.
 void derivatives(const uint8_t* img, int w, int h, int16_t* dx, int16_t* dy);

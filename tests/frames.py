@@ -56,6 +56,23 @@ def small_batch():
     return out
 
 
+# Frame sizes whose pyramid does not halve exactly at every level: a 1366x768
+# panel (683x384 -> 341x192 -> 170x96), odd sides from level 0 on, and a 1080p
+# frame cropped by one column and row.
+ODD_SIZES = [(1366, 768), (1001, 751), (1919, 1079)]
+
+
+@functools.lru_cache(maxsize=None)
+def odd_size_frame(w: int, h: int, seed: int = 8100):
+    """Markers as in the 1280x720 configuration, at a frame size whose
+    floor-halved pyramid has levels that are not exact halves."""
+    cfg = synth.FrameConfig(w, h, (5, 5), (30, 120), 45.0, (0.8, 1.5), 5.0,
+                            occlusion_frac=0.2, distractors=(10, 10))
+    img, gts = synth.make_frame(cfg, seed + w + h)
+    img.setflags(write=False)
+    return img, gts
+
+
 @functools.lru_cache(maxsize=None)
 def border_frame(seed: int = 7000):
     img, gts = synth.border_frame(seed)

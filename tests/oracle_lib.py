@@ -168,14 +168,16 @@ def thin_lut(pass_: int) -> list:
     return [o.oracle_thin_lut(pass_, i) for i in range(512)]
 
 
-def downsample(img: np.ndarray) -> np.ndarray:
+def downsample(img: np.ndarray, size=None) -> np.ndarray:
+    """Level.cpp:67 resize to `size` = (w, h), default (w // 2, h // 2)."""
     o = load()
     img = np.ascontiguousarray(img, np.uint8)
     h, w = img.shape
-    out = np.zeros((h // 2, w // 2), np.uint8)
-    rc = o.oracle_downsample(img.ctypes.data, w, h, out.ctypes.data, w // 2, h // 2)
+    dw, dh = size if size is not None else (w // 2, h // 2)
+    out = np.zeros((dh, dw), np.uint8)
+    rc = o.oracle_downsample(img.ctypes.data, w, h, out.ctypes.data, dw, dh)
     if rc != 0:
-        raise ValueError("not an exact 2x level")
+        raise ValueError("not a shrink")
     return out
 
 

@@ -87,3 +87,21 @@ def test_uneven_batch_matches_oracle_per_frame():
         print("oracle on batch frame", k, im.shape, flush=True)
         match(got[f], oracle_lib.detect(im, threads=8))
     assert sum(m["status"] == 1 for ms in got for m in ms) >= 20
+
+
+@pytest.mark.gpu
+def test_floor_halved_batch_matches_oracle_per_frame():
+    """A batch mixing frames whose pyramids halve exactly with frames whose
+    levels are floor halves: the level-by-level resampling runs inside the
+    captured graph for the whole plane set."""
+    imgs = [frames.odd_size_frame(w, h)[0] for (w, h) in frames.ODD_SIZES]
+    imgs.insert(1, frames.small_marker_frame()[0])
+    print("floor-halved batch", [im.shape for im in imgs], flush=True)
+    with BatchDetector([(im.shape[1], im.shape[0]) for im in imgs]) as b:
+        for f, im in enumerate(imgs):
+            b.load(f, im)
+        got = b.detect()
+        again = b.detect()
+    assert again == got
+    for f, im in enumerate(imgs):
+        match(got[f], oracle_lib.detect(im, threads=8))

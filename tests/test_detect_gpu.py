@@ -144,3 +144,29 @@ def test_detect_four_crowns_matches_oracle(k):
     # the oracle reads one marker of frame 2 with a wrong ID as well: with 128
     # rows of seven rings, neighbouring profiles differ by one ring radius
     assert ids_ok >= len(gts) - 1 and len(errs) >= len(gts) - 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h", frames.ODD_SIZES)
+def test_detect_floor_halved_pyramid(w, h):
+    """Frame sizes whose pyramid levels are floor halves, not exact halves
+    (ImagePyramid.cpp:23-33): same markers as the oracle, ground truth found."""
+    img, gts = frames.odd_size_frame(w, h)
+    print("detect parity floor-halved", img.shape, flush=True)
+    ref = oracle_lib.detect(img, threads=8)
+    with Detector(w, h) as det:
+        got = det.detect(img)
+    match(got, ref)
+    errs, ids_ok = gt_errors(got, gts)
+    assert ids_ok == len(errs) and len(errs) >= len(gts) - 1 and max(errs) < 0.5
+
+
+@pytest.mark.gpu
+def test_frame_too_small_for_the_pyramid():
+    """Level 3 of a 7x7 frame would be empty: CCTAGX_ERR_UNSUPPORTED."""
+    from cctag_amd import _lib
+    with pytest.raises(_lib.CCTagError) as e:
+        Detector(7, 7)
+    assert e.value.rc == -6
+    with Detector(8, 8) as det:
+        assert det.detect(np.full((8, 8), 128, np.uint8)) == []

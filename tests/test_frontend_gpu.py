@@ -48,6 +48,19 @@ def test_front_end_parity_configs(config, k):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("w,h", frames.ODD_SIZES)
+def test_front_end_parity_floor_halved_pyramid(w, h):
+    """Sizes whose levels are floor halves (ImagePyramid.cpp:23-33), not exact
+    halves: every level bit for bit, resampling included."""
+    img, _ = frames.odd_size_frame(w, h)
+    print("front-end parity floor-halved", img.shape, flush=True)
+    ref = run_parity(img)
+    assert [r["width"] for r in ref] == [w >> l for l in range(4)]
+    assert [r["height"] for r in ref] == [h >> l for l in range(4)]
+    assert ref[0]["n_edges"] > 1000
+
+
+@pytest.mark.gpu
 def test_front_end_parity_small_markers():
     img, _ = frames.small_marker_frame()
     run_parity(img)

@@ -100,6 +100,69 @@ def test_downsample_closed_form():
         assert np.all(np.abs(out - s / 4.0) <= 0.5)
 
 
+def _ideal_bilinear(img, dw, dh):
+    """Float64 bilinear resampling with pixel centres aligned
+    ((d + 0.5) * sn / dn - 0.5) and the edge sample replicated."""
+    h, w = img.shape
+
+    def axis(sn, dn):
+        f = (np.arange(dn) + 0.5) * (sn / dn) - 0.5
+        s0 = np.floor(f).astype(int)
+        t = f - s0
+        return np.clip(s0, 0, sn - 1), np.clip(s0 + 1, 0, sn - 1), t
+
+    x0, x1, tx = axis(w, dw)
+    y0, y1, ty = axis(h, dh)
+    a = img.astype(np.float64)
+    top = a[y0][:, x0] * (1 - tx) + a[y0][:, x1] * tx
+    bot = a[y1][:, x0] * (1 - tx) + a[y1][:, x1] * tx
+    return top * (1 - ty)[:, None] + bot * ty[:, None]
+
+
+@pytest.mark.parametrize("w,h", [(1366, 768), (683, 384), (101, 77), (5, 3), (3, 9)])
+def test_downsample_floor_halving_is_bilinear(w, h):
+    """Levels of odd parents (ImagePyramid.cpp:23-33 floor-halves) take
+    cv::resize's generic bilinear path.  Its fixed-point details (11-bit
+    weights, the >>4 / >>16 truncations of the vertical pass) are an
+    assumption about OpenCV internals that nothing here can pin: parity
+    unpinned.  What is pinned: the result is within 1 DN of the exact
+    bilinear value with centre-aligned pixels (a wrong offset, scale or a
+    swapped axis is off by far more on a random image), a constant image
+    stays constant, and an axis that halves exactly averages pixel pairs."""
+    rng = np.random.Generator(np.random.PCG64(w * 1000 + h))
+    img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+    dw, dh = w // 2, h // 2
+    out = oracle_lib.downsample(img).astype(np.float64)
+    ideal = _ideal_bilinear(img, dw, dh)
+    assert np.abs(out - ideal).max() <= 1.0
+    # a wrong pixel-centre convention misses by far more
+    shifted = _ideal_bilinear(img[:, 1:], dw, dh) if w > 3 else None
+    if shifted is not None and w > 50:
+        assert np.abs(out - shifted).mean() > 10
+    for c in (0, 1, 77, 128, 254, 255):
+        flat = np.full((h, w), c, np.uint8)
+        assert (oracle_lib.downsample(flat) == c).all()
+    if w == 683:  # rows halve exactly: the vertical weights are 1/2, 1/2
+        rows = np.repeat(rng.integers(0, 256, size=(h // 2, w), dtype=np.uint8), 2, axis=0)
+        a = oracle_lib.downsample(rows).astype(np.float64)
+        assert np.abs(a - _ideal_bilinear(rows[::2], dw, dh)).max() <= 1.0
+
+
+def test_downsample_dispatch():
+    """Exact halves on both axes use the 2x2 area rule; anything else, even a
+    level that halves exactly in one axis, the bilinear path."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    img = rng.integers(0, 256, size=(40, 70), dtype=np.uint8)
+    assert np.array_equal(oracle_lib.downsample(img), oracle_lib.downsample(img, (35, 20)))
+    odd = img[:, :69]
+    s = (odd[0::2, 0:68:2].astype(int) + odd[0::2, 1:68:2] + odd[1::2, 0:68:2] + odd[1::2, 1:68:2])
+    lin = oracle_lib.downsample(odd).astype(int)
+    assert lin.shape == (20, 34)
+    assert not np.array_equal(lin, (s + 2) >> 2)
+    with pytest.raises(ValueError):
+        oracle_lib.downsample(img, (71, 20))
+
+
 # ---- a3..a5: magnitude, NMS, hysteresis -------------------------------------
 def _canny_by_components(dx, dy, low=2, high=10):
     """Magnitude rint(sqrt) in float32, the Q15 sector NMS with zero
