@@ -239,6 +239,11 @@ def load() -> ctypes.CDLL:
     lib.cctagx_detect.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                                   ctypes.c_uint64, P(Marker), ctypes.c_int, P(ctypes.c_int)]
     lib.cctagx_load.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+    lib.cctagx_load_bgr.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                    ctypes.c_int]
+    lib.cctagx_detect_bgr.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                      ctypes.c_int, ctypes.c_uint64, P(Marker), ctypes.c_int,
+                                      P(ctypes.c_int)]
     lib.cctagx_detect_loaded.argtypes = [vp, ctypes.c_uint64, P(Marker), ctypes.c_int,
                                          P(ctypes.c_int)]
     lib.cctagx_detect_times.argtypes = [vp, P(c_f32), P(c_f32), P(c_f32)]

@@ -191,6 +191,27 @@ int cctagx_detect(cctagx_handle* h, const uint8_t* gray, int width, int height, 
   return cctagx_detect_loaded(h, frame, out, max_out, n_out);
 }
 
+int cctagx_load_bgr(cctagx_handle* h, const uint8_t* bgr, int width, int height, size_t stride,
+                    int channels) {
+  if (channels != 3 && channels != 4) {
+    set_error("color frames have 3 (BGR) or 4 (BGRA) channels");
+    return CCTAGX_ERR_ARG;
+  }
+  // stride / channels >= width exactly when a row of width pixels fits
+  if (int rc = check_frame(h, bgr, width, height, stride / channels)) return rc;
+  if (int rc = h->fe.upload_color(0, bgr, stride, channels, h->stream)) return rc;
+  h->have_frame = true;
+  h->have_front_end = false;
+  return CCTAGX_OK;
+}
+
+int cctagx_detect_bgr(cctagx_handle* h, const uint8_t* bgr, int width, int height, size_t stride,
+                      int channels, uint64_t frame, cctagx_marker* out, int max_out, int* n_out) {
+  if ((max_out > 0 && !out) || !n_out) return CCTAGX_ERR_ARG;
+  if (int rc = cctagx_load_bgr(h, bgr, width, height, stride, channels)) return rc;
+  return cctagx_detect_loaded(h, frame, out, max_out, n_out);
+}
+
 int cctagx_set_bank(cctagx_handle* h, const float* ratios, int rows, int cols) {
   if (!h || !ratios || rows < 1 || cols < 1) return CCTAGX_ERR_ARG;
   if (rows > kMaxBankRows || rows > kMaxBank || cols > kMaxRatios) {

@@ -11,6 +11,9 @@ class FrontEndDevice {
   int init(const PlaneSet& ps, const FrontEndParams& prm);
   // Copies a row-major 8-bit frame into the level-0 plane of `frame`.
   int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s);
+  // The same from an 8-bit BGR (channels 3) or BGRA (4) frame, converted to
+  // gray on the device; the first call grows a staging buffer to the frame.
+  int upload_color(int frame, const uint8_t* bgr, size_t stride, int channels, hipStream_t s);
   // Enqueues the whole front end (no host synchronization).
   int run(hipStream_t s, StageTimer* timer);
 
@@ -28,6 +31,7 @@ class FrontEndDevice {
   PlaneSet ps_;
   DeviceArena arena_;
   FrontEndLaunch launch_;
+  DeviceBuffer color_;  // staged color frame
 };
 
 }  // namespace cctagx

@@ -81,6 +81,34 @@ class Detector:
         _lib.check(self._lib.cctagx_load(self._h, img.ctypes.data, w, h, img.strides[0]),
                    "cctagx_load")
 
+    @staticmethod
+    def _color(img: np.ndarray) -> np.ndarray:
+        img = np.asarray(img)
+        if img.dtype != np.uint8 or img.ndim != 3 or img.shape[2] not in (3, 4):
+            raise ValueError("expected an h x w x 3 (BGR) or h x w x 4 (BGRA) uint8 frame")
+        if img.strides[1:] != (img.shape[2], 1):
+            img = np.ascontiguousarray(img)
+        return img
+
+    def detect_bgr(self, img: np.ndarray, frame: int = 0, max_markers: int = 1024) -> list:
+        """detect() of a BGR / BGRA frame, converted to gray on the device as
+        the reference's applications convert it (cv::cvtColor BGR2GRAY)."""
+        img = self._color(img)
+        h, w, c = img.shape
+        buf = (_lib.Marker * max_markers)()
+        n = ctypes.c_int()
+        _lib.check(self._lib.cctagx_detect_bgr(self._h, img.ctypes.data, w, h, img.strides[0], c,
+                                               frame, buf, max_markers, ctypes.byref(n)),
+                   "cctagx_detect_bgr")
+        return self._markers(buf, n.value, max_markers)
+
+    def load_bgr(self, img: np.ndarray) -> None:
+        """load() of a BGR / BGRA frame (converted on the device)."""
+        img = self._color(img)
+        h, w, c = img.shape
+        _lib.check(self._lib.cctagx_load_bgr(self._h, img.ctypes.data, w, h, img.strides[0], c),
+                   "cctagx_load_bgr")
+
     def detect_loaded(self, frame: int = 0, max_markers: int = 1024) -> list:
         """Detection on the resident frame (no host-to-device copy)."""
         buf = (_lib.Marker * max_markers)()

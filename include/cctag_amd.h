@@ -186,6 +186,20 @@ int cctagx_load(cctagx_handle* h, const uint8_t* gray, int width, int height, si
 int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, int max_out,
                          int* n_out);
 
+/* Color input.  The reference's applications convert every image and video
+ * frame with cv::cvtColor(src, gray, COLOR_BGR2GRAY) before cctagDetection
+ * (applications/detection/main.cpp:357,427,515; regression/TestLog.cpp:93,
+ * 114); these take the 8-bit BGR (channels 3) or BGRA (channels 4, alpha
+ * ignored) frame instead and convert it on the device with OpenCV's integer
+ * weights, gray = (1868 B + 9617 G + 4899 R + 8192) >> 14.  stride is in
+ * bytes (>= width * channels).  The first call on a handle allocates a
+ * device staging buffer of width * height * channels bytes.  Otherwise as
+ * cctagx_load / cctagx_detect. */
+int cctagx_load_bgr(cctagx_handle* h, const uint8_t* bgr, int width, int height, size_t stride,
+                    int channels);
+int cctagx_detect_bgr(cctagx_handle* h, const uint8_t* bgr, int width, int height, size_t stride,
+                      int channels, uint64_t frame, cctagx_marker* out, int max_out, int* n_out);
+
 /* ---- identification of given markers ------------------------------------
  * The identification loop of cctagDetection (Detection.cpp:873-965:
  * identify_step_1, then TagPipe::uploadCuts / imageCenterOptLoop /

@@ -101,6 +101,11 @@ void oracle_thin(uint8_t* edges, int w, int h) { oracle::thin(edges, w, h); }
 
 int oracle_thin_lut(int pass, int index) { return oracle::thin_lut(pass, index); }
 
+void oracle_bgr_to_gray(const uint8_t* src, int w, int h, size_t stride, int channels,
+                        uint8_t* dst) {
+  oracle::bgr_to_gray(src, w, h, stride, channels, dst);
+}
+
 int oracle_downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
   try {
     oracle::downsample(src, sw, sh, dst, dw, dh);

@@ -88,6 +88,18 @@ static void downsample_linear(const uint8_t* src, int sw, int sh, uint8_t* dst, 
   }
 }
 
+// Input conversion of the reference's applications (detection/main.cpp:357,
+// 427,515; regression/TestLog.cpp:93,114): cv::cvtColor(COLOR_BGR2GRAY) on 8U
+// is OpenCV's fixed-point Rec. 601 luma with 14-bit weights B 1868, G 9617,
+// R 4899 (0.114, 0.587, 0.299 times 2^14, rounded) and rounding 2^13.
+void bgr_to_gray(const uint8_t* src, int w, int h, size_t stride, int channels, uint8_t* dst) {
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const uint8_t* p = src + (size_t)y * stride + (size_t)x * channels;
+      dst[(size_t)y * w + x] = (uint8_t)((p[0] * 1868 + p[1] * 9617 + p[2] * 4899 + 8192) >> 14);
+    }
+}
+
 void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh) {
   if (dw < 1 || dh < 1 || dw > sw || dh > sh)
     throw std::invalid_argument("oracle::downsample: the target must be a non-empty shrink");

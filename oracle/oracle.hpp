@@ -52,6 +52,8 @@ struct LevelResult {
   }
 };
 
+// cv::cvtColor(COLOR_BGR2GRAY) on an 8-bit BGR / BGRA frame.
+void bgr_to_gray(const uint8_t* src, int w, int h, size_t stride, int channels, uint8_t* dst);
 // Level.cpp:67 -- cv::resize(src, dst, Size(dw, dh)) on 8U, dw <= sw, dh <= sh.
 void downsample(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh);
 // cvRecode.cpp:74-92 -- 9x9 derivative filters, BORDER_REPLICATE, int16.

@@ -36,6 +36,9 @@ def load():
     o.oracle_thin.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     o.oracle_thin_lut.argtypes = [ctypes.c_int, ctypes.c_int]
     o.oracle_downsample.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
+    o.oracle_bgr_to_gray.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                     vp]
+    o.oracle_bgr_to_gray.restype = None
     o.oracle_time_front_end.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     o.oracle_time_front_end.restype = ctypes.c_double
     o.oracle_detect.argtypes = [ctypes.POINTER(_lib.Params), vp, ctypes.c_int, ctypes.c_int,
@@ -166,6 +169,16 @@ def thin(edges: np.ndarray) -> np.ndarray:
 def thin_lut(pass_: int) -> list:
     o = load()
     return [o.oracle_thin_lut(pass_, i) for i in range(512)]
+
+
+def bgr_to_gray(img: np.ndarray) -> np.ndarray:
+    """cv::cvtColor(COLOR_BGR2GRAY) of an h x w x 3 (BGR) or x 4 (BGRA) array."""
+    o = load()
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w, c = img.shape
+    out = np.zeros((h, w), np.uint8)
+    o.oracle_bgr_to_gray(img.ctypes.data, w, h, ctypes.c_size_t(w * c), c, out.ctypes.data)
+    return out
 
 
 def downsample(img: np.ndarray, size=None) -> np.ndarray:

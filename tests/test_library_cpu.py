@@ -71,6 +71,8 @@ def test_argument_errors_without_device():
     n = ctypes.c_int()
     assert lib.cctagx_detect(None, None, 0, 0, 0, 0, None, 0, ctypes.byref(n)) == -1
     assert lib.cctagx_load(None, None, 0, 0, 0) == -1
+    assert lib.cctagx_load_bgr(None, None, 0, 0, 0, 3) == -1
+    assert lib.cctagx_detect_bgr(None, None, 0, 0, 0, 3, 0, None, 0, ctypes.byref(n)) == -1
     assert lib.cctagx_front_end(None, None, 0, 0, 0) == -1
     assert lib.cctagx_last_markers(None, None, 0, ctypes.byref(n)) == -1
     bank = np.ones((32, 5), np.float32)

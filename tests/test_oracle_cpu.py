@@ -163,6 +163,30 @@ def test_downsample_dispatch():
         oracle_lib.downsample(img, (71, 20))
 
 
+# ---- input conversion (BGR -> gray) -----------------------------------------
+def test_bgr_to_gray_known_answers():
+    """cv::cvtColor(COLOR_BGR2GRAY) on 8U: the well-known values of pure
+    colors (red 76, green 150, blue 29: the green one is 149.68 before the
+    rounding term, so truncation would give 149), gray pixels unchanged, BGRA's
+    alpha ignored, and within 0.5 DN + the weight rounding of the Rec. 601
+    float luma 0.299 R + 0.587 G + 0.114 B (a swapped channel order misses
+    by tens of DN)."""
+    px = np.array([[[0, 0, 255], [0, 255, 0], [255, 0, 0], [255, 255, 255], [0, 0, 0]]],
+                  np.uint8)
+    assert oracle_lib.bgr_to_gray(px).tolist() == [[76, 150, 29, 255, 0]]
+    v = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    assert np.array_equal(oracle_lib.bgr_to_gray(np.stack([v, v, v], -1)), v)
+    rng = np.random.Generator(np.random.PCG64(21))
+    bgr = rng.integers(0, 256, size=(37, 53, 3), dtype=np.uint8)
+    g = oracle_lib.bgr_to_gray(bgr).astype(np.float64)
+    luma = 0.114 * bgr[..., 0] + 0.587 * bgr[..., 1] + 0.299 * bgr[..., 2]
+    assert np.abs(g - luma).max() <= 0.5 + 255 * 3 / 2 ** 15
+    swapped = 0.299 * bgr[..., 0] + 0.587 * bgr[..., 1] + 0.114 * bgr[..., 2]
+    assert np.abs(g - swapped).max() > 20
+    bgra = np.concatenate([bgr, rng.integers(0, 256, size=(37, 53, 1), dtype=np.uint8)], -1)
+    assert np.array_equal(oracle_lib.bgr_to_gray(bgra), oracle_lib.bgr_to_gray(bgr))
+
+
 # ---- a3..a5: magnitude, NMS, hysteresis -------------------------------------
 def _canny_by_components(dx, dy, low=2, high=10):
     """Magnitude rint(sqrt) in float32, the Q15 sector NMS with zero
