@@ -117,7 +117,21 @@ bool make_refine_job(const Ellipse& e, const Pt2& center0, const std::vector<Pt2
   r.stops = stops;
   r.begin_sig = kc.start_sig;
   r.end_sig = 1.f;
+  {
+    const double c = std::cos((double)e.angle()), s = std::sin((double)e.angle());
+    const double a = e.a(), b = e.b();
+    const double hx = std::sqrt(a * a * c * c + b * b * s * s), hy = std::sqrt(a * a * s * s + b * b * c * c);
+    r.box[0] = (float)(e.cx() - hx);
+    r.box[1] = (float)(e.cx() + hx);
+    r.box[2] = (float)(e.cy() - hy);
+    r.box[3] = (float)(e.cy() + hy);
+  }
   return true;
+}
+
+bool box_inside(const RefineJob& r, int w, int h) {
+  return r.box[0] >= 2.f && r.box[1] <= (float)w - 3.f && r.box[2] >= 2.f &&
+         r.box[3] <= (float)h - 3.f;
 }
 
 void IdentBackend::select_and_refine(const std::vector<CutJob>& cuts,

@@ -61,7 +61,11 @@ struct RefineJob {
   Pt2 center0;                       // initial imaged centre (CCTag::centerImg)
   std::vector<Pt2> stops;            // selected cut stops
   float begin_sig = 0.25f, end_sig = 1.f;
+  float box[4] = {0, 0, 0, 0};       // the ellipse's bounding box: x min, x max, y min, y max
 };
+
+// Whether a job's ellipse box lies at least 2 px inside a w x h image.
+bool box_inside(const RefineJob& r, int w, int h);
 
 struct RefineResult {
   bool ok = false;                   // every grid search found a readable point

@@ -177,6 +177,7 @@ int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs,
     d.begin_sig = r.begin_sig;
     d.end_sig = r.end_sig;
     d.n_iter = r.n_iter;
+    d.box_inside = box_inside(r, im.w, im.h) ? 1 : 0;
     d.c0x = r.center0.x;
     d.c0y = r.center0.y;
     if (sel_of) {

@@ -37,6 +37,7 @@ struct RefineJobDev {
   float t_can[9], t_inv_can[9], cond[9], cond_inv[9];
   float grid_scale, neighbour0, begin_sig, end_sig;
   int n_iter, n_cuts;
+  int box_inside;  // the outer ellipse's bounding box lies 2 px inside the image
   float c0x, c0y;
   int dist_offset;  // first float of its distances in the packed output
   int sel_index;    // >= 0: the selection whose stops the job takes on the device
@@ -70,9 +71,11 @@ struct RefineGridDev {
   float H[9], Hi[9];
   float gx, gy;
   double cost_local;   // residual sum assuming only this point's own flags
-  int m_local;         // valid cuts under that assumption
+  int16_t m_local;     // valid cuts under that assumption
+  int16_t has_gram;    // its Gram matrix was stored (the selection may recompute from it)
   uint32_t oob;        // bit k: a sample of cut k is outside the image
 };
+static_assert(sizeof(RefineGridDev) == 96, "grid record size");
 constexpr int kMaxPairs = kMaxCuts * (kMaxCuts + 1) / 2;
 // Two sets (by iteration parity): an iteration reads the previous set and
 // writes its own.

@@ -337,15 +337,25 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
 
 // ---------------------------------------------------------------------------
 // Centre refinement, one launch per grid-search iteration.  Block (g, m)
-// of k_refine_iter evaluates grid point g of marker m: its homography
-// (thread 0), the signals of every selected cut (LDS), their out-of-bounds
-// flags and the Gram matrix C_kl = sum_i s_ki s_li in double.  The
+// of k_refine_iter evaluates grid point g of marker m: its homography, the
+// signals of every selected cut (LDS), their out-of-bounds flags and the
+// residual sum_{k<l} sum_i (s_ki - s_li)^2 over its valid cuts.  The
 // selection that follows an iteration -- the sticky out-of-bounds flags in
-// grid-point order, the residual of grid point g over its valid cuts as
-// sum_{k<l} sum_i (s_ki - s_li)^2 = (m-1) sum_k C_kk - 2 sum_{k<l} C_kl, the
-// move to the first minimum as the reference's loop does -- is redone by
-// every block of the next launch (and by the final kernel), so an
+// grid-point order, each grid point's residual over the cuts valid there,
+// the move to the first minimum as the reference's loop does -- is redone
+// by every block of the next launch (and by the final kernel), so an
 // iteration costs one launch.
+//
+// A cut flagged at an earlier grid point of the same iteration is invalid
+// at the later ones, which a block cannot see while it runs.  So block g
+// first tests, for the grid points before it, whether each valid cut surely
+// stays inside the image (cut_surely_inside).  When every one does -- the
+// usual case -- the block knows its valid cuts and takes the residual as
+// sum_i (m S2_i - S1_i^2), S1_i and S2_i the sum and the sum of squares of
+// the valid cuts' samples (double, in cut order).  Otherwise it stores the
+// Gram matrix C_kl = sum_i s_ki s_li in double, from which the selection
+// recomputes the residual for any valid set as
+// (m-1) sum_k C_kk - 2 sum_{k<l} C_kl.
 constexpr int kRefineBlock = 256;
 constexpr int kFinalBlock = 1024;  // one block per marker: every loop of the read-out in one pass
 constexpr int kRefineChunk = 10;  // samples per sampling work item
@@ -428,6 +438,31 @@ __device__ void cut_walk_starts(const float* Hi, float2 stop, float begin_sig, f
     }
   }
   w.step = make_float2(stepX, stepY);
+}
+
+// Whether every sample of a cut surely falls inside the image at the grid
+// point whose homography is H (inverse Hi).  The samples lie, up to the
+// rounding of their running sums and of the mapping (far below a pixel), on
+// the image of the marker-plane segment t * b, t in [begin, end], b = the
+// stop back-projected (homogeneous (b0, b1, b2) = Hi (stop, 1)).  When the
+// projective denominator keeps one sign along t in [0, 1], H maps that
+// segment to a straight one ending at the stop; H maps the unit disk onto
+// the outer ellipse's interior, so with begin * b inside the disk the inner
+// end lies in the ellipse.  Stop and ellipse box 2 px inside the image
+// (box_inside) then keep every sample inside [0, w-1) x [0, h-1).
+__device__ __forceinline__ bool cut_surely_inside(const float* H, const float* Hi, float2 stop,
+                                                  float begin_sig, float end_sig, int w, int h) {
+  if (!(end_sig <= 1.f && stop.x >= 2.f && stop.x <= (float)w - 3.f && stop.y >= 2.f &&
+        stop.y <= (float)h - 3.f))
+    return false;
+  const float b0 = Hi[0] * stop.x + Hi[1] * stop.y + Hi[2];
+  const float b1 = Hi[3] * stop.x + Hi[4] * stop.y + Hi[5];
+  const float b2 = Hi[6] * stop.x + Hi[7] * stop.y + Hi[8];
+  const float e0 = H[8] * b2, e1 = H[6] * b0 + H[7] * b1 + H[8] * b2;
+  const float scale = fabsf(H[6] * b0) + fabsf(H[7] * b1) + fabsf(H[8] * b2);
+  const bool keeps_sign = (e0 > 0.f) == (e1 > 0.f) && fabsf(e0) > 1e-3f * scale &&
+                          fabsf(e1) > 1e-3f * scale;
+  return keeps_sign && begin_sig * begin_sig * (b0 * b0 + b1 * b1) <= 0.98f * (b2 * b2);
 }
 
 // Second part for samples [c*kRefineChunk, ...) of a cut: mapped by H, half
@@ -518,7 +553,7 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
     const uint32_t sp = prev.sticky;
     uint32_t own = 0;
     double acc = 0.0;
-    int ml = 0;
+    int ml = 0, hg = 0;
     float gx = 0.f, gy = 0.f;
     if (on) {
       const RefineGridDev& G = GM[tid];
@@ -527,6 +562,7 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
       ml = G.m_local;
       gx = G.gx;
       gy = G.gy;
+      hg = G.has_gram;
     }
     uint32_t run = own;
     for (int o = 1; o < 64; o <<= 1) {
@@ -538,7 +574,11 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
     const int mv = __popc(valid);
     const bool same = valid == (~(sp | own) & cuts) && mv == ml;
     // points whose valid cuts differ from the ones their block assumed, one
-    // at a time over the whole wavefront, as the block itself sums
+    // at a time over the whole wavefront, as the block itself sums.  Such a
+    // point's block found a flag possible at an earlier point and stored its
+    // Gram matrix; were one missing, the marker fails rather than keep a
+    // residual over the wrong cuts (cut_surely_inside rules that out).
+    const bool lost = __ballot(on && !same && !hg) != 0;
     for (unsigned long long redo = __ballot(on && !same); redo; redo &= redo - 1) {
       const int gg = __builtin_ctzll(redo);
       const uint32_t vg = __shfl(valid, gg, 64);
@@ -562,7 +602,7 @@ __device__ void pick_state(const RefineJobDev& J, int grid_n, const RefineStateD
       }
     }
     const uint32_t sticky = __shfl(run, G2 - 1, 64);
-    const int b = idx < 64 ? idx : -1;
+    const int b = idx < 64 && !lost ? idx : -1;
     const float bx = __shfl(gx, b < 0 ? 0 : b, 64), by = __shfl(gy, b < 0 ? 0 : b, 64);
     if (tid == 0) {
       S.sticky = sticky;
@@ -628,12 +668,15 @@ __device__ void state_at(const RefineJobDev& J, int it, int m, int n_jobs, int g
 __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     const RefineJobDev* __restrict__ jobs, int it, int n_samples, int grid_n, int fused,
     RefineWork W) {
-  __shared__ float sH[9], sHi[9];
+  __shared__ float gH[G2MAX][9], gHi[G2MAX][9];
   __shared__ float sig[kMaxCuts][kMaxSamples + 1];
   __shared__ int oobk[kMaxCuts];
-  __shared__ double sC[kMaxPairs];
+  __shared__ double sC[kMaxPairs];  // the Gram matrix, or the per-sample terms
   __shared__ CutWalk walk[kMaxCuts];
   __shared__ RefineStateDev S;
+  __shared__ int maybe_out;
+  __shared__ uint32_t own_oob;
+  static_assert(kMaxPairs >= kMaxSamples, "sC holds the per-sample terms");
 
   // Workgroups are dealt to the 8 XCDs in turn, each XCD with its own L2:
   // renumber them so that an XCD runs consecutive (grid point, marker)
@@ -662,21 +705,38 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   const int nc = J.n_cuts;
   const Img im{J.quad, J.iw, J.ih};
   RefineGridDev& GD = W.grid[((size_t)cb * n_jobs + m) * G2MAX + g];
-  if (tid == 0) {
+  // the homographies of grid points 0 .. g, lane p on point p (the block
+  // stores its own for the selection and the read-out)
+  if (tid <= g) {
     float Hf[9], Hi[9], qx, qy;
-    grid_homography(J, S.cx, S.cy, S.neighbour, g, grid_n, Hf, Hi, qx, qy);
+    grid_homography(J, S.cx, S.cy, S.neighbour, tid, grid_n, Hf, Hi, qx, qy);
     for (int r = 0; r < 9; ++r) {
-      sH[r] = Hf[r];
-      sHi[r] = Hi[r];
-      GD.H[r] = Hf[r];
-      GD.Hi[r] = Hi[r];
+      gH[tid][r] = Hf[r];
+      gHi[tid][r] = Hi[r];
     }
-    GD.gx = qx;
-    GD.gy = qy;
+    if (tid == g) {
+      for (int r = 0; r < 9; ++r) {
+        GD.H[r] = Hf[r];
+        GD.Hi[r] = Hi[r];
+      }
+      GD.gx = qx;
+      GD.gy = qy;
+    }
   }
   if (tid < kMaxCuts) oobk[tid] = 0;
+  if (tid == 0) maybe_out = g > 0 && !J.box_inside;
   __syncthreads();
-  if (tid < nc) cut_walk_starts(sHi, J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
+  const float* sH = gH[g];
+  if (tid < nc) cut_walk_starts(gHi[g], J.stops[tid], J.begin_sig, J.end_sig, n_samples, walk[tid]);
+  // may a valid cut leave the image at a grid point before this one?
+  if (J.box_inside) {
+    for (int p = tid; p < g * nc; p += kRefineBlock) {
+      const int gp = p / nc, k = p - gp * nc;
+      if (!((S.sticky >> k) & 1u) &&
+          !cut_surely_inside(gH[gp], gHi[gp], J.stops[k], J.begin_sig, J.end_sig, J.iw, J.ih))
+        maybe_out = 1;
+    }
+  }
   __syncthreads();
   const int nchunk = (n_samples + kRefineChunk - 1) / kRefineChunk;
   for (int w = tid; w < nc * nchunk; w += kRefineBlock) {
@@ -684,6 +744,49 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
     if (sample_chunk(im, sH, walk[k], n_samples, c, sig[k])) atomicOr(&oobk[k], 1);
   }
   __syncthreads();
+  if (tid < 64) {
+    const uint32_t o = (uint32_t)__ballot(tid < nc && oobk[tid] != 0);
+    if (tid == 0) own_oob = o;
+  }
+  __syncthreads();
+  const uint32_t oob = own_oob;
+  const uint32_t valid = ~(S.sticky | oob) & cut_bits(nc);
+  const int mv = __popc(valid);
+  if (!maybe_out) {
+    // the valid cuts are known: per sample, sum and sum of squares over them
+    // in cut order (read kRowBatch cuts at a time, indices clamped, so the
+    // reads are in flight together), then the terms summed in a fixed order
+    for (int i = tid; i < n_samples; i += kRefineBlock) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int k0 = 0; k0 < nc; k0 += kRowBatch) {
+        float v[kRowBatch];
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j) v[j] = sig[min(k0 + j, nc - 1)][i];
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j) {
+          if (k0 + j < nc && ((valid >> (k0 + j)) & 1u)) {
+            const double d = (double)v[j];
+            s1 += d;
+            s2 += d * d;
+          }
+        }
+      }
+      sC[i] = (double)mv * s2 - s1 * s1;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double part = 0.0;
+      for (int i = tid; i < n_samples; i += 64) part += sC[i];
+      const double sum = wave_sum(part);
+      if (tid == 0) {
+        GD.cost_local = sum;
+        GD.m_local = (int16_t)mv;
+        GD.has_gram = 0;
+        GD.oob = oob;
+      }
+    }
+    return;
+  }
   double* C = W.gram + (((size_t)cb * n_jobs + m) * G2MAX + g) * kMaxPairs;
   const int npairs = nc * (nc + 1) / 2;
   for (int q = tid; q < npairs; q += kRefineBlock) {
@@ -702,14 +805,12 @@ __global__ __launch_bounds__(kRefineBlock) void k_refine_iter(
   // residual of this grid point if no earlier grid point of the iteration
   // flags another cut (the selection checks that and recomputes otherwise)
   if (tid < 64) {
-    const uint32_t oob = (uint32_t)__ballot(tid < nc && oobk[tid] != 0);
-    const uint32_t valid = ~(S.sticky | oob) & cut_bits(nc);
-    const int mv = __popc(valid);
     const double part = tid < nc ? gram_row(sC, valid, tid, nc, mv) : 0.0;
     const double sum = wave_sum(part);
     if (tid == 0) {
       GD.cost_local = sum;
-      GD.m_local = mv;
+      GD.m_local = (int16_t)mv;
+      GD.has_gram = 1;
       GD.oob = oob;
     }
   }

@@ -148,6 +148,7 @@ int cctagx_refine_grid_static(const cctagx_params* params, const uint8_t* gray, 
       d.begin_sig = r.begin_sig;
       d.end_sig = r.end_sig;
       d.n_iter = r.n_iter;
+      d.box_inside = box_inside(r, width, height) ? 1 : 0;
       d.n_cuts = m.n_cuts;
       d.c0x = m.center0[0];
       d.c0y = m.center0[1];
