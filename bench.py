@@ -474,8 +474,9 @@ def main():
         t_avg = oracle_lib.time_detect(img, iters, threads=threads)
         cpu = {"value": 1.0 / t_avg, "unit": "frames/s", "cores": threads, "kind": "port",
                "sample": "%d+1 full detections (front end, candidates, identification) of the "
-                         "same 1920x1080 frame on %d threads (front end and identification "
-                         "serial, candidate stage parallel)" % (iters, threads),
+                         "same 1920x1080 frame on %d threads (front end serial; candidate "
+                         "stage and the identification of the candidates parallel)"
+                         % (iters, threads),
                "single_thread_value": 1.0 / t_serial,
                "cpu_model": cpu_model(), "nproc": os.cpu_count()}
         progress("CPU baseline done: %.2f frames/s on %d threads" % (cpu["value"], threads))

@@ -193,7 +193,8 @@ const std::vector<std::vector<float>>& bank_of(int n_crowns) {
 // The oracle identification of the candidate stage's markers, written back
 // into them.
 void identify_markers(std::vector<cctagx::Marker>& markers, const uint8_t* img, int w, int h,
-                      const cctagx_params& params, const std::vector<std::vector<float>>& bank) {
+                      const cctagx_params& params, const std::vector<std::vector<float>>& bank,
+                      cctagx::TaskPool* pool = nullptr) {
   std::vector<oracle::IdentMarker> im(markers.size());
   for (size_t i = 0; i < markers.size(); ++i) {
     const cctagx::Marker& m = markers[i];
@@ -206,7 +207,7 @@ void identify_markers(std::vector<cctagx::Marker>& markers, const uint8_t* img, 
     o.center_y = m.center_img.y;
     o.scale = m.scale;
   }
-  oracle::identify(img, w, h, ident_params(params), bank, im);
+  oracle::identify(img, w, h, ident_params(params), bank, im, pool);
   for (size_t i = 0; i < markers.size(); ++i) {
     cctagx::Marker& m = markers[i];
     const oracle::IdentMarker& o = im[i];
@@ -263,7 +264,8 @@ int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w
     std::vector<cctagx::Marker> markers, kept;
     const cctagx::HostParams hp = cctagx::host_params(*params);
     cctagx::frame_candidates(level_edges(levels), hp, pool.get(), markers);
-    if (hp.do_identification) identify_markers(markers, levels[0].img.data(), w, h, *params, bank);
+    if (hp.do_identification)
+      identify_markers(markers, levels[0].img.data(), w, h, *params, bank, pool.get());
     cctagx::finish_markers(markers, kept);
     for (size_t i = 0; i < kept.size() && (int)i < max_out; ++i) cctagx::to_c_marker(kept[i], out[i]);
     return (int)kept.size();

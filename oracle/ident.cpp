@@ -27,9 +27,12 @@
 // tag::mean the lazy sum / count.
 #include "oracle_ident.hpp"
 
+#include "../cctag_amd/csrc/host/pool.hpp"
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <functional>
 #include <limits>
 #include <map>
 #include <stdexcept>
@@ -765,20 +768,35 @@ int grid_iterations(float a, float b, float neighbour_size, int grid_n) {
 }
 
 void identify(const uint8_t* img, int w, int h, const IdentParams& p,
-              const std::vector<std::vector<float>>& bank, std::vector<IdentMarker>& markers) {
-  std::vector<std::vector<Cut>> selected(markers.size());
-  std::vector<int> detected(markers.size(), -1);
-  for (size_t i = 0; i < markers.size(); ++i) {
+              const std::vector<std::vector<float>>& bank, std::vector<IdentMarker>& markers,
+              cctagx::TaskPool* pool) {
+  const int n = (int)markers.size();
+  auto for_each = [&](const std::function<void(int)>& fn) {
+    if (pool && pool->threads() > 1) pool->run(n, fn);
+    else for (int i = 0; i < n; ++i) fn(i);
+  };
+  std::vector<std::vector<Cut>> selected(n);
+  std::vector<int> detected(n, -1);
+  for_each([&](int i) {
     detected[i] = step_1(markers[i], img, w, h, p, selected[i]);
     markers[i].stops.clear();
     for (const Cut& c : selected[i]) markers[i].stops.push_back(c.stop);
-  }
+  });
+  // the serial loop's first grid search (the first marker with selected
+  // cuts whose neighbourhood loop runs) sets the conditioner cache
   ConditionerCache cache;
-  for (size_t i = 0; i < markers.size(); ++i) {
+  for (int i = 0; i < n && !cache.set; ++i)
+    if (detected[i] == status::id_reliable &&
+        grid_iterations(markers[i].a, markers[i].b, p.imaged_center_neighbour_size,
+                        p.imaged_center_n_grid_sample) > 0) {
+      cache.set = true;
+      cache.mean_ab = mean_ab_of(markers[i].a, markers[i].b);
+    }
+  for_each([&](int i) {
     if (detected[i] == status::id_reliable)
       detected[i] = step_2(markers[i], selected[i], img, w, h, p, bank, cache);
     markers[i].status = detected[i];
-  }
+  });
 }
 
 }  // namespace oracle

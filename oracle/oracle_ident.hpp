@@ -8,6 +8,10 @@
 #include <cstdint>
 #include <vector>
 
+namespace cctagx {
+class TaskPool;
+}
+
 namespace oracle {
 
 // DirectedPoint2d (x, y, gradient).
@@ -49,8 +53,13 @@ struct IdentMarker {
 // first).  conditionerFromEllipse caches (a + b) / 2 of the first ellipse
 // it conditions in the process (conditioner.hpp:27); here the cache lives
 // for one call: the first marker that reaches the grid search sets it.
+// With a pool, both steps run the markers on its threads: the first step's
+// markers are independent, and the second step's share only that cache,
+// which is set first from the marker the serial loop would reach the grid
+// search with first -- the same results as one thread.
 void identify(const uint8_t* img, int w, int h, const IdentParams& p,
-              const std::vector<std::vector<float>>& bank, std::vector<IdentMarker>& markers);
+              const std::vector<std::vector<float>>& bank, std::vector<IdentMarker>& markers,
+              cctagx::TaskPool* pool = nullptr);
 
 // Ellipse.cpp:259-298: the points sorted by their angle about the ellipse
 // centre, then sub-sampled to at most `requested`.

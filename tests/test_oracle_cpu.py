@@ -359,10 +359,14 @@ def test_oracle_detection_recovers_ground_truth():
 
 
 @pytest.mark.parametrize("threads", [2, 5])
-def test_candidate_stage_independent_of_thread_count(threads):
-    """The host candidate stage runs as independent tasks; the product runs
-    it on a thread pool, the oracle serially: both must agree exactly."""
-    img, _ = frames.config_frame(1)
+@pytest.mark.parametrize("frame", ["720p", "border"])
+def test_candidate_stage_independent_of_thread_count(threads, frame):
+    """The host candidate stage runs as independent tasks, and the oracle's
+    identification runs its markers on the same pool (the conditioner scale
+    fixed first from the marker the serial loop would reach the grid search
+    with first): every thread count must agree exactly with one thread, also
+    on the frame whose markers touch the border (out-of-bounds cuts)."""
+    img, _ = frames.config_frame(1) if frame == "720p" else frames.border_frame()
     assert oracle_lib.detect(img, threads=threads) == oracle_lib.detect(img)
 
 
