@@ -6,6 +6,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <algorithm>
 #include <exception>
 #include <stdexcept>
@@ -129,10 +132,12 @@ double oracle_time_front_end(const uint8_t* img, int w, int h, int n_levels, int
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Full CPU detection: the oracle front end and the oracle identification
-// around the shared host candidate stage and de-duplication.
+// Full CPU detection: the oracle front end, candidate stage, identification
+// and de-duplication.  The library's host candidate stage (pipeline.hpp) is
+// linked only as the thing under test of oracle_candidate_mismatch.
 #include "../cctag_amd/csrc/host/cabi.hpp"
 #include "../cctag_amd/csrc/host/pipeline.hpp"
+#include "oracle_cand.hpp"
 #include "oracle_ident.hpp"
 
 namespace {
@@ -151,6 +156,21 @@ std::vector<cctagx::LevelEdges> level_edges(const std::vector<oracle::LevelResul
     out.push_back(e);
   }
   return out;
+}
+
+oracle::CandidateParams candidate_params(const cctagx_params& c) {
+  oracle::CandidateParams p;
+  p.average_vote_min = c.average_vote_min;
+  p.thr_median_distance_ellipse = c.thr_median_distance_ellipse;
+  p.maximum_nb_seeds = c.maximum_nb_seeds;
+  p.maximum_nb_candidates_loop_two = c.maximum_nb_candidates_loop_two;
+  p.n_crowns = c.n_crowns;
+  p.min_points_segment_candidate = c.min_points_segment_candidate;
+  p.thresh_robust_estimation_of_outer_ellipse = c.thresh_robust_estimation_of_outer_ellipse;
+  p.ellipse_growing_elliptic_hull_width = c.ellipse_growing_elliptic_hull_width;
+  p.window_size_on_inner_elliptic_segment = c.window_size_on_inner_elliptic_segment;
+  p.search_for_another_segment = c.search_for_another_segment != 0;
+  return p;
 }
 
 oracle::Params front_params(const cctagx_params& c) {
@@ -249,10 +269,78 @@ int oracle_set_bank(int n_crowns, const float* ratios, int rows, int cols) {
   return 0;
 }
 
+namespace {
+bool same_f(float a, float b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+bool same_ellipse(const cctagx::Ellipse& a, const cctagx::Ellipse& b) {
+  return same_f(a.cx(), b.cx()) && same_f(a.cy(), b.cy()) && same_f(a.a(), b.a()) &&
+         same_f(a.b(), b.b()) && same_f(a.angle(), b.angle());
+}
+bool same_points(const std::vector<cctagx::Pt2>& a, const std::vector<cctagx::Pt2>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (!same_f(a[i].x, b[i].x) || !same_f(a[i].y, b[i].y) || !same_f(a[i].dx, b[i].dx) ||
+        !same_f(a[i].dy, b[i].dy))
+      return false;
+  return true;
+}
+}  // namespace
+
+// The library's host candidate stage (the thing under test, on `threads`
+// threads) against the oracle's on the oracle front end's levels of one
+// frame, before identification: the same markers in the same order with
+// bit-identical ellipses, qualities and point lists.  0 when identical,
+// 1 with a description in msg, -1 on an error.
+int oracle_candidate_mismatch(const cctagx_params* params, const uint8_t* img, int w, int h,
+                              int threads, int* n_markers, char* msg, int msg_len) {
+  auto say = [&](const std::string& t) {
+    if (msg && msg_len > 0) std::snprintf(msg, (size_t)msg_len, "%s", t.c_str());
+  };
+  try {
+    std::vector<oracle::LevelResult> levels;
+    oracle::front_end(img, w, h, front_params(*params), levels);
+    std::unique_ptr<cctagx::TaskPool> pool;
+    if (threads > 1) pool = std::make_unique<cctagx::TaskPool>(threads);
+    std::vector<cctagx::Marker> lib, ref;
+    cctagx::frame_candidates(level_edges(levels), cctagx::host_params(*params), pool.get(), lib);
+    oracle::candidates(levels, candidate_params(*params), ref, nullptr);
+    if (n_markers) *n_markers = (int)ref.size();
+    if (lib.size() != ref.size()) {
+      say("marker count " + std::to_string(lib.size()) + " vs oracle " + std::to_string(ref.size()));
+      return 1;
+    }
+    for (size_t i = 0; i < ref.size(); ++i) {
+      const cctagx::Marker &a = lib[i], &b = ref[i];
+      std::string what;
+      if (a.pyramid_level != b.pyramid_level || !same_f(a.scale, b.scale)) what = "level";
+      else if (!same_ellipse(a.outer_ellipse, b.outer_ellipse)) what = "outer ellipse";
+      else if (!same_ellipse(a.rescaled_outer_ellipse, b.rescaled_outer_ellipse)) what = "rescaled ellipse";
+      else if (!same_f(a.center_img.x, b.center_img.x) || !same_f(a.center_img.y, b.center_img.y)) what = "centre";
+      else if (!same_f(a.quality, b.quality)) what = "quality";
+      else if (a.points.size() != b.points.size()) what = "circle count";
+      else if (!same_points(a.rescaled_outer_ellipse_points, b.rescaled_outer_ellipse_points)) what = "rescaled points";
+      else
+        for (size_t k = 0; k < a.points.size() && what.empty(); ++k)
+          if (!same_points(a.points[k], b.points[k])) what = "points of circle " + std::to_string(k);
+      if (!what.empty()) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, " (level %d, centre %.3f %.3f vs %.3f %.3f)", b.pyramid_level,
+                      a.outer_ellipse.cx(), a.outer_ellipse.cy(), b.outer_ellipse.cx(), b.outer_ellipse.cy());
+        say("marker " + std::to_string(i) + ": " + what + buf);
+        return 1;
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    say(e.what());
+    return -1;
+  }
+}
+
 // cctagDetection on the CPU; returns the marker count or -1.  `threads` > 1
-// runs the shared host candidate stage on a pool of that many threads (the
-// reference's tbb::parallel_for loops), 1 serially; the front end and the
-// identification are serial either way.
+// runs the independent tasks of the candidate stage (loops two and three,
+// the level-0 re-fit) and the identification of the markers on a pool of
+// that many threads (the reference's tbb::parallel_for loops), 1 serially;
+// the front end and loop one are serial either way.
 int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w, int h,
                           cctagx_marker* out, int max_out, int threads) {
   try {
@@ -262,11 +350,10 @@ int oracle_detect_threads(const cctagx_params* params, const uint8_t* img, int w
     std::unique_ptr<cctagx::TaskPool> pool;
     if (threads > 1) pool = std::make_unique<cctagx::TaskPool>(threads);
     std::vector<cctagx::Marker> markers, kept;
-    const cctagx::HostParams hp = cctagx::host_params(*params);
-    cctagx::frame_candidates(level_edges(levels), hp, pool.get(), markers);
-    if (hp.do_identification)
+    oracle::candidates(levels, candidate_params(*params), markers, pool.get());
+    if (params->do_identification)
       identify_markers(markers, levels[0].img.data(), w, h, *params, bank, pool.get());
-    cctagx::finish_markers(markers, kept);
+    oracle::deduplicate(markers, kept);
     for (size_t i = 0; i < kept.size() && (int)i < max_out; ++i) cctagx::to_c_marker(kept[i], out[i]);
     return (int)kept.size();
   } catch (const std::exception&) {
