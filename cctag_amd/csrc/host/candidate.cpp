@@ -796,9 +796,11 @@ static bool is_another_segment(const EdgeSet& es, Ellipse& outerEllipse, std::ve
     rng.rand_5_k(perm, anotherOuter.size());
     for (int i = 0; i < 4; ++i)
       points.emplace_back((float)es.pts[anotherOuter[perm[i]]].x, (float)es.pts[anotherOuter[perm[i]]].y);
+    // the fit is outside the try (Vote.cpp:646-647): a degenerate fit ends
+    // the search and, through the caller, the candidate
+    Ellipse eToto;
+    fit_ellipse(points, eToto);
     try {
-      Ellipse eToto;
-      fit_ellipse(points, eToto);
       Ellipse q(eToto.matrix());
       const float ratio = q.a() / q.b();
       if ((ratio < 0.12) || (ratio > 8)) {
