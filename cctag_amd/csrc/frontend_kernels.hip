@@ -425,45 +425,72 @@ __device__ void lds_merge(int* L, int a, int b) {
   } while (!done);
 }
 
+// The tile-local joins by runs: a tile row is one wavefront's 64-bit
+// candidate mask (a ballot); a candidate's parent is the first pixel of its
+// run (its maximal horizontal segment of candidates), the run's root; then
+// the first pixel of each run joins every run of the row above that touches
+// the run 8-connectedly (the row above masked to the run's columns widened
+// by one on each side).  The components, and so their smallest indices, are
+// those the per-pixel joins give.
+static_assert(kStencilTileW == 64, "a tile row is one wavefront");
 __global__ __launch_bounds__(256) void k_cc_local(const PlaneDesc* __restrict__ P, int np,
                                                    const uint8_t* __restrict__ cls,
                                                    int32_t* __restrict__ lab) {
   constexpr int TW = kStencilTileW, TH = kStencilTileH, N = TW * TH;
+  constexpr int kWaves = 256 / 64;
   __shared__ int L[N];
-  __shared__ uint8_t C[N];
+  __shared__ uint64_t M[TH];
   const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
   const PlaneDesc d = P[p];
   const int t = blockIdx.x - d.tile_s;
   const int tx_n = (d.w + TW - 1) / TW;
   const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
-  for (int k = threadIdx.x; k < N; k += 256) {
-    const int x = x0 + (k % TW), y = y0 + k / TW;
-    const uint8_t c = (x < d.w && y < d.h) ? (cls[d.pix + y * d.w + x] & 1) : 0;
-    C[k] = c;
-    L[k] = c ? k : -1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = x0 + lane;
+  for (int ly = wave; ly < TH; ly += kWaves) {
+    const int y = y0 + ly;
+    const bool c = x < d.w && y < d.h && (cls[d.pix + y * d.w + x] & 1);
+    const uint64_t m = __ballot(c);
+    if (lane == 0) M[ly] = m;
+    // the run's first pixel: one past the highest non-candidate below
+    const uint64_t gaps = ~m & ((1ull << lane) - 1ull);
+    const int first = gaps ? 64 - __clzll(gaps) : 0;
+    L[ly * TW + lane] = c ? ly * TW + first : -1;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < N; k += 256) {
-    if (!C[k]) continue;
-    const int lx = k % TW, ly = k / TW;
-    if (lx > 0 && C[k - 1]) lds_merge(L, k, k - 1);
-    if (ly > 0) {
-      if (lx > 0 && C[k - TW - 1]) lds_merge(L, k, k - TW - 1);
-      if (C[k - TW]) lds_merge(L, k, k - TW);
-      if (lx + 1 < TW && C[k - TW + 1]) lds_merge(L, k, k - TW + 1);
+  for (int ly = 1 + wave; ly < TH; ly += kWaves) {
+    const uint64_t m = M[ly], up = M[ly - 1];
+    const bool starts = ((m >> lane) & 1ull) && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+    if (!starts || !up) continue;
+    const uint64_t rest = ~(m >> lane);  // bit j clear: lane + j is in the run
+    const int len = rest ? __ffsll((long long)rest) - 1 : 64;
+    const int lo = lane > 0 ? lane - 1 : 0, hi = min(TW - 1, lane + len);
+    const int width = hi - lo + 1;
+    const uint64_t window = width == 64 ? ~0ull : ((1ull << width) - 1ull) << lo;
+    uint64_t touch = up & window;  // one segment per run above
+    while (touch) {
+      const int b = __ffsll((long long)touch) - 1;
+      lds_merge(L, ly * TW + lane, (ly - 1) * TW + b);
+      const uint64_t seg = ~(touch >> b);
+      const int n = seg ? __ffsll((long long)seg) - 1 : 64 - b;
+      touch = b + n >= 64 ? 0ull : touch & (~0ull << (b + n));
     }
   }
   __syncthreads();
   for (int k = threadIdx.x; k < N; k += 256) {
-    if (!C[k]) continue;
+    if (L[k] < 0) continue;
     const int r = lds_find(L, k);
     lab[d.pix + (y0 + k / TW) * d.w + x0 + k % TW] = d.pix + (y0 + r / TW) * d.w + x0 + r % TW;
   }
 }
 
-// The neighbour pairs that cross a tile edge: left / up-left on the tile's
-// first column, up-left / up / up-right on its first row, up-right on its
-// last column.  One lane per edge pixel.
+// The neighbour pairs that cross a tile edge, joined in global memory.  The
+// tile's first row against the row above by runs, one wavefront: each run
+// of the first row joins every run of the row above that touches it
+// 8-connectedly (its columns widened by one, into the neighbouring tiles'
+// columns at the ends), and the first pixel joins its left neighbour.  The
+// other rows pixel by pixel, one lane each: left and up-left on the first
+// column, up-right on the last.
 __global__ __launch_bounds__(128) void k_cc_border(const PlaneDesc* __restrict__ P, int np,
                                                    const uint8_t* __restrict__ cls,
                                                    int32_t* __restrict__ lab) {
@@ -474,29 +501,51 @@ __global__ __launch_bounds__(128) void k_cc_border(const PlaneDesc* __restrict__
   const int tx_n = (d.w + TW - 1) / TW;
   const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
   const int i = threadIdx.x;
-  int lx, ly;
-  if (i < TW) {
-    lx = i;
-    ly = 0;
-  } else if (i < TW + TH - 1) {
-    lx = 0;
-    ly = i - TW + 1;
-  } else if (i < TW + 2 * (TH - 1)) {
-    lx = TW - 1;
-    ly = i - TW - TH + 2;
-  } else {
+  if (i < 64) {
+    const int x = x0 + i;
+    const int g = d.pix + y0 * d.w + x;  // row y0 < d.h: the tile exists
+    const bool in = x < d.w;
+    const bool c = in && (cls[g] & 1);
+    if (i == 0 && c && x0 > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
+    if (y0 == 0) return;
+    const uint64_t m = __ballot(c);
+    const uint64_t up = __ballot(in && (cls[g - d.w] & 1));
+    const int u0 = g - i - d.w;  // row above, column x0
+    const bool up_left = x0 > 0 && (cls[u0 - 1] & 1);
+    const bool up_right = x0 + TW < d.w && (cls[u0 + TW] & 1);
+    const bool starts = c && (i == 0 || !((m >> (i - 1)) & 1ull));
+    if (!starts) return;
+    const uint64_t rest = ~(m >> i);  // bit j clear: i + j is in the run
+    const int len = rest ? __ffsll((long long)rest) - 1 : 64;
+    const int lo = i > 0 ? i - 1 : 0, hi = min(TW - 1, i + len);
+    const int width = hi - lo + 1;
+    const uint64_t window = width == 64 ? ~0ull : ((1ull << width) - 1ull) << lo;
+    uint64_t touch = up & window;  // one segment per run above
+    while (touch) {
+      const int b = __ffsll((long long)touch) - 1;
+      uf_merge(lab, g, u0 + b);
+      const uint64_t seg = ~(touch >> b);
+      const int n = seg ? __ffsll((long long)seg) - 1 : 64 - b;
+      touch = b + n >= 64 ? 0ull : touch & (~0ull << (b + n));
+    }
+    if (i == 0 && up_left) uf_merge(lab, g, u0 - 1);
+    if (i + len >= TW && up_right) uf_merge(lab, g, u0 + TW);
     return;
   }
+  // rows 1 .. TH-1 of the first column (j < TH-1) and of the last (j >= TH-1)
+  const int j = i - 64;
+  if (j >= 2 * (TH - 1)) return;
+  const bool first_col = j < TH - 1;
+  const int lx = first_col ? 0 : TW - 1, ly = first_col ? j + 1 : j - (TH - 1) + 1;
   const int x = x0 + lx, y = y0 + ly;
   if (x >= d.w || y >= d.h) return;
   const int g = d.pix + y * d.w + x;
   if (!(cls[g] & 1)) return;
-  if (lx == 0 && x > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
-  if (y > 0) {
-    const int u = g - d.w;
-    if ((lx == 0 || ly == 0) && x > 0 && (cls[u - 1] & 1)) uf_merge(lab, g, u - 1);
-    if (ly == 0 && (cls[u] & 1)) uf_merge(lab, g, u);
-    if ((ly == 0 || lx == TW - 1) && x + 1 < d.w && (cls[u + 1] & 1)) uf_merge(lab, g, u + 1);
+  if (first_col) {
+    if (x > 0 && (cls[g - 1] & 1)) uf_merge(lab, g, g - 1);
+    if (x > 0 && (cls[g - d.w - 1] & 1)) uf_merge(lab, g, g - d.w - 1);
+  } else if (x + 1 < d.w && (cls[g - d.w + 1] & 1)) {
+    uf_merge(lab, g, g - d.w + 1);
   }
 }
 
