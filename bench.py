@@ -87,8 +87,7 @@ def frontend_bytes(name, lv):
             "cc_local": N + 4 * N,                         # cls in, labels out
             "cc_border": 0,                                # tile edges only
             "cc_flatten": N + 4 * N + 4 * N + N,           # cls, labels in/out, root flag
-            "thin1": N + 4 * N + N + 2 * N,                # cls, labels, roots; map + pass-1 out
-            "thin2": 2 * N,                                # pass-1 in, thinned out
+            "thin": N + 4 * N + N + N,                     # cls, labels, roots in; thinned map out
             "compact_count": N,                            # thinned map
             "compact_scan": 0,
             "compact_write": N + 4 * E + 4 * N + 12 * E + 4 * E,  # map, grads at edges; emap, list, flags

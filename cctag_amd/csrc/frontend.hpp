@@ -74,7 +74,6 @@ struct FrontEndBuffers {
   int32_t* lab;        // Npix: union-find parent of candidate pixels
   uint8_t* root_strong;// Npix: component root holds a strong pixel
   uint8_t* edges;      // Npix: Canny map, then thinned in place
-  uint8_t* tmp;        // Npix: thinning pass-1 output
   uint32_t* ebits;     // Npix/32: edge bitmask, bit i of word j = pixel 32j+i (plane starts are 256-aligned)
   int32_t* erank;      // Npix/32: index (raster rank in the plane) of word j's first edge pixel
   int32_t* tcount;     // compaction chunks: count, then exclusive offset

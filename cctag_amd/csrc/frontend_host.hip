@@ -85,7 +85,6 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   const size_t o_lab = arena_.reserve(4 * N);
   const size_t o_rs = arena_.reserve(N);
   const size_t o_edges = arena_.reserve(N);
-  const size_t o_tmp = arena_.reserve(N);
   const size_t o_ebits = arena_.reserve(N / 8 + 4);
   const size_t o_erank = arena_.reserve(N / 8 + 4);
   const size_t o_tcount = arena_.reserve(4 * (size_t)std::max(ps.n_tiles_c, 1));
@@ -120,7 +119,6 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   b.lab = arena_.at<int32_t>(o_lab);
   b.root_strong = arena_.at<uint8_t>(o_rs);
   b.edges = arena_.at<uint8_t>(o_edges);
-  b.tmp = arena_.at<uint8_t>(o_tmp);
   b.ebits = arena_.at<uint32_t>(o_ebits);
   b.erank = arena_.at<int32_t>(o_erank);
   b.tcount = arena_.at<int32_t>(o_tcount);

@@ -598,25 +598,29 @@ __device__ __forceinline__ int thin_index(const uint8_t (*s)[LW], int ly, int lx
          (s[ly + 1][lx + 1] == 255) * 256;
 }
 
-// Canny map e = 255 for candidates whose component holds a strong pixel;
-// writes the map (kept on the border) and thinning pass 1 (interior only;
-// the temporary's border is 0).
-__global__ __launch_bounds__(256) void k_thin1(const PlaneDesc* __restrict__ P, int np,
-                                                const uint8_t* __restrict__ cls,
-                                                const int32_t* __restrict__ lab,
-                                                const uint8_t* __restrict__ root_strong,
-                                                uint8_t* __restrict__ edges,
-                                                uint8_t* __restrict__ tmp) {
-  constexpr int TW = kStencilTileW, TH = kStencilTileH, LW = TW + 2;
-  __shared__ uint8_t se[TH + 2][LW];
+// Canny map e = 255 for candidates whose component holds a strong pixel,
+// then both thinning passes in one block: pass 1 over the tile and a one-pixel
+// ring around it (from the map with a two-pixel ring; interior only, the
+// temporary's border is 0), pass 2 over the tile.  Border pixels keep the
+// Canny map's value, interior pixels take pass 2's.
+__global__ __launch_bounds__(256) void k_thin(const PlaneDesc* __restrict__ P, int np,
+                                               const uint8_t* __restrict__ cls,
+                                               const int32_t* __restrict__ lab,
+                                               const uint8_t* __restrict__ root_strong,
+                                               uint8_t* __restrict__ edges) {
+  constexpr int TW = kStencilTileW, TH = kStencilTileH;
+  constexpr int EW = TW + 4, EH = TH + 4;  // the map: tile + 2-pixel ring
+  constexpr int TWW = TW + 2, THH = TH + 2;  // pass 1: tile + 1-pixel ring
+  __shared__ uint8_t se[EH][EW];
+  __shared__ uint8_t st[THH][TWW];
   const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
   const PlaneDesc d = P[p];
   const int t = blockIdx.x - d.tile_s;
   const int tx_n = (d.w + TW - 1) / TW;
   const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
-  for (int k = threadIdx.x; k < (TH + 2) * LW; k += 256) {
-    const int ly = k / LW, lx = k - ly * LW;
-    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+  for (int k = threadIdx.x; k < EH * EW; k += 256) {
+    const int ly = k / EW, lx = k - ly * EW;
+    const int gx = x0 + lx - 2, gy = y0 + ly - 2;
     uint8_t e = 0;
     if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) {
       const int g = d.pix + gy * d.w + gx;
@@ -625,49 +629,24 @@ __global__ __launch_bounds__(256) void k_thin1(const PlaneDesc* __restrict__ P, 
     se[ly][lx] = e;
   }
   __syncthreads();
+  for (int k = threadIdx.x; k < THH * TWW; k += 256) {
+    const int ly = k / TWW, lx = k - ly * TWW;
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    uint8_t o = 0;
+    if (gx > 0 && gy > 0 && gx < d.w - 1 && gy < d.h - 1 && se[ly + 1][lx + 1] != 0)
+      o = lut_bit(0, thin_index<EW>(se, ly + 1, lx + 1)) ? 255 : 0;
+    st[ly][lx] = o;
+  }
+  __syncthreads();
   const int tx = threadIdx.x & 63;
   const int x = x0 + tx;
   for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
     const int y = y0 + ty;
     if (x >= d.w || y >= d.h) continue;
-    const int g = d.pix + y * d.w + x;
-    const uint8_t e = se[ty + 1][tx + 1];
-    edges[g] = e;
-    uint8_t o = 0;
-    if (e != 0 && x > 0 && y > 0 && x < d.w - 1 && y < d.h - 1)
-      o = lut_bit(0, thin_index<LW>(se, ty + 1, tx + 1)) ? 255 : 0;
-    tmp[g] = o;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_thin2(const PlaneDesc* __restrict__ P, int np,
-                                                const uint8_t* __restrict__ tmp,
-                                                uint8_t* __restrict__ edges) {
-  constexpr int TW = kStencilTileW, TH = kStencilTileH, LW = TW + 2;
-  __shared__ uint8_t st[TH + 2][LW];
-  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
-  const PlaneDesc d = P[p];
-  const int t = blockIdx.x - d.tile_s;
-  const int tx_n = (d.w + TW - 1) / TW;
-  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
-  for (int k = threadIdx.x; k < (TH + 2) * LW; k += 256) {
-    const int ly = k / LW, lx = k - ly * LW;
-    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-    uint8_t v = 0;
-    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) v = tmp[d.pix + gy * d.w + gx];
-    st[ly][lx] = v;
-  }
-  __syncthreads();
-  const int tx = threadIdx.x & 63;
-  const int x = x0 + tx;
-  if (x <= 0 || x >= d.w - 1) return;
-  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
-    const int y = y0 + ty;
-    if (y <= 0 || y >= d.h - 1) continue;
-    const int g = d.pix + y * d.w + x;
-    uint8_t o = 0;
-    if (st[ty + 1][tx + 1] != 0) o = lut_bit(1, thin_index<LW>(st, ty + 1, tx + 1)) ? 255 : 0;
-    edges[g] = o;
+    uint8_t o = se[ty + 2][tx + 2];
+    if (x > 0 && y > 0 && x < d.w - 1 && y < d.h - 1)
+      o = st[ty + 1][tx + 1] != 0 && lut_bit(1, thin_index<TWW>(st, ty + 1, tx + 1)) ? 255 : 0;
+    edges[d.pix + y * d.w + x] = o;
   }
 }
 
@@ -1326,10 +1305,8 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
   mark("cc_border");
   k_cc_flatten<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong);
   mark("cc_flatten");
-  k_thin1<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong, b.edges, b.tmp);
-  mark("thin1");
-  k_thin2<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.tmp, b.edges);
-  mark("thin2");
+  k_thin<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong, b.edges);
+  mark("thin");
   k_compact_count<<<L.n_tiles_c, 256, 0, s>>>(P, np, b.edges, b.tcount, b.ebits);
   mark("compact_count");
   k_compact_scan<<<np, 1024, 0, s>>>(P, np, b.tcount, b.counts);
