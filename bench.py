@@ -83,8 +83,7 @@ def frontend_bytes(name, lv):
         per = {
             "pyramid": N + (4 * N if l == 0 else 0),       # level 0 read + its quads written, coarser levels written
             "filter": N + 4 * N,                           # u8 in, dx/dy int16 out
-            "nms": 4 * N + N + 4 * N + N,                  # dx,dy in; cls, lab, root flag out
-            "cc_local": N + 4 * N,                         # cls in, labels out
+            "nms_local": 4 * N + N + 4 * N + N,            # dx,dy in; cls, labels, root flag out
             "cc_border": 0,                                # tile edges only
             "cc_flatten": N + 4 * N + 4 * N + N,           # cls, labels in/out, root flag
             "thin": N + 4 * N + N + N,                     # cls, labels, roots in; thinned map out

@@ -300,59 +300,29 @@ __device__ __forceinline__ int magnitude(int x, int y) {
   return (int)rintf(sqrt_rn((float)x * (float)x + (float)y * (float)y));
 }
 
-__global__ __launch_bounds__(256) void k_nms(const PlaneDesc* __restrict__ P, int np,
-                                              const int16_t* __restrict__ dx,
-                                              const int16_t* __restrict__ dy, int low, int high,
-                                              uint8_t* __restrict__ cls, int32_t* __restrict__ lab,
-                                              uint8_t* __restrict__ root_strong) {
-  constexpr int TW = kStencilTileW, TH = kStencilTileH;
-  __shared__ int smag[TH + 2][TW + 2];
-  const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
-  const PlaneDesc d = P[p];
-  const int t = blockIdx.x - d.tile_s;
-  const int tx_n = (d.w + TW - 1) / TW;
-  const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
-  for (int k = threadIdx.x; k < (TH + 2) * (TW + 2); k += 256) {
-    const int ly = k / (TW + 2), lx = k - ly * (TW + 2);
-    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-    int m = 0;
-    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) {
-      const int g = d.pix + gy * d.w + gx;
-      m = magnitude(dx[g], dy[g]);
-    }
-    smag[ly][lx] = m;
+// The class of pixel (tx, ty) of a tile from its gradient and the tile's
+// magnitudes with a one-pixel ring.
+constexpr int TG22 = 13573;
+__device__ __forceinline__ uint8_t nms_class(int sx, int sy,
+                                             const int (*smag)[kStencilTileW + 2], int tx,
+                                             int ty, int low, int high) {
+  const int m = smag[ty + 1][tx + 1];
+  const int ax = sx < 0 ? -sx : sx;
+  int ay = sy < 0 ? -sy : sy;
+  if (m <= low) return 0;
+  const int tg22x = ax * TG22;
+  const int tg67x = tg22x + ((ax + ax) << 15);
+  ay <<= 15;
+  bool pass;
+  if (ay < tg22x) {
+    pass = m > smag[ty + 1][tx] && m >= smag[ty + 1][tx + 2];
+  } else if (ay > tg67x) {
+    pass = m > smag[ty][tx + 1] && m >= smag[ty + 2][tx + 1];
+  } else {
+    const int s = (sx ^ sy) < 0 ? -1 : 1;
+    pass = m > smag[ty][tx + 1 - s] && m > smag[ty + 2][tx + 1 + s];
   }
-  __syncthreads();
-  const int tx = threadIdx.x & 63;
-  const int x = x0 + tx;
-  constexpr int TG22 = 13573;
-  for (int ty = threadIdx.x >> 6; ty < TH; ty += 4) {
-    const int y = y0 + ty;
-    if (x >= d.w || y >= d.h) continue;
-    const int g = d.pix + y * d.w + x;
-    const int sx = dx[g], sy = dy[g];
-    const int m = smag[ty + 1][tx + 1];
-    const int ax = sx < 0 ? -sx : sx;
-    int ay = sy < 0 ? -sy : sy;
-    uint8_t c = 0;
-    if (m > low) {
-      const int tg22x = ax * TG22;
-      const int tg67x = tg22x + ((ax + ax) << 15);
-      ay <<= 15;
-      bool pass;
-      if (ay < tg22x) {
-        pass = m > smag[ty + 1][tx] && m >= smag[ty + 1][tx + 2];
-      } else if (ay > tg67x) {
-        pass = m > smag[ty][tx + 1] && m >= smag[ty + 2][tx + 1];
-      } else {
-        const int s = (sx ^ sy) < 0 ? -1 : 1;
-        pass = m > smag[ty][tx + 1 - s] && m > smag[ty + 2][tx + 1 + s];
-      }
-      if (pass) c = (uint8_t)(m > high ? 3 : 1);
-    }
-    cls[g] = c;
-    root_strong[g] = 0;
-  }
+  return pass ? (uint8_t)(m > high ? 3 : 1) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -389,7 +359,7 @@ __device__ void uf_merge(int32_t* L, int a, int b) {
   } while (!done);
 }
 
-// Two levels: k_cc_local joins the candidates of one 64x16 tile in LDS and
+// Two levels: k_nms_local joins the candidates of one 64x16 tile in LDS and
 // labels each with the global index of its tile-local root (the smallest
 // index of its tile-local component, since local and global raster orders
 // agree inside a tile); k_cc_border then joins across tile edges in global
@@ -425,19 +395,25 @@ __device__ void lds_merge(int* L, int a, int b) {
   } while (!done);
 }
 
-// The tile-local joins by runs: a tile row is one wavefront's 64-bit
-// candidate mask (a ballot); a candidate's parent is the first pixel of its
-// run (its maximal horizontal segment of candidates), the run's root; then
-// the first pixel of each run joins every run of the row above that touches
-// the run 8-connectedly (the row above masked to the run's columns widened
-// by one on each side).  The components, and so their smallest indices, are
-// those the per-pixel joins give.
+// Magnitude, NMS and the tile-local joins in one block per 64x16 tile: the
+// classes (written, with the strong-root flags cleared) and then the joins
+// by runs: a tile row is one wavefront's 64-bit candidate mask (a ballot); a
+// candidate's parent is the first pixel of its run (its maximal horizontal
+// segment of candidates), the run's root; then the first pixel of each run
+// joins every run of the row above that touches it 8-connectedly (the row
+// above masked to the run's columns widened by one on each side).  The
+// components, and so their smallest indices, are those the per-pixel joins
+// give.
 static_assert(kStencilTileW == 64, "a tile row is one wavefront");
-__global__ __launch_bounds__(256) void k_cc_local(const PlaneDesc* __restrict__ P, int np,
-                                                   const uint8_t* __restrict__ cls,
-                                                   int32_t* __restrict__ lab) {
+__global__ __launch_bounds__(256) void k_nms_local(const PlaneDesc* __restrict__ P, int np,
+                                                    const int16_t* __restrict__ dx,
+                                                    const int16_t* __restrict__ dy, int low,
+                                                    int high, uint8_t* __restrict__ cls,
+                                                    int32_t* __restrict__ lab,
+                                                    uint8_t* __restrict__ root_strong) {
   constexpr int TW = kStencilTileW, TH = kStencilTileH, N = TW * TH;
   constexpr int kWaves = 256 / 64;
+  __shared__ int smag[TH + 2][TW + 2];
   __shared__ int L[N];
   __shared__ uint64_t M[TH];
   const int p = plane_of<F_TILE_S>(P, np, blockIdx.x);
@@ -445,11 +421,29 @@ __global__ __launch_bounds__(256) void k_cc_local(const PlaneDesc* __restrict__ 
   const int t = blockIdx.x - d.tile_s;
   const int tx_n = (d.w + TW - 1) / TW;
   const int x0 = (t % tx_n) * TW, y0 = (t / tx_n) * TH;
+  for (int k = threadIdx.x; k < (TH + 2) * (TW + 2); k += 256) {
+    const int ly = k / (TW + 2), lx = k - ly * (TW + 2);
+    const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+    int m = 0;
+    if (gx >= 0 && gx < d.w && gy >= 0 && gy < d.h) {
+      const int g = d.pix + gy * d.w + gx;
+      m = magnitude(dx[g], dy[g]);
+    }
+    smag[ly][lx] = m;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = x0 + lane;
   for (int ly = wave; ly < TH; ly += kWaves) {
     const int y = y0 + ly;
-    const bool c = x < d.w && y < d.h && (cls[d.pix + y * d.w + x] & 1);
+    bool c = false;
+    if (x < d.w && y < d.h) {
+      const int g = d.pix + y * d.w + x;
+      const uint8_t k = nms_class(dx[g], dy[g], smag, lane, ly, low, high);
+      cls[g] = k;
+      root_strong[g] = 0;
+      c = k & 1;
+    }
     const uint64_t m = __ballot(c);
     if (lane == 0) M[ly] = m;
     // the run's first pixel: one past the highest non-candidate below
@@ -1296,11 +1290,9 @@ int launch_front_end(const FrontEndLaunch& L, hipStream_t s, StageTimer* timer) 
     else k_filter<8><<<nt, 256, 0, s>>>(P, np, fs, b.img, b.dx, b.dy);
   }
   mark("filter");
-  k_nms<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.dx, b.dy, L.prm.low, L.prm.high, b.cls, b.lab,
-                                    b.root_strong);
-  mark("nms");
-  k_cc_local<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab);
-  mark("cc_local");
+  k_nms_local<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.dx, b.dy, L.prm.low, L.prm.high, b.cls,
+                                          b.lab, b.root_strong);
+  mark("nms_local");
   k_cc_border<<<L.n_tiles_s, 128, 0, s>>>(P, np, b.cls, b.lab);
   mark("cc_border");
   k_cc_flatten<<<L.n_tiles_s, 256, 0, s>>>(P, np, b.cls, b.lab, b.root_strong);
