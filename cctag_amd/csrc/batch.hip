@@ -23,7 +23,7 @@ using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
 }
-constexpr int kEdgeBlockChoices[] = {64, 128, 256, 512, 1024};
+constexpr int kEdgeBlockChoices[] = {64, 128, 256, 512, 1024, 2048};
 }  // namespace
 
 struct cctagx_batch {
@@ -86,7 +86,7 @@ int autotune(cctagx_batch* b) {
     return 0;
   };
   int fv = b->filter_variant >= 0 ? b->filter_variant : 1;
-  int eb = b->edge_blocks > 0 ? b->edge_blocks : 512;
+  int eb = b->edge_blocks > 0 ? b->edge_blocks : b->fe.launch().edge_blocks;
   if (b->filter_variant < 0) {
     static const char* const kF[] = {"filter"};
     float best = 1e30f;

@@ -37,7 +37,7 @@ def test_batch_matches_plain_per_frame():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fv,eb", [(0, 64), (1, 512), (2, 1024), (2, 128)])
+@pytest.mark.parametrize("fv,eb", [(0, 64), (1, 512), (2, 1024), (2, 128), (1, 2048)])
 def test_batch_partitions_agree(fv, eb):
     imgs = frames.small_batch()
     ref = plain(imgs)
