@@ -2,6 +2,7 @@
 // prepared, launches the identification kernels once per step for all
 // markers and unpacks the results.
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -111,8 +112,10 @@ void GpuIdentBackend::pack_select(const std::vector<CutJob>& jobs) {
   }
 }
 
-void GpuIdentBackend::unpack_select(int n, std::vector<CutSelection>& out) const {
-  const SelectOutDev* ho = h_sel_out_.as<SelectOutDev>();
+void GpuIdentBackend::unpack_select(int n, std::vector<CutSelection>& out) {
+  host_sel_.resize(n);
+  std::memcpy(host_sel_.data(), h_sel_out_.as<SelectOutDev>(), sizeof(SelectOutDev) * n);
+  const SelectOutDev* ho = host_sel_.data();
   out.assign(n, CutSelection());
   for (int i = 0; i < n; ++i) {
     out[i].n_collected = ho[i].n_collected;
@@ -158,6 +161,7 @@ int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs,
   RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
   const int nb = k_.bank_rows;
   int n_dist = 0;
+  dist_offset_.resize(jobs.size());
   for (size_t i = 0; i < jobs.size(); ++i) {
     const RefineJob& r = jobs[i];
     RefineJobDev& d = hj[i];
@@ -192,7 +196,9 @@ int GpuIdentBackend::pack_refine_jobs(const std::vector<RefineJob>& jobs,
       n_dist += d.n_cuts * nb;
       for (size_t k = 0; k < r.stops.size(); ++k) d.stops[k] = make_float2(r.stops[k].x, r.stops[k].y);
     }
+    dist_offset_[i] = d.dist_offset;
   }
+  dist_total_ = n_dist;
   return n_dist;
 }
 
@@ -264,10 +270,17 @@ void GpuIdentBackend::refine_gradient(const std::vector<RefineJob>& jobs, const 
 }
 
 void GpuIdentBackend::unpack_refine(const std::vector<RefineJob>& jobs,
-                                    std::vector<RefineResult>& out) const {
-  const RefineOutDev* ho = h_ref_out_.as<RefineOutDev>();
-  const float* hd = h_ref_dist_.as<float>();
-  const RefineJobDev* hj = h_ref_jobs_.as<RefineJobDev>();
+                                    std::vector<RefineResult>& out) {
+  // the headers and the distances leave the mapped buffers in one bulk copy
+  // each (field-by-field reads from them took most of the unpacking time)
+  const size_t nj = jobs.size();
+  host_out_.resize(nj);
+  std::memcpy(host_out_.data(), h_ref_out_.as<RefineOutDev>(), sizeof(RefineOutDev) * nj);
+  host_dist_.resize((size_t)dist_total_);
+  if (dist_total_ > 0)
+    std::memcpy(host_dist_.data(), h_ref_dist_.as<float>(), sizeof(float) * (size_t)dist_total_);
+  const RefineOutDev* ho = host_out_.data();
+  const float* hd = host_dist_.data();
   const int nb = k_.bank_rows;
   out.assign(jobs.size(), RefineResult());
   for (size_t i = 0; i < jobs.size(); ++i) {
@@ -284,7 +297,7 @@ void GpuIdentBackend::unpack_refine(const std::vector<RefineJob>& jobs,
     r.bar_max = o.bar_max;
     r.oob.assign(o.oob, o.oob + nc);
     r.id_dist.assign((size_t)nc * nb, 0.f);
-    const float* src = hd + hj[i].dist_offset;
+    const float* src = hd + dist_offset_[i];
     for (int k = 0; k < nc; ++k)
       if (!r.oob[k]) std::copy(src + (size_t)k * nb, src + (size_t)(k + 1) * nb, &r.id_dist[(size_t)k * nb]);
   }

@@ -164,8 +164,15 @@ class GpuIdentBackend : public IdentBackend {
   // device (its distances then at a fixed stride of kMaxCuts rows)
   int pack_refine_jobs(const std::vector<RefineJob>& jobs, const std::vector<int>* sel_of = nullptr);
   void pack_select(const std::vector<CutJob>& jobs);
-  void unpack_select(int n, std::vector<CutSelection>& out) const;
-  void unpack_refine(const std::vector<RefineJob>& jobs, std::vector<RefineResult>& out) const;
+  void unpack_select(int n, std::vector<CutSelection>& out);
+  void unpack_refine(const std::vector<RefineJob>& jobs, std::vector<RefineResult>& out);
+  // host copies of the mapped result buffers, and where each job's distances
+  // start in them (set by pack_refine_jobs)
+  std::vector<SelectOutDev> host_sel_;
+  std::vector<RefineOutDev> host_out_;
+  std::vector<float> host_dist_;
+  std::vector<int> dist_offset_;
+  int dist_total_ = 0;
   hipStream_t stream_ = nullptr;
   std::vector<DeviceImage> images_;
   const DeviceImage& image(int i) const;
