@@ -39,6 +39,26 @@ def edge_case(name: str) -> np.ndarray:
         y, x = np.mgrid[0:256, 0:256]
         r = np.hypot(x - 127.3, y - 121.8)
         return np.where((r.astype(int) // 9) % 2 == 0, 40, 210).astype(np.uint8)
+    if name == "spiral":
+        # a band along a spiral through many 64x16 tiles, 4 DN above the
+        # background (weak edges, m ~ 3) except one 40-degree arc at +60 DN
+        # (strong): the whole band's edges come from one strong arc far away
+        y, x = np.mgrid[0:300, 0:420]
+        dx, dy = x - 211.5, y - 148.5
+        r, a = np.hypot(dx, dy), np.arctan2(dy, dx)
+        turn = (r - 12.0 * (a + np.pi) / (2 * np.pi)) / 12.0
+        band = np.abs(turn - np.round(turn)) < 0.22
+        img = np.full(x.shape, 128, np.int32)
+        img[band & (r > 10)] = 132
+        img[band & (r > 100) & (r < 125) & (np.abs(a - 0.6) < 0.35)] = 188
+        return img.astype(np.uint8)
+    if name == "stripes":
+        # diagonal and near-horizontal stripes: rows of candidates that fill
+        # whole 64-pixel tile rows and join across tile edges diagonally
+        y, x = np.mgrid[0:200, 0:330]
+        a = ((x + y) // 3) % 2
+        b = ((7 * x + y) // 4) % 2
+        return np.where((x < 165) ^ (y < 100), 60 + 130 * a, 70 + 120 * b).astype(np.uint8)
     raise KeyError(name)
 
 

@@ -67,7 +67,7 @@ def test_front_end_parity_small_markers():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["blank", "noise", "checker", "rings", "tiny"])
+@pytest.mark.parametrize("name", ["blank", "noise", "checker", "rings", "tiny", "spiral", "stripes"])
 def test_front_end_parity_edge_cases(name):
     img = frames.edge_case(name)
     print("front-end parity edge case", name, img.shape, flush=True)
