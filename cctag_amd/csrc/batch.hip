@@ -1,6 +1,7 @@
 // Batched detection (include/cctag_amd.h, cctagx_batch_*): one plane set
 // for every level of every frame, the front end captured once in a hipGraph,
 // the host candidate stage parallel over frames, identification batched.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -262,8 +263,17 @@ int cctagx_batch_detect(cctagx_batch* b, cctagx_marker* out, int max_out, int32_
   try {
     t = Clock::now();
     const int nl = b->n_levels;
-    // frames in parallel, each frame's tasks serial on its worker
-    parallel_for(b->pool.get(), b->n_frames, [&](int f) {
+    // frames in parallel, each frame's tasks serial on its worker; the
+    // largest frames are handed out first (their candidate stages are the
+    // longest, so starting them last would leave the other workers idle)
+    std::vector<int> order(b->n_frames);
+    for (int f = 0; f < b->n_frames; ++f) order[f] = f;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+      return (long long)b->sizes[a].first * b->sizes[a].second >
+             (long long)b->sizes[c].first * b->sizes[c].second;
+    });
+    parallel_for(b->pool.get(), b->n_frames, [&](int k) {
+      const int f = order[k];
       std::vector<LevelEdges> lv(b->views.begin() + (size_t)f * nl,
                                  b->views.begin() + (size_t)(f + 1) * nl);
       frame_candidates(lv, b->host, nullptr, b->cand[f], nullptr, &b->workspace[f]);
