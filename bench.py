@@ -203,6 +203,31 @@ def gt_centre_errors(markers, gts):
     return errs, ids_ok
 
 
+def other_configs(Detector, synth, steps=10):
+    """The plain drop-in call on the other single-frame configurations: the
+    1280x720 frame with 4 markers and the 3840x2160 frame with 32 markers and
+    200 distractors; frames/s with the upload, and the markers against the
+    synthetic ground truth."""
+    out = {}
+    for cfg, name in ((1, "1280x720, 4 markers"), (3, "3840x2160, 32 markers, 200 distractors")):
+        img, gts = synth.config_frame(cfg, 0)
+        h, w = img.shape
+        det = Detector(w, h)
+        for _ in range(2):
+            ref = det.detect(img)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            got = det.detect(img)
+        dt = (time.perf_counter() - t0) / steps
+        det.close()
+        errs, ids_ok = gt_centre_errors(got, gts)
+        out[name] = {"frames_per_s": 1.0 / dt, "ms_per_frame": 1e3 * dt,
+                     "identical_across_steps": got == ref, "identified": len(errs),
+                     "ids_correct": ids_ok, "gt_markers": len(gts),
+                     "center_err_px_mean": (sum(errs) / len(errs)) if errs else None}
+    return out
+
+
 def gradient_comparison(frames, Detector, BatchDetector, steps, reps=3):
     """1920x1080 frames with 16 markers: the plain path frame by frame
     against the batched path, with the reference grid search and with the
@@ -320,6 +345,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipes", action="store_true",
                     help="skip the concurrent-pipes comparison")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the 1280x720 and 3840x2160 plain-path rates")
     ap.add_argument("--batch-frames", type=int, default=64,
                     help="frames of the uneven-size batch (0 skips the batched comparison)")
     ap.add_argument("--refine-markers", type=int, default=4096,
@@ -486,6 +513,8 @@ def main():
     gradient = gradient_comparison(side.get("hd") or [], Detector, BatchDetector,
                                    args.gradient_steps)
     progress("batched gradient comparison done")
+    others = None if args.no_other_configs else other_configs(Detector, synth)
+    progress("other configurations done")
 
     line = {
         "metric": "1080p frames/s (16 markers), plain pipeline",
@@ -517,6 +546,7 @@ def main():
         "grad_refine": refine,
         "batched_gradient": gradient,
         "concurrent_pipes": pipes,
+        "other_configs": others,
     }
     det.close()
     print(json.dumps(line))
