@@ -103,14 +103,19 @@ __device__ __forceinline__ void fill_inv(double* inv) {
 
 // Running mean and variance, one value at a time (at most kMaxSamples).
 // The reciprocals a step needs are read from fill_inv's table one step
-// ahead, so the table's latency stays off the chain of divisions.
+// ahead, so the table's latency stays off the chain of products.  With
+// EXACT false a product in the subnormal range (where it need not round to
+// the quotient) is only recorded in `tiny`, keeping branches off the chain:
+// the caller then runs the same values through MeanVar<true>, which divides
+// there, so the result is always the division's.
+template <bool EXACT>
 struct MeanVar {
   float mean = 0.f, var = 0.f;
   unsigned cnt = 0;
+  bool tiny = false;
   const double* inv;
   double inv_cnt, inv_prev;  // 1/(cnt + 1) and 1/cnt for the next add
   __device__ explicit MeanVar(const double* table) : inv(table), inv_cnt(table[1]), inv_prev(0.0) {}
-  // the subnormal fallbacks are branched to once per wavefront and step
   __device__ __forceinline__ void add(float v) {
     ++cnt;
     const double inv_next = inv[min(cnt + 1, (unsigned)kMaxSamples)];
@@ -118,16 +123,21 @@ struct MeanVar {
     const float xm = mean * before + v;
     const double pm = div_n_product(xm, inv_cnt);
     mean = (float)pm;
-    if (__builtin_expect(__ballot(div_n_tiny(pm)) != 0, 0))
+    if (EXACT) {
       if (div_n_tiny(pm)) mean = xm / (float)cnt;
+    } else {
+      tiny |= div_n_tiny(pm);
+    }
     if (cnt > 1) {
       const float t = v - mean;
       const float xa = var * before, xb = t * t;
       const double pa = div_n_product(xa, inv_cnt), pb = div_n_product(xb, inv_prev);
       float a = (float)pa, b = (float)pb;
-      if (__builtin_expect(__ballot(div_n_tiny(pa) || div_n_tiny(pb)) != 0, 0)) {
+      if (EXACT) {
         if (div_n_tiny(pa)) a = xa / (float)cnt;
         if (div_n_tiny(pb)) b = xb / before;
+      } else {
+        tiny |= div_n_tiny(pa) | div_n_tiny(pb);
       }
       var = a + b;
     }
@@ -226,10 +236,16 @@ __global__ __launch_bounds__(kSelectBlock) void k_select_cuts(const SelectJobDev
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
   if (t < n) {
     s = pts[J.pt_offset + t];
-    MeanVar mv(inv);
+    MeanVar<false> mv(inv);
     valid = cut_walk(im, J.cx, J.cy, s.x, s.y, start_sig, 1.f, n_samples,
                      [&](int, float v) { mv.add(v); });
     var = mv.var;
+    if (valid && mv.tiny) {  // the same walk again, dividing where a product was subnormal
+      MeanVar<true> me(inv);
+      cut_walk(im, J.cx, J.cy, s.x, s.y, start_sig, 1.f, n_samples,
+               [&](int, float v) { me.add(v); });
+      var = me.var;
+    }
   }
   // number of collected cuts and their largest variance
   {
@@ -933,7 +949,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
     // so the reads are in flight together), then consumed in order
     constexpr int kStatBatch = 8;
     const float* sg = sig[tid];
-    MeanVar mv(inv);
+    MeanVar<false> mv(inv);
     for (int i0 = 30; i0 < n_samples; i0 += kStatBatch) {
       float v[kStatBatch];
 #pragma unroll
@@ -941,6 +957,13 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
 #pragma unroll
       for (int j = 0; j < kStatBatch; ++j)
         if (i0 + j < n_samples) mv.add(v[j]);
+    }
+    float s_mean = mv.mean, s_var = mv.var;
+    if (mv.tiny) {  // again, dividing where a product was subnormal
+      MeanVar<true> me(inv);
+      for (int i = 30; i < n_samples; ++i) me.add(sg[i]);
+      s_mean = me.mean;
+      s_var = me.var;
     }
     float sumInf = 0, sumSup = 0;
     unsigned nInf = 0, nSup = 0;
@@ -954,7 +977,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
 #pragma unroll
       for (int j = 0; j < kStatBatch; ++j) {
         const bool live = i0 + j < n_samples;
-        const bool lo = live && v[j] < mv.mean;
+        const bool lo = live && v[j] < s_mean;
         acc = acc || lo;
         const bool take_lo = acc && lo, take_hi = live && acc && !lo;
         sumInf += take_lo ? v[j] : 0.f;
@@ -963,7 +986,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
         nSup += take_hi;
       }
     }
-    stat_var[tid] = mv.var;
+    stat_var[tid] = s_var;
     stat_muw[tid] = sumSup / (float)nSup;
     stat_mub[tid] = sumInf / (float)nInf;
   } else if (tid == kFinalBlock - 64) {
