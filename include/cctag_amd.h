@@ -334,8 +334,8 @@ int cctagx_batch_get_stats(cctagx_batch* b, cctagx_batch_stats* out);
  * adjoint kernel pair returns the score and its gradient; a batched descent
  * moves every marker's imaged centre c along -dC/dc with H = H(c) the closed
  * form of computeHomographyFromEllipseAndImagedCenter (:739-798); the
- * reference's grid search runs on the same markers as the fallback and the
- * judge.  No counterpart in the reference. */
+ * reference's grid search runs on the same markers as the fallback and as
+ * the result to compare with.  No counterpart in the reference. */
 typedef struct cctagx_refiner cctagx_refiner;
 typedef struct cctagx_refine_marker {
   float ellipse[5];     /* outer ellipse: cx, cy, a, b, angle (image coordinates) */

@@ -7,7 +7,7 @@ reference's detection path, read from its sources as data.
     (src/cctag/filter/thinning.cpp:12-13)
   * bank3 / bank4: radius-ratio banks (src/cctag/CCTagMarkersBank.cpp:120-282)
 
-Run once in the build container (the reference tree is not on the GPU box):
+Run once where the reference tree is available (the tests only read the JSON):
     python tests/golden/make_reference_tables.py /root/reference
 The committed JSON is what the tests read.
 """
