@@ -145,6 +145,7 @@ class DetectStats(ctypes.Structure):
         ("loop_three_us", c_f32), ("rescale_us", c_f32), ("ident_us", c_f32),
         ("finish_us", c_f32), ("ident_prep_us", c_f32), ("ident_select_us", c_f32),
         ("ident_refine_us", c_f32), ("ident_post_us", c_f32),
+        ("staged_bytes", ctypes.c_int64), ("staged_two_step", c_i32),
     ]
 
     def as_dict(self) -> dict:

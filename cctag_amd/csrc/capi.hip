@@ -62,6 +62,7 @@ struct cctagx_handle {
   std::unique_ptr<TaskPool> pool;  // host candidate-stage workers
   double host_ms = 0;  // last detection: host wall time after the front end
   double stage_ms = 0; // last detection: wall time of the level staging copies
+  bool staged_two_step = false;  // last detection: the levels did not fit the one-step gather
   HostTimes host_times;  // last detection: host stages
   FrameWorkspace workspace;  // host edge sets reused across detections
   bool have_frame = false;
@@ -298,12 +299,26 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   if (!h->have_frame) return CCTAGX_ERR_STATE;
   *n_out = 0;
   h->timer.reset();
+  // the staging gather lays the levels out from the device counts and
+  // follows the front end without a host round trip (in its graph)
+  const bool planned = (int)h->fe.planes().planes.size() <= kMaxPlannedPlanes;
+  if (planned && h->stager.reserve_planned(h->fe.planes())) {
+    set_error("pinned staging buffer allocation failed");
+    return CCTAGX_ERR_HIP;
+  }
+  auto enqueue_gather = [&]() {
+    return planned ? h->stager.enqueue_planned(h->fe.planes(), h->fe.launch().dev_planes,
+                                               h->fe.buffers(), h->stream)
+                   : 0;
+  };
   if (h->timer.per_kernel) {
     if (int rc = h->fe.run(h->stream, &h->timer)) return rc;
+    if (enqueue_gather()) return CCTAGX_ERR_HIP;
   } else {
     if (!h->fe_exec) {
       CCTAGX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = h->fe.run(h->stream, nullptr);
+      int rc = h->fe.run(h->stream, nullptr);
+      if (rc == 0 && enqueue_gather()) rc = CCTAGX_ERR_HIP;
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(h->stream, &g);
       if (rc != 0 || e != hipSuccess) {
@@ -321,7 +336,20 @@ int cctagx_detect_loaded(cctagx_handle* h, uint64_t frame, cctagx_marker* out, i
   h->have_front_end = true;
   h->level_loaded.assign(h->level_loaded.size(), false);
   const auto ts = std::chrono::steady_clock::now();
-  if (int rc = stage_levels(h)) return rc;
+  int staged = planned ? h->stager.finish_planned(h->fe.planes(), h->stream, h->views) : 1;
+  if (staged < 0) return CCTAGX_ERR_HIP;
+  h->staged_two_step = staged == 1;
+  if (staged == 1) {
+    // levels larger than the planned buffer (or too many planes): the
+    // two-step staging, and the graph captured again with the grown buffer
+    if (planned && h->fe_exec) {
+      hipGraphExecDestroy(h->fe_exec);
+      hipGraphDestroy(h->fe_graph);
+      h->fe_exec = nullptr;
+      h->fe_graph = nullptr;
+    }
+    if (int rc = stage_levels(h)) return rc;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   h->stage_ms = std::chrono::duration<double, std::milli>(t0 - ts).count();
   h->ident.clear_stats();
@@ -393,6 +421,8 @@ int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* st) {
     st->front_end_us = ms * 1000.f;
   }
   st->stage_us = (float)(h->stage_ms * 1000.0);
+  st->staged_bytes = (int64_t)h->stager.last_bytes();
+  st->staged_two_step = h->staged_two_step ? 1 : 0;
   st->host_us = (float)(h->host_ms * 1000.0);
   st->select_us = h->ident.last_select_us();
   st->refine_us = h->ident.last_refine_us();

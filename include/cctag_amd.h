@@ -280,6 +280,12 @@ typedef struct cctagx_detect_stats {
    * the refinement are one backend call, counted in ident_refine_us;
    * ident_select_us is zero */
   float ident_prep_us, ident_select_us, ident_refine_us, ident_post_us;
+  /* bytes of candidate-stage input copied to the host, and 1 when the frame
+   * was staged in two steps (its counts read back, then the gather) because
+   * its levels did not fit the buffer of the one-step gather, which then
+   * grows for the next frame */
+  int64_t staged_bytes;
+  int32_t staged_two_step;
 } cctagx_detect_stats;
 int cctagx_get_detect_stats(cctagx_handle* h, cctagx_detect_stats* out);
 

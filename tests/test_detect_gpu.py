@@ -170,3 +170,33 @@ def test_frame_too_small_for_the_pyramid():
     assert e.value.rc == -6
     with Detector(8, 8) as det:
         assert det.detect(np.full((8, 8), 128, np.uint8)) == []
+
+
+@pytest.mark.gpu
+def test_staging_fallback_then_grown_buffer():
+    """A frame whose levels do not fit the one-step gather's first buffer
+    (noise: about 10 staged bytes per plane pixel) is staged in two steps;
+    the next detection uses the grown buffer in the captured front end, and
+    both equal the oracle; a following marker frame keeps its result."""
+    img = frames.edge_case("noise")
+    h, w = img.shape
+    ref = oracle_lib.detect(img)
+    with Detector(w, h) as det:
+        a = det.detect(img)
+        s1 = det.detect_stats()
+        b = det.detect(img)
+        s2 = det.detect_stats()
+        print("staged bytes", s1["staged_bytes"], s2["staged_bytes"], flush=True)
+        assert s1["staged_two_step"] == 1 and s2["staged_two_step"] == 0
+        assert s1["staged_bytes"] == s2["staged_bytes"] > 0
+        c = det.detect(img)
+    match(a, ref)
+    assert a == b == c
+    marker_img, _ = frames.small_marker_frame()
+    mh, mw = marker_img.shape
+    with Detector(mw, mh) as det:
+        first = det.detect(marker_img)
+        assert det.detect_stats()["staged_two_step"] == 0
+        for _ in range(2):
+            assert det.detect(marker_img) == first
+    match(first, oracle_lib.detect(marker_img))

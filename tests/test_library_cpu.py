@@ -4,6 +4,7 @@ documented (defaults of Params.hpp:27-66, argument errors, no device)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -89,3 +90,36 @@ def test_create_without_device_fails_cleanly():
     assert lib.cctagx_create(ctypes.byref(p), 64, 48, ctypes.byref(h)) == -5
     assert not h.value
     assert b"device" in lib.cctagx_last_error()
+
+
+MIRRORS = [("Params", "cctagx_params"), ("LevelView", "cctagx_level_view"),
+           ("Marker", "cctagx_marker"), ("IdentMarker", "cctagx_ident_marker"),
+           ("IdentResult", "cctagx_ident_result"), ("DetectStats", "cctagx_detect_stats"),
+           ("BatchStats", "cctagx_batch_stats"), ("RefineMarker", "cctagx_refine_marker")]
+
+
+def test_ctypes_mirrors_match_the_c_layout(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler
+    gives the header's struct (a probe built with gcc against the header)."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cctag_amd.h"',
+             'int main(void) {']
+    for py, c in MIRRORS:
+        lines.append(f'  printf("{py} size %zu\\n", sizeof({c}));')
+        for name, _ in getattr(_lib, py)._fields_:
+            lines.append(f'  printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
+                    str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    c_layout = {}
+    for line in out.splitlines():
+        py, name, v = line.split()
+        c_layout[(py, name)] = int(v)
+    for py, _ in MIRRORS:
+        cls = getattr(_lib, py)
+        assert c_layout[(py, "size")] == ctypes.sizeof(cls), py
+        for name, _ in cls._fields_:
+            assert c_layout[(py, name)] == getattr(cls, name).offset, (py, name)
