@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <limits>
 #include <utility>
 
@@ -330,20 +332,39 @@ Pt2 point_on_ellipse(const Ellipse& e, const Pt2& p) {
 
 void sorted_outer_points(const Ellipse& e, const std::vector<Pt2>& points, std::vector<Pt2>& out,
                          size_t requested) {
-  std::vector<std::pair<float, size_t>> ang;
-  ang.reserve(points.size());
-  for (size_t i = 0; i < points.size(); ++i)
-    ang.emplace_back(std::atan2(points[i].y - e.cy(), points[i].x - e.cx()), i);
-  std::sort(ang.begin(), ang.end());
-  const size_t n_outer = std::min(requested, points.size());
-  const float step = std::max(1.f, (float)points.size() / (float)(n_outer - 1));
+  // The points in the order a sort of (atan2 angle about the centre, index)
+  // pairs gives, by a stable least-significant-digit radix sort of the
+  // indices on the angles' bit patterns made monotone (-0 folded onto +0,
+  // which compares equal to it; atan2 of finite coordinates is never NaN),
+  // then every step-th of them.
+  const size_t n = points.size();
   out.clear();
+  if (n == 0) return;
+  std::vector<uint32_t> key(n), idx(n), tmp(n);
+  for (size_t i = 0; i < n; ++i) {
+    float a = std::atan2(points[i].y - e.cy(), points[i].x - e.cx());
+    if (a == 0.f) a = 0.f;
+    uint32_t b;
+    std::memcpy(&b, &a, 4);
+    key[i] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    idx[i] = (uint32_t)i;
+  }
+  for (int shift = 0; shift < 32; shift += 8) {
+    size_t count[257] = {};
+    for (size_t i = 0; i < n; ++i) ++count[((key[i] >> shift) & 255u) + 1];
+    if (count[((key[0] >> shift) & 255u) + 1] == n) continue;  // one digit value throughout
+    for (int d = 0; d < 256; ++d) count[d + 1] += count[d];
+    for (size_t i = 0; i < n; ++i) tmp[count[(key[idx[i]] >> shift) & 255u]++] = idx[i];
+    idx.swap(tmp);
+  }
+  const size_t n_outer = std::min(requested, n);
+  const float step = std::max(1.f, (float)n / (float)(n_outer - 1));
   for (size_t k = 0;; ++k) {
     const float f = k * step;
-    if (!(f < (float)ang.size())) break;
+    if (!(f < (float)n)) break;
     const size_t i = (size_t)f;
-    if (i >= ang.size()) break;
-    out.push_back(points[ang[i].second]);
+    if (i >= n) break;
+    out.push_back(points[idx[i]]);
   }
 }
 

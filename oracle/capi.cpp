@@ -336,6 +336,33 @@ int oracle_candidate_mismatch(const cctagx_params* params, const uint8_t* img, i
   }
 }
 
+// The library's ordering of a marker's outer points for the cut selection
+// (cctagx::sorted_outer_points, a radix sort of the angles) against the
+// oracle's (a sort of (angle, index) pairs) on the same points about the
+// centre (cx, cy): 0 when the two pick the same points in the same order.
+int oracle_outer_points_mismatch(const float* x, const float* y, int n, float cx, float cy,
+                                 int requested) {
+  std::vector<cctagx::Pt2> lib_in;
+  std::vector<oracle::DPt> ref_in;
+  for (int i = 0; i < n; ++i) {
+    lib_in.emplace_back(x[i], y[i]);
+    oracle::DPt d;
+    d.x = x[i];
+    d.y = y[i];
+    ref_in.push_back(d);
+  }
+  const cctagx::Ellipse e(cx, cy, 10.f, 5.f, 0.f);
+  if (e.cx() != cx || e.cy() != cy) return -1;
+  std::vector<cctagx::Pt2> lib_out;
+  std::vector<oracle::DPt> ref_out;
+  cctagx::sorted_outer_points(e, lib_in, lib_out, (size_t)requested);
+  oracle::sorted_outer_points(cx, cy, ref_in, ref_out, (size_t)requested);
+  if (lib_out.size() != ref_out.size()) return 1;
+  for (size_t i = 0; i < lib_out.size(); ++i)
+    if (lib_out[i].x != ref_out[i].x || lib_out[i].y != ref_out[i].y) return 1;
+  return 0;
+}
+
 // cctagDetection on the CPU; returns the marker count or -1.  `threads` > 1
 // runs the independent tasks of the candidate stage (loops two and three,
 // the level-0 re-fit) and the identification of the markers on a pool of
