@@ -362,19 +362,19 @@ void GpuIdentBackend::select_and_refine(const std::vector<CutJob>& cuts,
   pack_select(cuts);
   pack_refine_jobs(jobs, &sel_of);
   hipStream_t s = stream_;
-  hip_check(hipMemcpyAsync(d_ref_jobs_, h_ref_jobs_.as<RefineJobDev>(), sizeof(RefineJobDev) * n,
-                           hipMemcpyHostToDevice, s),
-            "H2D");
-  hip_check(hipEventRecord(jobs_copied_, s), "event");
-  jobs_in_flight_ = true;
   hipEventRecord(ev_[0], s);
   if (launch_select_cuts(h_sel_jobs_.dev<SelectJobDev>(), nc, h_pts_.dev<float4>(), k_.start_sig,
                          k_.n_samples, k_.n_refine_samples, p.num_cuts_in_ident_step,
                          h_sel_out_.dev<SelectOutDev>(), s) != 0)
     throw std::runtime_error("k_select_cuts launch failed");
   hipEventRecord(ev_[1], s);
-  if (launch_take_stops(d_ref_jobs_, n, h_sel_out_.dev<SelectOutDev>(), s) != 0)
+  // k_take_stops brings the refinement jobs over from mapped memory itself
+  // (no copy-engine transfer between the selection and the refinement)
+  if (launch_take_stops(h_ref_jobs_.dev<RefineJobDev>(), d_ref_jobs_, n,
+                        h_sel_out_.dev<SelectOutDev>(), s) != 0)
     throw std::runtime_error("k_take_stops launch failed");
+  hip_check(hipEventRecord(jobs_copied_, s), "event");
+  jobs_in_flight_ = true;
   hipEventRecord(ev_[2], s);
   int max_iter = 0;
   for (const RefineJob& r : jobs) max_iter = std::max(max_iter, r.n_iter);

@@ -55,9 +55,11 @@ struct RefineOutDev {
 
 int launch_select_cuts(const SelectJobDev* jobs, int n_jobs, const float4* pts, float start_sig,
                        int n_samples, int n_ref, int select_max, SelectOutDev* out, hipStream_t s);
-// Jobs with sel_index >= 0 take the stops of that selection (n_cuts = the
+// Copies the jobs from src (host memory the device maps) to jobs; those
+// with sel_index >= 0 take the stops of that selection (n_cuts = the
 // selected count; no iteration when it is zero).
-int launch_take_stops(RefineJobDev* jobs, int n_jobs, const SelectOutDev* sel, hipStream_t s);
+int launch_take_stops(const RefineJobDev* src, RefineJobDev* jobs, int n_jobs,
+                      const SelectOutDev* sel, hipStream_t s);
 // Grid-search state of one marker between iteration launches.
 static_assert(kMaxCuts <= 32, "one bit per cut in the flag words");
 struct RefineStateDev {

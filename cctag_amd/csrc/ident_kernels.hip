@@ -1093,8 +1093,15 @@ __global__ __launch_bounds__(kFinalBlock) void k_refine_final(
 // A refinement job takes the stops of the selection it names (one block of
 // 64 threads per job): its cut count is the selected count, and a job whose
 // selection is empty gets no iteration.
-__global__ __launch_bounds__(64) void k_take_stops(RefineJobDev* __restrict__ jobs,
+__global__ __launch_bounds__(64) void k_take_stops(const RefineJobDev* __restrict__ src,
+                                                   RefineJobDev* __restrict__ jobs,
                                                    const SelectOutDev* __restrict__ sel) {
+  static_assert(sizeof(RefineJobDev) % 4 == 0, "jobs copied as words");
+  constexpr int kWords = (int)(sizeof(RefineJobDev) / 4);
+  const uint32_t* from = reinterpret_cast<const uint32_t*>(src + blockIdx.x);
+  uint32_t* to = reinterpret_cast<uint32_t*>(jobs + blockIdx.x);
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) to[i] = from[i];
+  __syncthreads();
   RefineJobDev& J = jobs[blockIdx.x];
   const int s = J.sel_index;
   if (s < 0) return;
@@ -1144,9 +1151,10 @@ int launch_refine_readout(const RefineJobDev* jobs, int n_jobs, int n_samples, c
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_take_stops(RefineJobDev* jobs, int n_jobs, const SelectOutDev* sel, hipStream_t s) {
+int launch_take_stops(const RefineJobDev* src, RefineJobDev* jobs, int n_jobs,
+                      const SelectOutDev* sel, hipStream_t s) {
   if (n_jobs <= 0) return 0;
-  k_take_stops<<<n_jobs, 64, 0, s>>>(jobs, sel);
+  k_take_stops<<<n_jobs, 64, 0, s>>>(src, jobs, sel);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
