@@ -14,6 +14,7 @@ inline HostParams host_params(const cctagx_params& c) {
   p.maximum_nb_candidates_loop_two = c.maximum_nb_candidates_loop_two;
   p.n_crowns = c.n_crowns;
   p.min_points_segment_candidate = c.min_points_segment_candidate;
+  p.min_votes_to_select_candidate = c.min_votes_to_select_candidate;
   p.thresh_robust_estimation_of_outer_ellipse = c.thresh_robust_estimation_of_outer_ellipse;
   p.ellipse_growing_elliptic_hull_width = c.ellipse_growing_elliptic_hull_width;
   p.window_size_on_inner_elliptic_segment = c.window_size_on_inner_elliptic_segment;

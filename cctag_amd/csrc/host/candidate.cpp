@@ -115,7 +115,7 @@ void EdgeSet::rank_words(size_t w0, size_t w1, int32_t before) {
 }
 
 
-void EdgeSet::fill(const LevelEdges& L, int begin, int end) {
+void EdgeSet::fill(const LevelEdges& L, int begin, int end, int min_votes) {
   for (int i = begin; i < end; ++i) {
     EdgePoint& e = pts[i];
     e.x = L.x[i];
@@ -124,7 +124,8 @@ void EdgeSet::fill(const LevelEdges& L, int begin, int end) {
     e.gy = (float)L.gy[i];
     e.norm = std::sqrt(e.gx * e.gx + e.gy * e.gy);
     e.flow_length = L.flow[i];
-    e.is_max = L.ismax[i];
+    const int nv = L.vindex[i + 1] - L.vindex[i];
+    e.is_max = nv >= min_votes ? nv : -1;
     e.processed = 0;
     e.n_segment_out = -1;
   }

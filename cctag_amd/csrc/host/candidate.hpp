@@ -28,7 +28,6 @@ struct LevelEdges {
   const int32_t *before = nullptr, *after = nullptr;
   const int32_t *vindex = nullptr, *vlist = nullptr;  // CSR voter lists
   const float* flow = nullptr;
-  const int32_t* ismax = nullptr;
   int n_seeds = 0;
   const int32_t* seeds = nullptr;  // in the order they reached min votes
 };
@@ -46,12 +45,7 @@ struct EdgePoint {
 // EdgePointCollection (Types.hpp:19-158).
 class EdgeSet {
  public:
-  void load(const LevelEdges& L) {
-    prepare(L);
-    rank_words(0, n_words(), 0);
-    fill(L, 0, L.n);
-  }
-  // load() in steps: sizes and flags; the point index of the first edge of
+  // Loading a level, in steps: sizes and flags; the point index of the first edge of
   // every 32-pixel word of the bit mask, for words [w0, w1) given the edges
   // before w0 (word_edges counts the edges of a word range); the points
   // [begin, end).  Independent ranges may be done concurrently.
@@ -59,7 +53,9 @@ class EdgeSet {
   size_t n_words() const { return rank_.size(); }
   int32_t word_edges(size_t w0, size_t w1) const;
   void rank_words(size_t w0, size_t w1, int32_t before);
-  void fill(const LevelEdges& L, int begin, int end);
+  // Points [begin, end): is_max is the point's number of voters when it has
+  // at least min_votes, else -1 (Vote.cpp:236-240), read off the voter lists.
+  void fill(const LevelEdges& L, int begin, int end, int min_votes);
   // Point index at (x, y) or -1; the caller guarantees 0 <= x < shape_w and
   // 0 <= y < shape_h (the collection's shape is the level-0 size).
   int at(int x, int y) const {
@@ -103,6 +99,7 @@ struct HostParams {
   int maximum_nb_candidates_loop_two = 40;
   int n_crowns = 3;
   int min_points_segment_candidate = 10;
+  int min_votes_to_select_candidate = 3;
   float thresh_robust_estimation_of_outer_ellipse = 30.f;
   float ellipse_growing_elliptic_hull_width = 2.3f;
   int window_size_on_inner_elliptic_segment = 20;

@@ -62,7 +62,8 @@ void frame_candidates(const std::vector<LevelEdges>& levels, const HostParams& p
       return;
     }
     const int l = chunks[c - nr].first, b = chunks[c - nr].second;
-    es[l].fill(levels[l], b, std::min(b + kFillChunk, levels[l].n));
+    es[l].fill(levels[l], b, std::min(b + kFillChunk, levels[l].n),
+               p.min_votes_to_select_candidate);
   });
   if (times) times->load += ms_since(t);
   detect_candidates(es, level_h, p, pool, markers, times);

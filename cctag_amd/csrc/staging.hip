@@ -60,10 +60,9 @@ __global__ __launch_bounds__(256) void k_pack_planned(const PlaneDesc* __restric
       const void* s_[kStageArrays] = {b.ebits + (d.pix >> 5), b.ex + d.pix,     b.ey + d.pix,
                                       b.egx + d.pix,          b.egy + d.pix,    b.before + d.pix,
                                       b.after + d.pix,        b.vindex + d.pix + p,
-                                      b.vlist + d.pix,        b.flow + d.pix,   b.ismax + d.pix,
-                                      b.seeds + d.pix};
+                                      b.vlist + d.pix,        b.flow + d.pix,   b.seeds + d.pix};
       const uint64_t z[kStageArrays] = {4 * ((N + 31) / 32), 2 * n, 2 * n, 2 * n, 2 * n,
-                                        4 * n, 4 * n, 4 * (n + 1), 4 * nv, 4 * n, 4 * n, 4 * nsd};
+                                        4 * n, 4 * n, 4 * (n + 1), 4 * nv, 4 * n, 4 * nsd};
       for (int a = 0; a < kStageArrays; ++a, ++k) {
         src[k] = static_cast<const uint8_t*>(s_[a]);
         bytes[k] = z[a];
@@ -158,9 +157,8 @@ int LevelStager::finish_planned(const PlaneSet& ps, hipStream_t s, std::vector<L
     v.vindex = reinterpret_cast<const int32_t*>(base + sp.off[7]);
     v.vlist = reinterpret_cast<const int32_t*>(base + sp.off[8]);
     v.flow = reinterpret_cast<const float*>(base + sp.off[9]);
-    v.ismax = reinterpret_cast<const int32_t*>(base + sp.off[10]);
     v.n_seeds = sp.counts.n_seeds;
-    v.seeds = reinterpret_cast<const int32_t*>(base + sp.off[11]);
+    v.seeds = reinterpret_cast<const int32_t*>(base + sp.off[10]);
   }
   return 0;
 }
@@ -188,7 +186,7 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
   std::vector<CopySeg> segs;
   segs.reserve((size_t)count * 16);
   struct Off {
-    uint64_t bits, ex, ey, egx, egy, before, after, vindex, vlist, flow, ismax, seeds;
+    uint64_t bits, ex, ey, egx, egy, before, after, vindex, vlist, flow, seeds;
   };
   std::vector<Off> offs(count);
   uint64_t at = 0;
@@ -216,7 +214,6 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
     o.vindex = add(b.vindex + d.pix + l, 4 * (n + 1));
     o.vlist = add(b.vlist + d.pix, 4 * nv);
     o.flow = add(b.flow + d.pix, 4 * n);
-    o.ismax = add(b.ismax + d.pix, 4 * n);
     o.seeds = add(b.seeds + d.pix, 4 * nsd);
   }
   const uint64_t total = std::max<uint64_t>(at, 16);
@@ -257,7 +254,6 @@ int LevelStager::stage(const PlaneSet& ps, const FrontEndBuffers& b, hipStream_t
     v.vindex = reinterpret_cast<const int32_t*>(base + o.vindex);
     v.vlist = reinterpret_cast<const int32_t*>(base + o.vlist);
     v.flow = reinterpret_cast<const float*>(base + o.flow);
-    v.ismax = reinterpret_cast<const int32_t*>(base + o.ismax);
     v.n_seeds = cnt[i].n_seeds;
     v.seeds = reinterpret_cast<const int32_t*>(base + o.seeds);
   }

@@ -29,7 +29,7 @@ int launch_pack(const CopySeg* segs, int n_segs, uint8_t* dst, hipStream_t s);
 // no counts read-back and can be captured with the front end.  The buffer
 // starts with a header: the bytes in all, whether they fit, and per plane
 // its counts and each array's offset from the data start (kStageDataStart).
-constexpr int kStageArrays = 12;  // bits, x, y, gx, gy, before, after, vindex, vlist, flow, ismax, seeds
+constexpr int kStageArrays = 11;  // bits, x, y, gx, gy, before, after, vindex, vlist, flow, seeds
 constexpr int kMaxPlannedPlanes = 16;
 struct StagePlane {
   PlaneCounts counts;

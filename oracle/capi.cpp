@@ -151,7 +151,7 @@ std::vector<cctagx::LevelEdges> level_edges(const std::vector<oracle::LevelResul
     e.x = L.ex.data(); e.y = L.ey.data(); e.gx = L.edx.data(); e.gy = L.edy.data();
     e.before = L.before.data(); e.after = L.after.data();
     e.vindex = L.voters_index.data(); e.vlist = L.voters_list.data();
-    e.flow = L.flow_length.data(); e.ismax = L.is_max.data();
+    e.flow = L.flow_length.data();
     e.n_seeds = (int)L.seeds.size(); e.seeds = L.seeds.data();
     out.push_back(e);
   }
