@@ -179,7 +179,7 @@ static int check_frame(const cctagx_handle* h, const uint8_t* gray, int width, i
 
 int cctagx_load(cctagx_handle* h, const uint8_t* gray, int width, int height, size_t stride) {
   if (int rc = check_frame(h, gray, width, height, stride)) return rc;
-  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream, h->pool.get())) return rc;
   h->have_frame = true;
   h->have_front_end = false;
   return CCTAGX_OK;
@@ -460,7 +460,7 @@ int cctagx_destroy(cctagx_handle* h) {
 int cctagx_front_end(cctagx_handle* h, const uint8_t* gray, int width, int height,
                      size_t stride) {
   if (int rc = check_frame(h, gray, width, height, stride)) return rc;
-  if (int rc = h->fe.upload(0, gray, stride, h->stream)) return rc;
+  if (int rc = h->fe.upload(0, gray, stride, h->stream, h->pool.get())) return rc;
   h->have_frame = true;
   h->timer.reset();
   if (int rc = h->fe.run(h->stream, &h->timer)) return rc;

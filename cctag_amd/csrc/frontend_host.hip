@@ -183,7 +183,15 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   return 0;
 }
 
-int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s) {
+FrontEndDevice::~FrontEndDevice() {
+  if (staged_sent_) {
+    hipEventSynchronize(staged_sent_);
+    hipEventDestroy(staged_sent_);
+  }
+}
+
+int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
+                           TaskPool* pool) {
   int p0 = -1;
   for (size_t i = 0; i < ps_.planes.size(); ++i)
     if (ps_.planes[i].frame == frame && ps_.planes[i].level == 0) {
@@ -192,6 +200,26 @@ int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStr
     }
   if (p0 < 0) return -1;
   const PlaneDesc& d = ps_.planes[p0];
+  if (pool) {
+    const size_t bytes = (size_t)d.w * d.h;
+    if (!staged_sent_) CCTAGX_HIP(hipEventCreateWithFlags(&staged_sent_, hipEventDisableTiming));
+    if (staged_in_flight_) CCTAGX_HIP(hipEventSynchronize(staged_sent_));
+    staged_in_flight_ = false;
+    if (staged_.ensure(bytes)) return -2;
+    uint8_t* dst = staged_.as<uint8_t>();
+    const int parts = std::min(d.h, 4 * pool->threads());
+    parallel_for(pool, parts, [&](int k) {
+      const int r0 = (int)((long long)d.h * k / parts), r1 = (int)((long long)d.h * (k + 1) / parts);
+      if (stride == (size_t)d.w)
+        std::memcpy(dst + (size_t)r0 * d.w, gray + (size_t)r0 * d.w, (size_t)(r1 - r0) * d.w);
+      else
+        for (int r = r0; r < r1; ++r) std::memcpy(dst + (size_t)r * d.w, gray + r * stride, d.w);
+    });
+    CCTAGX_HIP(hipMemcpyAsync(launch_.buf.img + d.pix, dst, bytes, hipMemcpyHostToDevice, s));
+    CCTAGX_HIP(hipEventRecord(staged_sent_, s));
+    staged_in_flight_ = true;
+    return 0;
+  }
   // rows packed: one contiguous copy (a 2-D copy from pageable memory goes row by row)
   if (stride == (size_t)d.w)
     CCTAGX_HIP(hipMemcpyAsync(launch_.buf.img + d.pix, gray, (size_t)d.w * d.h,

@@ -3,14 +3,24 @@
 
 #include "device.hpp"
 #include "frontend_launch.hpp"
+#include "host/pool.hpp"
 
 namespace cctagx {
 
 class FrontEndDevice {
  public:
+  FrontEndDevice() = default;
+  FrontEndDevice(const FrontEndDevice&) = delete;
+  FrontEndDevice& operator=(const FrontEndDevice&) = delete;
+  ~FrontEndDevice();
   int init(const PlaneSet& ps, const FrontEndParams& prm);
-  // Copies a row-major 8-bit frame into the level-0 plane of `frame`.
-  int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s);
+  // Copies a row-major 8-bit frame into the level-0 plane of `frame`.  With
+  // a pool, the pool's threads copy the rows into a pinned buffer of the
+  // library and one transfer takes it over (the runtime's own copy from
+  // pageable memory stages it through one thread); the buffer is reused
+  // once that transfer has ended.
+  int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
+             TaskPool* pool = nullptr);
   // The same from an 8-bit BGR (channels 3) or BGRA (4) frame, converted to
   // gray on the device; the first call grows a staging buffer to the frame.
   int upload_color(int frame, const uint8_t* bgr, size_t stride, int channels, hipStream_t s);
@@ -32,6 +42,9 @@ class FrontEndDevice {
   DeviceArena arena_;
   FrontEndLaunch launch_;
   DeviceBuffer color_;  // staged color frame
+  PinnedBuffer staged_;  // gray frame on its way to the device
+  hipEvent_t staged_sent_ = nullptr;  // the last transfer out of staged_
+  bool staged_in_flight_ = false;
 };
 
 }  // namespace cctagx
