@@ -208,7 +208,7 @@ int cctagx_batch_load(cctagx_batch* b, int frame, const uint8_t* gray, int width
     set_error("frame size differs from the one given at cctagx_batch_create");
     return CCTAGX_ERR_ARG;
   }
-  if (int rc = b->fe.upload(frame, gray, stride, b->stream)) return rc;
+  if (int rc = b->fe.upload(frame, gray, stride, b->stream, b->pool.get())) return rc;
   b->loaded[frame] = true;
   return CCTAGX_OK;
 }

@@ -184,10 +184,11 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
 }
 
 FrontEndDevice::~FrontEndDevice() {
-  if (staged_sent_) {
-    hipEventSynchronize(staged_sent_);
-    hipEventDestroy(staged_sent_);
-  }
+  for (hipEvent_t e : staged_sent_)
+    if (e) {
+      hipEventSynchronize(e);
+      hipEventDestroy(e);
+    }
 }
 
 int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
@@ -202,11 +203,27 @@ int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStr
   const PlaneDesc& d = ps_.planes[p0];
   if (pool) {
     const size_t bytes = (size_t)d.w * d.h;
-    if (!staged_sent_) CCTAGX_HIP(hipEventCreateWithFlags(&staged_sent_, hipEventDisableTiming));
-    if (staged_in_flight_) CCTAGX_HIP(hipEventSynchronize(staged_sent_));
-    staged_in_flight_ = false;
-    if (staged_.ensure(bytes)) return -2;
-    uint8_t* dst = staged_.as<uint8_t>();
+    if (staged_off_.empty()) {
+      // one region per frame, in frame order
+      int nf = 0;
+      for (const PlaneDesc& q : ps_.planes) nf = std::max(nf, q.frame + 1);
+      std::vector<size_t> size(nf, 0);
+      for (const PlaneDesc& q : ps_.planes)
+        if (q.level == 0) size[q.frame] = (size_t)q.w * q.h;
+      staged_off_.assign(nf + 1, 0);
+      for (int f = 0; f < nf; ++f) staged_off_[f + 1] = staged_off_[f] + ((size[f] + 255) & ~size_t(255));
+      if (staged_.ensure(std::max<size_t>(staged_off_[nf], 1))) {
+        staged_off_.clear();
+        return -2;
+      }
+      staged_sent_.assign(nf, nullptr);
+      staged_in_flight_.assign(nf, 0);
+    }
+    if (!staged_sent_[frame])
+      CCTAGX_HIP(hipEventCreateWithFlags(&staged_sent_[frame], hipEventDisableTiming));
+    if (staged_in_flight_[frame]) CCTAGX_HIP(hipEventSynchronize(staged_sent_[frame]));
+    staged_in_flight_[frame] = 0;
+    uint8_t* dst = staged_.as<uint8_t>() + staged_off_[frame];
     const int parts = std::min(d.h, 4 * pool->threads());
     parallel_for(pool, parts, [&](int k) {
       const int r0 = (int)((long long)d.h * k / parts), r1 = (int)((long long)d.h * (k + 1) / parts);
@@ -216,8 +233,8 @@ int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStr
         for (int r = r0; r < r1; ++r) std::memcpy(dst + (size_t)r * d.w, gray + r * stride, d.w);
     });
     CCTAGX_HIP(hipMemcpyAsync(launch_.buf.img + d.pix, dst, bytes, hipMemcpyHostToDevice, s));
-    CCTAGX_HIP(hipEventRecord(staged_sent_, s));
-    staged_in_flight_ = true;
+    CCTAGX_HIP(hipEventRecord(staged_sent_[frame], s));
+    staged_in_flight_[frame] = 1;
     return 0;
   }
   // rows packed: one contiguous copy (a 2-D copy from pageable memory goes row by row)

@@ -15,10 +15,10 @@ class FrontEndDevice {
   ~FrontEndDevice();
   int init(const PlaneSet& ps, const FrontEndParams& prm);
   // Copies a row-major 8-bit frame into the level-0 plane of `frame`.  With
-  // a pool, the pool's threads copy the rows into a pinned buffer of the
-  // library and one transfer takes it over (the runtime's own copy from
-  // pageable memory stages it through one thread); the buffer is reused
-  // once that transfer has ended.
+  // a pool, the pool's threads copy the rows into the frame's region of a
+  // pinned buffer of the library and one transfer takes it over (the
+  // runtime's own copy from pageable memory stages it through one thread);
+  // a region is reused once its last transfer has ended.
   int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
              TaskPool* pool = nullptr);
   // The same from an 8-bit BGR (channels 3) or BGRA (4) frame, converted to
@@ -42,9 +42,10 @@ class FrontEndDevice {
   DeviceArena arena_;
   FrontEndLaunch launch_;
   DeviceBuffer color_;  // staged color frame
-  PinnedBuffer staged_;  // gray frame on its way to the device
-  hipEvent_t staged_sent_ = nullptr;  // the last transfer out of staged_
-  bool staged_in_flight_ = false;
+  PinnedBuffer staged_;  // gray frames on their way to the device, one region per frame
+  std::vector<size_t> staged_off_;       // per frame: its region's offset
+  std::vector<hipEvent_t> staged_sent_;  // per frame: the last transfer out of its region
+  std::vector<uint8_t> staged_in_flight_;
 };
 
 }  // namespace cctagx
