@@ -200,7 +200,7 @@ int cctagx_load_bgr(cctagx_handle* h, const uint8_t* bgr, int width, int height,
   }
   // stride / channels >= width exactly when a row of width pixels fits
   if (int rc = check_frame(h, bgr, width, height, stride / channels)) return rc;
-  if (int rc = h->fe.upload_color(0, bgr, stride, channels, h->stream)) return rc;
+  if (int rc = h->fe.upload_color(0, bgr, stride, channels, h->stream, h->pool.get())) return rc;
   h->have_frame = true;
   h->have_front_end = false;
   return CCTAGX_OK;

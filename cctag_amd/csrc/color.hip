@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_bgr_to_gray(const uint8_t* __restrict__
 }  // namespace
 
 int FrontEndDevice::upload_color(int frame, const uint8_t* bgr, size_t stride, int channels,
-                                 hipStream_t s) {
+                                 hipStream_t s, TaskPool* pool) {
   if (channels != 3 && channels != 4) return -1;
   int p0 = -1;
   for (size_t i = 0; i < ps_.planes.size(); ++i)
@@ -49,7 +49,17 @@ int FrontEndDevice::upload_color(int frame, const uint8_t* bgr, size_t stride, i
   const size_t row = (size_t)d.w * channels;
   if (int rc = color_.ensure(row * d.h)) return rc;
   uint8_t* staged = color_.as<uint8_t>();
-  if (stride == row)
+  if (pool) {
+    if (!color_sent_) CCTAGX_HIP(hipEventCreateWithFlags(&color_sent_, hipEventDisableTiming));
+    if (color_in_flight_) CCTAGX_HIP(hipEventSynchronize(color_sent_));
+    color_in_flight_ = false;
+    if (color_pinned_.ensure(row * d.h)) return -2;
+    copy_rows(pool, color_pinned_.as<uint8_t>(), bgr, row, d.h, stride);
+    CCTAGX_HIP(hipMemcpyAsync(staged, color_pinned_.as<uint8_t>(), row * d.h,
+                              hipMemcpyHostToDevice, s));
+    CCTAGX_HIP(hipEventRecord(color_sent_, s));
+    color_in_flight_ = true;
+  } else if (stride == row)
     CCTAGX_HIP(hipMemcpyAsync(staged, bgr, row * d.h, hipMemcpyHostToDevice, s));
   else
     CCTAGX_HIP(hipMemcpy2DAsync(staged, row, bgr, stride, row, d.h, hipMemcpyHostToDevice, s));

@@ -183,12 +183,29 @@ int FrontEndDevice::init(const PlaneSet& ps, const FrontEndParams& prm) {
   return 0;
 }
 
+void copy_rows(TaskPool* pool, uint8_t* dst, const uint8_t* src, size_t row_bytes, int rows,
+               size_t stride) {
+  const int parts = std::max(1, std::min(rows, 4 * (pool ? pool->threads() : 1)));
+  parallel_for(pool, parts, [&](int k) {
+    const int r0 = (int)((long long)rows * k / parts), r1 = (int)((long long)rows * (k + 1) / parts);
+    if (stride == row_bytes)
+      std::memcpy(dst + (size_t)r0 * row_bytes, src + (size_t)r0 * row_bytes,
+                  (size_t)(r1 - r0) * row_bytes);
+    else
+      for (int r = r0; r < r1; ++r) std::memcpy(dst + (size_t)r * row_bytes, src + r * stride, row_bytes);
+  });
+}
+
 FrontEndDevice::~FrontEndDevice() {
   for (hipEvent_t e : staged_sent_)
     if (e) {
       hipEventSynchronize(e);
       hipEventDestroy(e);
     }
+  if (color_sent_) {
+    hipEventSynchronize(color_sent_);
+    hipEventDestroy(color_sent_);
+  }
 }
 
 int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
@@ -224,14 +241,7 @@ int FrontEndDevice::upload(int frame, const uint8_t* gray, size_t stride, hipStr
     if (staged_in_flight_[frame]) CCTAGX_HIP(hipEventSynchronize(staged_sent_[frame]));
     staged_in_flight_[frame] = 0;
     uint8_t* dst = staged_.as<uint8_t>() + staged_off_[frame];
-    const int parts = std::min(d.h, 4 * pool->threads());
-    parallel_for(pool, parts, [&](int k) {
-      const int r0 = (int)((long long)d.h * k / parts), r1 = (int)((long long)d.h * (k + 1) / parts);
-      if (stride == (size_t)d.w)
-        std::memcpy(dst + (size_t)r0 * d.w, gray + (size_t)r0 * d.w, (size_t)(r1 - r0) * d.w);
-      else
-        for (int r = r0; r < r1; ++r) std::memcpy(dst + (size_t)r * d.w, gray + r * stride, d.w);
-    });
+    copy_rows(pool, dst, gray, (size_t)d.w, d.h, stride);
     CCTAGX_HIP(hipMemcpyAsync(launch_.buf.img + d.pix, dst, bytes, hipMemcpyHostToDevice, s));
     CCTAGX_HIP(hipEventRecord(staged_sent_[frame], s));
     staged_in_flight_[frame] = 1;

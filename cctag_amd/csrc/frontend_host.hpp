@@ -7,6 +7,11 @@
 
 namespace cctagx {
 
+// Copies `rows` rows of `row_bytes` bytes (source rows `stride` apart) to
+// the packed destination, the rows shared among the pool's threads.
+void copy_rows(TaskPool* pool, uint8_t* dst, const uint8_t* src, size_t row_bytes, int rows,
+               size_t stride);
+
 class FrontEndDevice {
  public:
   FrontEndDevice() = default;
@@ -22,8 +27,10 @@ class FrontEndDevice {
   int upload(int frame, const uint8_t* gray, size_t stride, hipStream_t s,
              TaskPool* pool = nullptr);
   // The same from an 8-bit BGR (channels 3) or BGRA (4) frame, converted to
-  // gray on the device; the first call grows a staging buffer to the frame.
-  int upload_color(int frame, const uint8_t* bgr, size_t stride, int channels, hipStream_t s);
+  // gray on the device; the first call grows a staging buffer to the frame
+  // (with a pool, the rows go through a pinned buffer as for gray frames).
+  int upload_color(int frame, const uint8_t* bgr, size_t stride, int channels, hipStream_t s,
+                   TaskPool* pool = nullptr);
   // Enqueues the whole front end (no host synchronization).
   int run(hipStream_t s, StageTimer* timer);
 
@@ -42,6 +49,9 @@ class FrontEndDevice {
   DeviceArena arena_;
   FrontEndLaunch launch_;
   DeviceBuffer color_;  // staged color frame
+  PinnedBuffer color_pinned_;  // color frame on its way to color_
+  hipEvent_t color_sent_ = nullptr;
+  bool color_in_flight_ = false;
   PinnedBuffer staged_;  // gray frames on their way to the device, one region per frame
   std::vector<size_t> staged_off_;       // per frame: its region's offset
   std::vector<hipEvent_t> staged_sent_;  // per frame: the last transfer out of its region
